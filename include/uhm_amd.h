@@ -1,0 +1,181 @@
+/*
+ * uhm_amd.h -- drop-in C ABI of the MI355X multifrontal solver.
+ *
+ * The entry points below carry the names, argument lists and pass-by-pointer
+ * conventions of the reference's Fortran/C interface
+ * (UHM/include/uhm/wrapper/fort.hxx:38-190, implemented in
+ * UHM/src/wrapper/fort.cxx:47-321): handles are `long`, every argument is a
+ * pointer, names end with an underscore, functions return void.  A program
+ * that binds the reference's interface for the factorize / solve path links
+ * against libuhm_amd.so instead and needs no source change.
+ *
+ * Errors: the reference aborts (assert) or throws; here a failing call
+ * leaves its outputs untouched and records a message readable with
+ * uhm_get_last_error_ (extension).  A zero pivot or a non-positive Cholesky
+ * pivot is reported by uhm_get_info_ (extension) as element id + 1.
+ *
+ * Scope: the LU-with-partial-pivoting and Cholesky paths.  The reference's
+ * lu_nopiv / lu_incpiv / qr / svd entries are not part of this library.
+ */
+#ifndef UHM_AMD_H_
+#define UHM_AMD_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef long uhm_fort_p;          /* fort.hxx:38 */
+typedef char uhm_fort_char;
+typedef int uhm_fort_int;
+typedef unsigned int uhm_fort_uint;
+typedef double uhm_fort_double;
+
+/* Constants with the reference's values (UHM/include/uhm/const.hxx:70-81;
+ * UHM_REAL is libflame's FLA_DOUBLE code behind LINAL_REAL). */
+#define UHM_AMD_REAL 102
+#define UHM_AMD_CHOL 1
+#define UHM_AMD_LU_PIV 3
+#define UHM_AMD_LHS 1
+#define UHM_AMD_RHS 2
+#define UHM_AMD_PHYSICS_SINGLE 1
+#define UHM_AMD_PHYSICS_MULTI 2
+
+/* ---- global state and utilities (fort.hxx:47-74) ------------------------ */
+void uhm_initialize_fla_(void);                           /* fort.cxx:47  */
+void uhm_finalize_fla_(void);                             /* fort.cxx:51  */
+void uhm_set_num_threads_(uhm_fort_int* n_threads);       /* fort.cxx:55  */
+void uhm_get_num_threads_(uhm_fort_int* n_threads);       /* fort.cxx:59  */
+/* Panel width of the GPU factorization (the reference's hier block size,
+ * default 256 at UHM/src/matrix/uhm/matrix.cxx:39); values above 64 are
+ * clamped to 64, the widest panel the kernels take. */
+void uhm_set_hier_block_size_(uhm_fort_int* size);        /* fort.cxx:63  */
+void uhm_get_hier_block_size_(uhm_fort_int* size);        /* fort.cxx:66  */
+/* Algorithmic (LAWN-41) flops of the last factorization; the reference's
+ * counter is never incremented (UHM/src/matrix/uhm/common.cxx:44-50). */
+void uhm_reset_flop_(void);                               /* fort.cxx:69  */
+void uhm_get_flop_(uhm_fort_double* flop);                /* fort.cxx:72  */
+/* Device bytes held by the fronts (all fronts stay resident in HBM). */
+void uhm_get_buffer_used_(uhm_fort_double* buffer);       /* fort.cxx:75  */
+void uhm_get_max_buffer_used_(uhm_fort_double* buffer);   /* fort.cxx:78  */
+void uhm_is_multithreading_enable_(uhm_fort_int* flag);   /* fort.cxx:81  */
+void uhm_is_hier_matrix_enable_(uhm_fort_int* flag);      /* fort.cxx:84  */
+void uhm_timer_(uhm_fort_double* t);                      /* fort.cxx:87  */
+
+/* ---- mesh object (fort.hxx:78-145) --------------------------------------- */
+void uhm_mesh_create_(uhm_fort_p* mesh);                                    /* fort.cxx:95  */
+void uhm_mesh_delete_(uhm_fort_p* mesh);                                    /* fort.cxx:100 */
+void uhm_add_element_(uhm_fort_p* mesh, uhm_fort_p* elt);                   /* fort.cxx:105 */
+void uhm_add_node_(uhm_fort_p* mesh, uhm_fort_int* id, uhm_fort_int* phy, uhm_fort_int* n_dof,
+                   uhm_fort_int* p, uhm_fort_p* nod);                       /* fort.cxx:111 */
+void uhm_element_add_node_(uhm_fort_p* elt, uhm_fort_p* nod);               /* fort.cxx:123 */
+void uhm_copy_in_(uhm_fort_p* mesh, uhm_fort_p* elt, uhm_fort_int* datatype, uhm_fort_int* mm,
+                  uhm_fort_int* nn, uhm_fort_int* node_disp, uhm_fort_int* nods, uhm_fort_int* side,
+                  uhm_fort_double* buffer);                                 /* fort.cxx:130 */
+void uhm_copy_out_(uhm_fort_p* mesh, uhm_fort_p* elt, uhm_fort_int* datatype, uhm_fort_int* mm,
+                   uhm_fort_int* nn, uhm_fort_int* node_disp, uhm_fort_int* nods, uhm_fort_int* side,
+                   uhm_fort_double* buffer);                                /* fort.cxx:145 */
+void uhm_mesh_disp_(uhm_fort_p* mesh);                                      /* fort.cxx:160 */
+void uhm_get_n_dof_(uhm_fort_p* mesh, uhm_fort_int* n_dof);                 /* fort.cxx:165 */
+void uhm_get_n_nonzero_(uhm_fort_p* mesh, uhm_fort_int* n_nonzero);         /* fort.cxx:171 */
+void uhm_estimate_cost_(uhm_fort_p* mesh, uhm_fort_int* method, uhm_fort_int* datatype,
+                        uhm_fort_int* n_rhs, uhm_fort_double* flop_decompose,
+                        uhm_fort_double* flop_solve, uhm_fort_uint* n_nonzero_factor,
+                        uhm_fort_double* buffer);                           /* fort.cxx:177 */
+void uhm_mesh_import_(uhm_fort_p* mesh, uhm_fort_char* filename);           /* fort.cxx:191 */
+/* METIS-free tree: recursive bisection of the breadth-first element order
+ * (the reference's default builder needs METIS internals,
+ * UHM/src/operation/build_tree_var4.cxx:50-66). */
+void uhm_build_tree_(uhm_fort_p* mesh);                                     /* fort.cxx:197 */
+void uhm_create_matrix_without_buffer_(uhm_fort_p* mesh, uhm_fort_int* datatype,
+                                       uhm_fort_int* n_rhs);                /* fort.cxx:202 */
+void uhm_create_leaf_matrix_buffer_(uhm_fort_p* mesh);                      /* fort.cxx:211 */
+void uhm_create_element_matrix_buffer_(uhm_fort_p* mesh, uhm_fort_int* is_schur); /* fort.cxx:216 */
+void uhm_set_rhs_(uhm_fort_p* mesh);                                        /* fort.cxx:222 */
+/* Element matrices drawn on the device (see uhm_set_random_params_);
+ * SPD when beta = 0.  The reference's pair has the SPD flag inverted
+ * (UHM/src/mesh/matrix.cxx:155-156, 303-306); here both follow the
+ * configured parameters, random_spd_matrix forcing beta = 0. */
+void uhm_random_spd_matrix_(uhm_fort_p* mesh);                              /* fort.cxx:227 */
+void uhm_random_matrix_(uhm_fort_p* mesh);                                  /* fort.cxx:232 */
+/* No effect: the Cholesky kernels never read the upper triangle. */
+void uhm_triangularize_(uhm_fort_p* mesh);                                  /* fort.cxx:236 */
+void uhm_lock_(uhm_fort_p* mesh);                                           /* fort.cxx:241 */
+void uhm_unlock_(uhm_fort_p* mesh);                                         /* fort.cxx:245 */
+
+/* ---- decomposition / solution / check (fort.hxx:149-186) ---------------- */
+void uhm_chol_with_free_(uhm_fort_p* mesh);                                 /* fort.cxx:253 */
+void uhm_lu_piv_with_free_(uhm_fort_p* mesh);                               /* fort.cxx:261 */
+void uhm_solve_chol_(uhm_fort_p* mesh);                                     /* fort.cxx:277 */
+void uhm_solve_lu_piv_(uhm_fort_p* mesh);                                   /* fort.cxx:285 */
+/* True residual r = b - A x against the assembled matrix (the reference's
+ * hierarchical check multiplies through a no-op trmm,
+ * LINAL/src/dense/trmm.cxx:104-130). */
+void uhm_check_chol_(uhm_fort_p* mesh);                                     /* fort.cxx:297 */
+void uhm_check_lu_piv_(uhm_fort_p* mesh);                                   /* fort.cxx:305 */
+/* ||b - A x||_1 after a check (the reference sums ||rt_e||_1 over elements,
+ * UHM/src/mesh/matrix.cxx:241-275, which partitions the same rows). */
+void uhm_get_residual_(uhm_fort_p* mesh, uhm_fort_double* res);             /* fort.cxx:317 */
+
+/* ---- extensions (no counterpart in fort.hxx) ------------------------------ */
+/* Element_::set_parent / add_child (UHM/include/uhm/mesh/element.hxx:97,128),
+ * for callers that build the element tree themselves. */
+void uhm_element_set_parent_(uhm_fort_p* elt, uhm_fort_p* parent);
+void uhm_element_add_child_(uhm_fort_p* elt, uhm_fort_p* child);
+/* Synthetic mesh + geometric nested-dissection tree, in place of the
+ * ".uhm" files the reference reads (UHM/src/mesh/mesh.cxx:511-579).
+ * kind 1: 2D Q_p on dims[0] x dims[1]; 2: 3D Q_p on dims[0..2];
+ * 3: 2D variable order p_e in 1..p from seed. */
+void uhm_build_synthetic_mesh_(uhm_fort_p* mesh, uhm_fort_int* kind, uhm_fort_int* dims, uhm_fort_int* p,
+                               long long* seed);
+/* Parameters of uhm_random_*_matrix_: K_e = s_e (M^T M + alpha I + beta S)
+ * (alpha < 0 means |alpha| * element size), scale kind 0 none,
+ * 1 lognormal(0, p0), 2 10^U(p0, p1); b_e ~ N(0,1). */
+void uhm_set_random_params_(uhm_fort_p* mesh, long long* seed, uhm_fort_double* alpha,
+                            uhm_fort_double* beta, uhm_fort_int* scale_kind, uhm_fort_double* p0,
+                            uhm_fort_double* p1);
+/* 0, or element id + 1 of the first front with a zero / non-positive pivot
+ * in the last factorization. */
+void uhm_get_info_(uhm_fort_p* mesh, uhm_fort_int* info);
+/* Normwise backward error ||b - A x||inf / (||A||inf ||x||inf + ||b||inf)
+ * of the last check (worst right-hand side). */
+void uhm_get_backward_error_(uhm_fort_p* mesh, uhm_fort_double* eta);
+/* Milliseconds of the last factorization and solve (device events). */
+void uhm_get_timing_(uhm_fort_p* mesh, uhm_fort_double* factor_ms, uhm_fort_double* solve_ms);
+/* Last error message (empty when none); returns its length. */
+int uhm_get_last_error_(char* buf, int len);
+
+/* Introspection of the symbolic analysis and of device results (tests). */
+void uhm_get_n_fronts_(uhm_fort_p* mesh, uhm_fort_int* n);
+/* out[0..8] = element id, fs, ss, level, parent front, generation,
+ * factor nodes, schur nodes, is_leaf. */
+void uhm_get_front_info_(uhm_fort_p* mesh, uhm_fort_int* front, uhm_fort_int* out);
+/* which 0: factor list, 1: schur list; node ids and local offsets
+ * (Element_::factor / schur, UHM/include/uhm/mesh/element.hxx:61). */
+void uhm_get_front_nodes_(uhm_fort_p* mesh, uhm_fort_int* front, uhm_fort_int* which, uhm_fort_int* ids,
+                          uhm_fort_int* offs, uhm_fort_int* n);
+/* Extend-add runs of (parent, child front) as {offs_c, offs_p, fs_p, n_dof}
+ * (Helper_::set_mapper, UHM/include/uhm/matrix/uhm/helper.hxx:98-158). */
+void uhm_get_mapper_(uhm_fort_p* mesh, uhm_fort_int* child_front, uhm_fort_int* runs, uhm_fort_int* n);
+void uhm_get_node_offset_(uhm_fort_p* mesh, uhm_fort_int* id, uhm_fort_int* phy, uhm_fort_int* offset);
+void uhm_fetch_front_(uhm_fort_p* mesh, uhm_fort_int* front, uhm_fort_double* out);
+void uhm_fetch_pivots_(uhm_fort_p* mesh, uhm_fort_int* front, uhm_fort_int* out);
+void uhm_fetch_stage_(uhm_fort_p* mesh, uhm_fort_int* front, uhm_fort_double* out);
+/* right-hand side block b of a front (n x n_rhs, front order) */
+void uhm_fetch_rhs_front_(uhm_fort_p* mesh, uhm_fort_int* front, uhm_fort_double* out);
+void uhm_fetch_x_global_(uhm_fort_p* mesh, uhm_fort_double* out);
+void uhm_fetch_b_global_(uhm_fort_p* mesh, uhm_fort_double* out);
+/* Connectivity dump (the role of Mesh_::export_connectivity,
+ * UHM/include/uhm/mesh/mesh.hxx:93): nodes in creation order (id, n_dof,
+ * kind), element node lists as creation indices (eptr/enode), parent
+ * element index and children lists (cptr/cidx).  Call with null arrays to
+ * get the sizes (n_nodes, n_elem, len(enode), len(cidx)) in sizes[0..3]. */
+void uhm_export_connectivity_(uhm_fort_p* mesh, uhm_fort_int* sizes, uhm_fort_int* node_id, uhm_fort_int* ndof,
+                              uhm_fort_int* kind, uhm_fort_int* eptr, uhm_fort_int* enode, uhm_fort_int* parent,
+                              uhm_fort_int* cptr, uhm_fort_int* cidx);
+void uhm_get_csr_size_(uhm_fort_p* mesh, long long* nnz);
+void uhm_fetch_csr_(uhm_fort_p* mesh, long long* rowptr, uhm_fort_int* col, uhm_fort_double* val);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,146 @@
+/*
+ * uhmk.h -- kernel-level C ABI of the MI355X multifrontal path.
+ *
+ * Every entry point launches hand-written HIP kernels (gfx950) on the
+ * stream it is given and returns 0 on success or a negative HIP error code.
+ * All pointers are device pointers; all calls are stream ordered and made
+ * from one host thread.  These functions replace the per-front numeric seam
+ * of the reference, where each front operation went through libflame
+ * (LINAL/src/internal/gemm.cxx:78-106, LINAL/src/internal/trsm.cxx:70-97)
+ * one b x b tile at a time; here one launch covers every front of a tree
+ * level (a "batch"), bucketed by work item.
+ *
+ * Front layout (shared by every kernel): front f is one column-major
+ * n x n matrix, n = fs + ss, leading dimension n, stored at arena + off.
+ * Its quadrants are the reference's ATL (fs x fs), ABL (ss x fs),
+ * ATR (fs x ss) and ABR (ss x ss) (UHM/src/matrix/uhm/fla/matrix.cxx:75-91).
+ * Pivots are int32, libflame's relative format: p[k] = pivot row - k.
+ */
+#ifndef UHMK_H_
+#define UHMK_H_
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct uhmk_front {
+  long long off;   /* element offset of the front in its arena            */
+  long long xoff;  /* element offset of the n x n_rhs block in x / b      */
+  long long soff;  /* element offset of the assembly stage, -1 if none    */
+  int n;           /* fs + ss                                             */
+  int fs;          /* number of fully summed (factor) dofs                */
+  int poff;        /* offset of the fs pivots in the pivot arena          */
+  int parent;      /* parent front index, -1 for a root                   */
+  int emap;        /* offset of the ss-long map child schur -> parent idx */
+  int id;          /* element id, reported on a zero pivot / non-SPD      */
+} uhmk_front;
+
+/* Assembly source of a front: its own stage (src = -1) or the ABR block of
+ * child front src, gathered through inv[0..n) (parent index -> child schur
+ * index, -1 where the child does not contribute). */
+typedef struct uhmk_source {
+  int src;
+  int inv;
+} uhmk_source;
+
+/* Work item: four ints whose meaning is fixed per kernel. */
+typedef struct uhmk_item {
+  int f, a, b, c;
+} uhmk_item;
+
+/* ---- factorization (one launch per tree level and panel step) ---------- */
+
+/* Extend-add in gather form: every front in items is rebuilt as
+ * 0 + own stage + children's ABR (merge order).  Replaces op_merge /
+ * Helper_::merge_A / Matrix_FLA_::merge (UHM/src/operation/element.cxx:135-185,
+ * UHM/include/uhm/matrix/uhm/helper.hxx:191-222,
+ * UHM/src/matrix/uhm/fla/matrix.cxx:284-303).  items: {f, row tile, col tile}. */
+int uhmk_assemble(const uhmk_front* fronts, const uhmk_item* items, int n_items, const int* src_beg,
+                  const uhmk_source* srcs, const int* inv, double* arena, const double* stage,
+                  hipStream_t stream);
+
+/* Panel LU with partial pivoting restricted to ATL rows [c0, fs) for the
+ * fronts in flist; writes relative pivots.  Replaces FLA_LU_piv on the
+ * panel inside linal::dense::lu_piv (LINAL/src/dense/lu_piv.cxx:79).
+ * info[f] <- first zero-pivot row + 1. */
+int uhmk_lu_panel(const uhmk_front* fronts, const int* flist, int n_fronts, int max_rows, int c0, int nb,
+                  double* arena, int* piv, int* info, hipStream_t stream);
+
+/* Row swaps outside the panel (apply_pivots, LINAL/src/dense/pivot.cxx:46-91),
+ * U12 = L11^-1 A12 (trsm left lower unit) and L21 = A21 U11^-1 on the ABL
+ * rows (trsm right upper, UHM/src/matrix/uhm/fla/lu_piv/decompose.cxx:136-146).
+ * items: {f, kind (0 left cols, 1 ABL rows, 2 right cols), start, 0}. */
+int uhmk_lu_rows(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb,
+                 double* arena, const int* piv, hipStream_t stream);
+
+/* Cholesky of the diagonal panel block (FLA_Chol in linal::dense::chol,
+ * LINAL/src/dense/chol.cxx:67).  info[f] <- first non-positive pivot + 1. */
+int uhmk_chol_diag(const uhmk_front* fronts, const int* flist, int n_fronts, int c0, int nb, double* arena,
+                   int* info, hipStream_t stream);
+
+/* Panel rows below the diagonal block: X <- X L11^-T (trsm right lower
+ * transpose, LINAL/src/dense/chol.cxx:80-84).  items: {f, 0, first row, 0}. */
+int uhmk_chol_trsm(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb,
+                   double* arena, hipStream_t stream);
+
+/* C -= A op(B) on 64x64 tiles, FP64 MFMA: panel update (schur = 0, K =
+ * [c0, c0+nb)) or final Schur complement ABR -= ABL ATR / ABL ABL^T
+ * (schur = 1, K = [0, fs)).  Replaces linal::dense::gemm / syrk
+ * (LINAL/src/dense/gemm.cxx:48-137, syrk.cxx:43-123).  items: {f, i0, j0, 0}. */
+int uhmk_gemm_update(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb,
+                     int schur, int transb, double* arena, hipStream_t stream);
+
+/* ---- solve ---------------------------------------------------------------- */
+
+/* Forward solve of one rhs column for one tree level: gather the children's
+ * xb, pivots, L solve, xb -= ABL xt (op_solve level 1 + solve_*_1_hier,
+ * UHM/src/operation/element.cxx:243-294, lu_piv/solve.cxx:161-187,
+ * chol/solve.cxx:152-168). */
+int uhmk_solve_fwd(const uhmk_front* fronts, const int* flist, int n_fronts, int max_n, int chol,
+                   const int* src_beg, const uhmk_source* srcs, const int* inv, const double* arena,
+                   const int* piv, double* x, const double* b, int col, hipStream_t stream);
+
+/* Backward solve of one rhs column for one tree level: branch the parent's
+ * solution into xb, xt -= ATR xb (ABL^T xb), U (L^T) solve
+ * (lu_piv/solve.cxx:211-231, chol/solve.cxx:186-202, helper.hxx:302-331). */
+int uhmk_solve_bwd(const uhmk_front* fronts, const int* flist, int n_fronts, int max_n, int chol,
+                   const int* emap, const double* arena, double* x, int col, hipStream_t stream);
+
+/* ---- inputs and residuals -------------------------------------------------- */
+
+/* Synthetic element matrices K_e = s_e (M^T M + alpha I + beta S) and
+ * b_e ~ N(0,1) from a Philox-4x32-10 stream, written into the leaf stages.
+ * Stands in for Mesh_::random_matrix / random_spd_matrix
+ * (UHM/src/mesh/matrix.cxx:293-309).  items: {f, element id, lmap offset, m}. */
+int uhmk_generate_elements(const uhmk_front* fronts, const uhmk_item* items, int n_items, int max_m,
+                           const int* lmap, unsigned long long seed, double alpha, double beta,
+                           int scale_kind, double p0, double p1, double* stage, double* b, int nrhs,
+                           int with_lhs, int with_rhs, hipStream_t stream);
+
+/* CSR values of the assembled matrix from the leaf stages (DSparse_
+ * assembly, UHM/src/interf/sparse.cxx:196-267). */
+int uhmk_csr_values(long long nnz, const long long* cptr, const long long* cidx, const double* stage, double* val,
+                    hipStream_t stream);
+
+/* Assembled right-hand side column from the leaf right-hand sides. */
+int uhmk_gather_rhs(long long nrow, const long long* bptr, const long long* bidx, const int* bn, const double* b,
+                    double* bg, int col, hipStream_t stream);
+
+/* r = b - A x and out4 = {max|r|, max row-sum |A|, max|x|, max|b|} (out4
+ * zeroed by the caller); replaces check_* / get_residual
+ * (UHM/src/mesh/lu_piv.cxx:184-196, UHM/src/mesh/matrix.cxx:241-275). */
+int uhmk_residual(int nrow, const long long* rowptr, const int* colidx, const double* val, const double* xg,
+                  const double* bg, double* r, double* out4, hipStream_t stream);
+
+/* Copy between per-front solution blocks and a global vector (dir 0:
+ * fronts -> global, 1: global -> fronts) over the factor rows; the analogue
+ * of copy_out for the whole mesh (UHM/src/mesh/interf.cxx:183-199). */
+int uhmk_front_global(const uhmk_front* fronts, int n_fronts, const long long* goff, double* xf, double* g,
+                      int col, long long nglob, int dir, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

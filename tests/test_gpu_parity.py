@@ -1,0 +1,215 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle and the
+golden vectors.  Bars: integer / index work (pivots, assembly) bit-exact;
+FP64 factors within 1e-10 of the oracle's (max-norm relative to the front);
+solutions within 1e-12 relative of the oracle's and the golden dense solve;
+normwise backward error <= 1e-13 on the assembled matrix."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import uhm_amd as U
+from meshes import load_q2, qp_spec, spec_from_connectivity
+
+pytestmark = pytest.mark.gpu
+
+FACTOR_TOL = 1e-10
+X_TOL = 1e-12
+ETA_TOL = 1e-13
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _copy_in_golden(m, E, spec, z, N):
+    el, K, b = z["N%d_elems" % N], z["N%d_K" % N], z["N%d_b" % N]
+    for e in range(len(el)):
+        m.copy_in(E[e], el[e], U.UHM_LHS, K[e])
+        m.copy_in(E[e], el[e], U.UHM_RHS, b[e].reshape(-1, 1))
+
+
+def _oracle_from_product(m, method, b=64):
+    """Oracle on the same mesh and the same input bits (the device stages)."""
+    spec = spec_from_connectivity(m.export_connectivity())
+    om = spec.oracle()
+    om.init_numeric(m.n_rhs)
+    for f in range(m.n_fronts()):
+        info = m.front_info(f)
+        if info["leaf"]:
+            om.set_front(info["id"], m.fetch_stage(f))
+    return spec, om
+
+
+def _compare_fronts(m, om, method):
+    worst = 0.0
+    for f in range(m.n_fronts()):
+        info = m.front_info(f)
+        e, fs = info["id"], info["fs"]
+        G = m.fetch_front(f)
+        R = om.front(e)
+        if method == U.UHM_CHOL:
+            lo = np.tril(np.ones(G.shape, bool))
+            G, R = G[lo], R[lo]
+        else:
+            pg, pr = m.fetch_pivots(f), om.pivots(e)
+            assert np.array_equal(pg, pr), "pivot mismatch in front %d (element %d)" % (f, e)
+        if G.size:
+            err = np.abs(G - R).max() / max(1e-300, np.abs(R).max())
+            worst = max(worst, err)
+            assert err <= FACTOR_TOL, (f, e, err)
+    return worst
+
+
+def _solution_by_node(m, x):
+    spec = spec_from_connectivity(m.export_connectivity())
+    out = np.zeros(len(spec.nodes))
+    for nid, _ in spec.nodes:
+        out[nid] = x[m.node_offset(nid)]
+    return out
+
+
+@pytest.mark.parametrize("N,method", [(4, U.UHM_LU_PIV), (16, U.UHM_CHOL)])
+def test_golden_q2_meshes(lib, oracle_lib, N, method):
+    """Copy-in path on the golden Q2 meshes (16x16 is the 1089-dof CPU
+    configuration): solution against numpy.linalg.solve of the assembled
+    system, every front against the oracle."""
+    say("case golden q2 N=%d method=%d" % (N, method))
+    z = load_q2()
+    spec = qp_spec((N, N), 2)
+    m, E = spec.build_product(U)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    m.create_matrix_buffer()
+    _copy_in_golden(m, E, spec, z, N)
+    m.set_rhs()
+    assert (m.chol_with_free() if method == U.UHM_CHOL else m.lu_piv_with_free()) == 0
+    (m.solve_chol if method == U.UHM_CHOL else m.solve_lu_piv)()
+    (m.check_chol if method == U.UHM_CHOL else m.check_lu_piv)()
+    x = _solution_by_node(m, m.fetch_x()[:, 0])
+    ref = z["N%d_x" % N]
+    assert np.abs(x - ref).max() / np.abs(ref).max() <= X_TOL
+    assert m.backward_error() <= ETA_TOL
+    # copy_out returns the element's solution in the user's node order
+    xe = m.copy_out(E[0], spec.elems[0], U.UHM_RHS, len(spec.elems[0]), 1)[:, 0]
+    np.testing.assert_allclose(xe, ref[spec.elems[0]], rtol=0, atol=X_TOL * np.abs(ref).max())
+    # oracle on the same input
+    om = spec.oracle()
+    om.init_numeric(1)
+    el, K, b = z["N%d_elems" % N], z["N%d_K" % N], z["N%d_b" % N]
+    for e in range(len(el)):
+        om.copy_in(e, [spec.seq[n] for n in el[e]], 1, K[e])
+    assert om.factor(method, 64) == 0
+    _compare_fronts(m, om, method)
+
+
+CASES = [
+    # name, kind, dims, p, method, alpha, beta, scale_kind, p0, p1
+    ("q3-3d-lu", 2, (4, 4, 4), 3, U.UHM_LU_PIV, 64.0, 0.1, 1, 1.0, 0.0),
+    ("q4-3d-chol", 2, (4, 4, 3), 4, U.UHM_CHOL, 125.0, 0.0, 2, -2.0, 2.0),
+    ("q3-3d-chol-deep", 2, (6, 6, 6), 3, U.UHM_CHOL, 64.0, 0.0, 1, 0.5, 0.0),
+    ("q3-3d-lu-deep", 2, (6, 6, 5), 3, U.UHM_LU_PIV, 64.0, 0.1, 1, 1.0, 0.0),
+    ("varp-2d-lu", 3, (24, 24), 8, U.UHM_LU_PIV, -1.0, 0.1, 0, 0.0, 0.0),
+    ("q2-2d-chol", 1, (20, 12), 2, U.UHM_CHOL, 9.0, 0.0, 0, 0.0, 0.0),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_device_generated_vs_oracle(lib, oracle_lib, case):
+    name, kind, dims, p, method, alpha, beta, sk, p0, p1 = case
+    say("case", name)
+    m = U.Mesh()
+    m.build_synthetic(kind, dims, p, 12345)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    m.create_matrix_buffer()
+    m.set_random_params(seed=7, alpha=alpha, beta=beta, scale_kind=sk, p0=p0, p1=p1)
+    m.random_matrix()
+    m.set_rhs()
+    chol = method == U.UHM_CHOL
+    assert (m.chol_with_free() if chol else m.lu_piv_with_free()) == 0
+    (m.solve_chol if chol else m.solve_lu_piv)()
+    (m.check_chol if chol else m.check_lu_piv)()
+    eta = m.backward_error()
+    say("  eta", eta)
+    assert eta <= ETA_TOL
+    spec, om = _oracle_from_product(m, method)
+    for f in range(m.n_fronts()):
+        info = m.front_info(f)
+        if info["leaf"]:
+            om.set_rhs_front(info["id"], m.fetch_rhs_front(f))
+    assert om.factor(method, 64) == 0
+    worst = _compare_fronts(m, om, method)
+    say("  worst front rel err", worst)
+    om.solve(method)
+    xo = om.x_global(m.get_n_dof())[:, 0]
+    xg = m.fetch_x()[:, 0]
+    err = np.abs(xg - xo).max() / np.abs(xo).max()
+    say("  x rel err vs oracle", err)
+    assert err <= X_TOL
+
+
+def test_solution_matches_oracle_device_rhs(lib, oracle_lib):
+    """Same input bits on both sides including the right-hand side: the GPU
+    solution equals the oracle's to 1e-12 relative."""
+    m = U.Mesh()
+    m.build_synthetic(2, (5, 4, 4), 3, 0)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    m.create_matrix_buffer()
+    m.set_random_params(seed=3, alpha=64.0, beta=0.1, scale_kind=1, p0=1.0)
+    m.random_matrix()
+    m.set_rhs()
+    assert m.lu_piv_with_free() == 0
+    m.solve_lu_piv()
+    xg = m.fetch_x()[:, 0]
+    spec, om = _oracle_from_product(m, U.UHM_LU_PIV)
+    for f in range(m.n_fronts()):
+        info = m.front_info(f)
+        if info["leaf"]:
+            om.set_rhs_front(info["id"], m.fetch_rhs_front(f))
+    assert om.factor(U.UHM_LU_PIV, 64) == 0
+    om.solve(U.UHM_LU_PIV)
+    xo = om.x_global(len(xg))[:, 0]
+    err = np.abs(xg - xo).max() / np.abs(xo).max()
+    say("  x rel err vs oracle", err)
+    assert err <= X_TOL
+    # a second solve on the same factors reproduces the bits
+    m.solve_lu_piv()
+    assert np.array_equal(m.fetch_x()[:, 0], xg)
+
+
+def test_refactorization_reproducible(lib):
+    """A second factorization of the same values gives the same bits."""
+    m = U.Mesh()
+    m.build_synthetic(2, (4, 4, 4), 3, 0)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    m.create_matrix_buffer()
+    m.set_random_params(seed=11, alpha=64.0, beta=0.1, scale_kind=1, p0=1.0)
+    m.random_matrix()
+    m.set_rhs()
+    assert m.lu_piv_with_free() == 0
+    F1 = [m.fetch_front(f) for f in range(m.n_fronts())]
+    assert m.lu_piv_with_free() == 0
+    for f in range(m.n_fronts()):
+        assert np.array_equal(F1[f], m.fetch_front(f))
+
+
+def test_zero_pivot_reports_element(lib):
+    """A singular leaf ATL is reported (element id + 1), not aborted."""
+    spec = qp_spec((2, 2), 1)
+    m, E = spec.build_product(U)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    m.create_matrix_buffer()
+    rng = np.random.default_rng(0)
+    for e in range(4):
+        K = rng.standard_normal((4, 4)) + 4 * np.eye(4)
+        if e == 0:
+            K[:] = 0.0
+        m.copy_in(E[e], spec.elems[e], U.UHM_LHS, K)
+    info = m.lu_piv_with_free()
+    assert info >= 1

@@ -1,0 +1,161 @@
+"""Host-side logic of the product against the oracle (CPU only): the lock
+analysis, numbering, mapper runs and front sizes are integer parity targets
+and must match bit for bit.  Also checks that the C ABI library loads and
+exports every symbol the headers declare, and that the numeric path refuses
+to run without a GPU (no CPU fallback)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import uhm_amd as U
+from meshes import ex1_spec, qp_spec, spec_from_connectivity
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(uhmk?_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol(lib):
+    names = _declared("uhm_amd.h") + _declared("uhmk.h")
+    assert len(names) > 60
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the ctypes table covers the whole uhm_amd.h surface
+    assert set(_declared("uhm_amd.h")) <= set(U.exported_symbols())
+
+
+def _compare(m, spec, om):
+    nf = m.n_fronts()
+    assert nf == spec.n_elem
+    seen = set()
+    for f in range(nf):
+        info = m.front_info(f)
+        e = info["id"]
+        seen.add(e)
+        el = om.element(e)
+        assert (info["fs"], info["ss"], info["generation"]) == (el["fs"], el["ss"], el["generation"]), e
+        ids, offs = m.front_nodes(f, 0)
+        assert ids.tolist() == [spec.nodes[k][0] for k in el["factor"]]
+        assert offs.tolist() == el["factor_off"].tolist()
+        ids, offs = m.front_nodes(f, 1)
+        assert ids.tolist() == [spec.nodes[k][0] for k in el["schur"]]
+        assert offs.tolist() == el["schur_off"].tolist()
+        assert m.mapper(f).tolist() == om.mapper(e).tolist()
+    assert seen == set(range(spec.n_elem))
+    for k, (nid, _) in enumerate(spec.nodes):
+        assert m.node_offset(nid) == om.node_offset(k)
+    # schedule: levels leaves first, element id order inside a level
+    lv = [m.front_info(f)["level"] for f in range(nf)]
+    assert lv == sorted(lv)
+    oracle_order = [int(e) for L in om.levels() for e in L]
+    assert [m.front_info(f)["id"] for f in range(nf)] == oracle_order
+
+
+def test_lock_ex1_matches_hand_derivation_and_oracle(lib, oracle_lib):
+    spec, g = ex1_spec()
+    m, E = spec.build_product(U)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    assert m.get_n_dof() == g["ndof"]
+    for f in range(3):
+        info = m.front_info(f)
+        e = info["id"]
+        assert info["fs"] == g["fs"][e] and info["ss"] == g["ss"][e]
+        ids, offs = m.front_nodes(f, 0)
+        assert [[int(a), int(b)] for a, b in zip(ids, offs)] == g["factor"][e]
+        assert m.mapper(f).tolist() == g["mapper"][e]
+    for nid, o in g["offset"].items():
+        assert m.node_offset(int(nid)) == o
+    _compare(m, spec, spec.oracle())
+
+
+@pytest.mark.parametrize("dims,p", [((4, 4), 2), ((16, 16), 2), ((3, 2, 2), 2), ((4, 4, 4), 3), ((7, 5), 3)])
+def test_lock_api_mesh_matches_oracle(lib, oracle_lib, dims, p):
+    spec = qp_spec(dims, p)
+    m, _ = spec.build_product(U)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    _compare(m, spec, spec.oracle())
+
+
+@pytest.mark.parametrize("kind,dims,p,seed", [(1, (16, 16), 2, 0), (2, (6, 5, 4), 3, 0), (3, (24, 20), 8, 12345),
+                                              (3, (9, 13), 4, 5)])
+def test_lock_synthetic_mesh_matches_oracle(lib, oracle_lib, kind, dims, p, seed):
+    m = U.Mesh()
+    m.build_synthetic(kind, dims, p, seed)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    spec = spec_from_connectivity(m.export_connectivity())
+    if kind != 3:
+        ref = qp_spec(dims, p)
+        assert spec.elems == ref.elems and spec.parent == ref.parent and spec.children == ref.children
+    _compare(m, spec, spec.oracle())
+
+
+@pytest.mark.parametrize("kind,dims,p,ndof,root", [(1, (16, 16), 2, 1089, 33), (2, (16, 16, 16), 3, 117649, 2401),
+                                                   (2, (16, 16, 16), 4, 274625, 4225),
+                                                   (2, (24, 24, 24), 3, 389017, 5329)])
+def test_config_sizes(lib, kind, dims, p, ndof, root):
+    """dof counts and root separators of the benchmark configurations."""
+    m = U.Mesh()
+    m.build_synthetic(kind, dims, p, 0)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    assert m.get_n_dof() == ndof
+    roots = [f for f in range(m.n_fronts()) if m.front_info(f)["parent"] < 0]
+    assert len(roots) == 1 and m.front_info(roots[0])["fs"] == root and m.front_info(roots[0])["ss"] == 0
+
+
+def test_flops_match_reference_formula(lib):
+    m = U.Mesh()
+    m.build_synthetic(2, (16, 16, 16), 3, 0)
+    m.lock()
+    fd, _, _, _ = m.estimate_cost(U.UHM_LU_PIV)
+    assert abs(fd / 1e9 - 177.777) < 1e-3
+    m = U.Mesh()
+    m.build_synthetic(2, (16, 16, 16), 4, 0)
+    m.lock()
+    fd, _, _, _ = m.estimate_cost(U.UHM_CHOL)
+    assert abs(fd / 1e9 - 494.846) < 1e-3
+
+
+def test_copy_in_checks_dimensions(lib):
+    spec = qp_spec((2, 2), 1)
+    m, E = spec.build_product(U)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    with pytest.raises(U.UHMError, match="m does not match"):
+        m.copy_in(E[0], spec.elems[0], U.UHM_LHS, np.eye(3))
+    with pytest.raises(U.UHMError, match="cannot find node"):
+        m.copy_in(E[0], [999, 0, 1, 2], U.UHM_LHS, np.eye(4))
+
+
+def test_lock_requires_tree_and_rejects_late_nodes(lib):
+    spec = qp_spec((2, 1), 1)
+    m, E = spec.build_product(U)
+    m.lock()
+    with pytest.raises(U.UHMError, match="locked"):
+        m.add_node(10_000, 1)
+
+
+def test_numeric_path_fails_loudly_without_gpu(lib):
+    import ctypes
+    try:
+        n = ctypes.c_int(0)
+        ok = ctypes.CDLL("libamdhip64.so").hipGetDeviceCount(ctypes.byref(n)) == 0 and n.value > 0
+    except OSError:
+        ok = False
+    if ok:
+        pytest.skip("a GPU is present")
+    spec = qp_spec((2, 2), 1)
+    m, E = spec.build_product(U)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    with pytest.raises(U.UHMError, match="HIP device|GPU"):
+        m.lu_piv_with_free()

@@ -1,0 +1,712 @@
+// Front factorization kernels for gfx950 (FP64 path).
+//
+// One launch covers every front of a tree level.  Work items (uhmk_item)
+// are built once per sparsity pattern on the host, so the whole level
+// schedule is a fixed sequence of launches (and can be captured in a graph).
+//
+// Reference operations restated (semantics, not code):
+//   extend-add   Helper_::_merge_A + Matrix_FLA_::merge
+//                UHM/include/uhm/matrix/uhm/helper.hxx:191-222,
+//                UHM/src/matrix/uhm/fla/matrix.cxx:284-303
+//   LU front     lu_piv_hier, UHM/src/matrix/uhm/fla/lu_piv/decompose.cxx:124-153
+//                linal::dense::lu_piv, LINAL/src/dense/lu_piv.cxx:47-111
+//                apply_pivots, LINAL/src/dense/pivot.cxx:46-91
+//   Chol front   chol_hier, UHM/src/matrix/uhm/fla/chol/decompose.cxx:85-98
+//                linal::dense::chol, LINAL/src/dense/chol.cxx:46-92
+//   trsm / gemm / syrk  LINAL/src/dense/{trsm,gemm,syrk}.cxx
+#include <hip/hip_runtime.h>
+
+#include "uhmk.h"
+
+namespace {
+
+constexpr int TILE = 64;
+
+// ------------------------------------------------------------------------
+// Extend-add in gather form.  Every parent entry is produced exactly once:
+//   F[i,j] = ((0 + stage[i,j]) + C1[inv1[i],inv1[j]]) + C2[...] ...
+// summing the sources in the reference's merge order (own copy-in first,
+// then children in insertion order) into a zero start, which is the exact
+// operation sequence of op_merge on a freshly zeroed front
+// (UHM/src/operation/element.cxx:135-185).  Item: f, a = row tile start,
+// b = column tile start.
+// ------------------------------------------------------------------------
+constexpr int MAX_SRC = 16;
+
+__global__ __launch_bounds__(256) void k_assemble(const uhmk_front* __restrict__ F,
+                                                  const uhmk_item* __restrict__ items,
+                                                  const int* __restrict__ src_beg,
+                                                  const uhmk_source* __restrict__ srcs,
+                                                  const int* __restrict__ inv,
+                                                  double* __restrict__ arena,
+                                                  const double* __restrict__ stage) {
+  const uhmk_item it = items[blockIdx.x];
+  const int f = it.f;
+  __shared__ long long s_off[MAX_SRC];
+  __shared__ int s_ld[MAX_SRC], s_inv[MAX_SRC], s_kind[MAX_SRC];
+  const int s0 = src_beg[f], ns = min(src_beg[f + 1] - s0, MAX_SRC);
+  const long long off = F[f].off;
+  const int n = F[f].n;
+  if (threadIdx.x < ns) {
+    const uhmk_source sc = srcs[s0 + threadIdx.x];
+    if (sc.src < 0) {
+      s_kind[threadIdx.x] = 0;
+      s_off[threadIdx.x] = F[f].soff;
+      s_ld[threadIdx.x] = n;
+      s_inv[threadIdx.x] = -1;
+    } else {
+      const uhmk_front ch = F[sc.src];
+      s_kind[threadIdx.x] = 1;
+      s_off[threadIdx.x] = ch.off + ch.fs + (long long)ch.fs * ch.n;
+      s_ld[threadIdx.x] = ch.n;
+      s_inv[threadIdx.x] = sc.inv;
+    }
+  }
+  __syncthreads();
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int row = it.a + tx;
+  if (row >= n) return;
+  int rmap[MAX_SRC];
+#pragma unroll
+  for (int s = 0; s < MAX_SRC; ++s)
+    if (s < ns) rmap[s] = s_kind[s] ? inv[s_inv[s] + row] : row;
+  double* A = arena + off;
+  for (int c = ty; c < TILE; c += 4) {
+    const int col = it.b + c;
+    if (col >= n) break;
+    double acc = 0.0;
+#pragma unroll
+    for (int s = 0; s < MAX_SRC; ++s) {
+      if (s < ns) {
+        const int cj = s_kind[s] ? inv[s_inv[s] + col] : col;
+        const int ri = rmap[s];
+        if (ri >= 0 && cj >= 0) {
+          const double* src = s_kind[s] ? arena : stage;
+          acc += src[s_off[s] + ri + (long long)cj * s_ld[s]];
+        }
+      }
+    }
+    A[row + (long long)col * n] = acc;
+  }
+}
+
+// ------------------------------------------------------------------------
+// Partial LU panel on rows [c0, fs) x columns [c0, c0+kb) of each front,
+// pivots searched in rows [c0, fs) only (the reference restricts pivoting to
+// ATL).  One workgroup (512 threads) per front; the panel is processed in
+// inner blocks of IB columns held in registers, R rows per thread.  Pivot
+// choice is the first row of maximal |a| (libflame / LAPACK i?amax), found
+// with a wavefront shuffle reduction and a cross-wave pass in LDS.
+// Swaps are applied to the whole outer panel here; the row kernel applies
+// them to every other column.
+// ------------------------------------------------------------------------
+constexpr int PNT = 512, PIB = 8, PNW = PNT / 64;
+
+template <int R>
+__global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__ F,
+                                                  const int* __restrict__ flist, int c0, int nb,
+                                                  double* __restrict__ arena, int* __restrict__ piv,
+                                                  int* __restrict__ info) {
+  const int f = flist[blockIdx.x];
+  const uhmk_front fr = F[f];
+  const int n = fr.n, fs = fr.fs;
+  const int kb = min(nb, fs - c0);
+  double* __restrict__ A = arena + fr.off;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+
+  __shared__ double s_redv[PNW];
+  __shared__ int s_redi[PNW];
+  __shared__ int s_pi;
+  __shared__ double s_rowA[PIB], s_rowB[PIB];
+  __shared__ int s_piv[TILE];
+  __shared__ double s_L[PIB][PIB];
+  __shared__ double s_U[PIB][TILE];
+  __shared__ int s_prow[2 * PIB], s_psrc[2 * PIB];
+  __shared__ int s_pcnt;
+  __shared__ double s_tmp[2 * PIB][TILE];
+
+  for (int ib = 0; ib < kb; ib += PIB) {
+    const int ibw = min(PIB, kb - ib);
+    const int rb = c0 + ib;
+    double v[R][PIB];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int r = rb + tid + PNT * q;
+#pragma unroll
+      for (int j = 0; j < PIB; ++j) v[q][j] = (r < fs && j < ibw) ? A[r + (long long)(rb + j) * n] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < PIB; ++j) {
+      if (j < ibw) {
+        const int cr = rb + j;
+        double bv = -1.0;
+        int bi = 0x7fffffff;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const int r = rb + tid + PNT * q;
+          if (r >= cr && r < fs) {
+            const double a = fabs(v[q][j]);
+            if (a > bv) {
+              bv = a;
+              bi = r;
+            }
+          }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          const double ov = __shfl_xor(bv, o);
+          const int oi = __shfl_xor(bi, o);
+          if (ov > bv || (ov == bv && oi < bi)) {
+            bv = ov;
+            bi = oi;
+          }
+        }
+        if (lane == 0) {
+          s_redv[wid] = bv;
+          s_redi[wid] = bi;
+        }
+        __syncthreads();
+        if (tid == 0) {
+          double b2 = s_redv[0];
+          int i2 = s_redi[0];
+          for (int w = 1; w < PNW; ++w)
+            if (s_redv[w] > b2 || (s_redv[w] == b2 && s_redi[w] < i2)) {
+              b2 = s_redv[w];
+              i2 = s_redi[w];
+            }
+          // A column of NaNs has no maximum: keep the diagonal row.
+          if (i2 < cr || i2 >= fs) i2 = cr;
+          s_pi = i2;
+        }
+        __syncthreads();
+        const int pi = s_pi;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const int r = rb + tid + PNT * q;
+          if (r == pi)
+#pragma unroll
+            for (int jj = 0; jj < PIB; ++jj) s_rowA[jj] = v[q][jj];
+          if (r == cr)
+#pragma unroll
+            for (int jj = 0; jj < PIB; ++jj) s_rowB[jj] = v[q][jj];
+        }
+        __syncthreads();
+        if (pi != cr) {
+#pragma unroll
+          for (int q = 0; q < R; ++q) {
+            const int r = rb + tid + PNT * q;
+            if (r == pi)
+#pragma unroll
+              for (int jj = 0; jj < PIB; ++jj) v[q][jj] = s_rowB[jj];
+            if (r == cr)
+#pragma unroll
+              for (int jj = 0; jj < PIB; ++jj) v[q][jj] = s_rowA[jj];
+          }
+        }
+        const double pv = s_rowA[j];
+        if (tid == 0) {
+          s_piv[ib + j] = pi - cr;
+          if (!(pv != 0.0) && info[f] == 0) info[f] = cr + 1;
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const int r = rb + tid + PNT * q;
+          if (r > cr && r < fs) {
+            double l = v[q][j];
+            if (pv != 0.0) l = l / pv;
+            v[q][j] = l;
+#pragma unroll
+            for (int kk = j + 1; kk < PIB; ++kk) v[q][kk] -= l * s_rowA[kk];
+          }
+        }
+      }
+    }
+    // store the factored inner block
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int r = rb + tid + PNT * q;
+      if (r < fs)
+#pragma unroll
+        for (int jj = 0; jj < PIB; ++jj)
+          if (jj < ibw) A[r + (long long)(rb + jj) * n] = v[q][jj];
+    }
+    __syncthreads();
+    // Net row permutation of this inner block's swaps (rows listed once;
+    // s_psrc[k] = list index of the row whose old value lands on s_prow[k]).
+    if (tid == 0) {
+      int cnt = 0;
+      int content[2 * PIB];
+      for (int s = 0; s < ibw; ++s) {
+        const int ra = rb + s, rb2 = rb + s + s_piv[ib + s];
+        int ia = -1, ib2 = -1;
+        for (int k = 0; k < cnt; ++k) {
+          if (s_prow[k] == ra) ia = k;
+          if (s_prow[k] == rb2) ib2 = k;
+        }
+        if (ia < 0) {
+          s_prow[cnt] = ra;
+          content[cnt] = ra;
+          ia = cnt++;
+        }
+        if (ib2 < 0) {
+          s_prow[cnt] = rb2;
+          content[cnt] = rb2;
+          ib2 = cnt++;
+        }
+        const int t = content[ia];
+        content[ia] = content[ib2];
+        content[ib2] = t;
+      }
+      for (int k = 0; k < cnt; ++k) {
+        int src = k;
+        for (int k2 = 0; k2 < cnt; ++k2)
+          if (s_prow[k2] == content[k]) src = k2;
+        s_psrc[k] = src;
+      }
+      s_pcnt = cnt;
+    }
+    __syncthreads();
+    const int pc = s_pcnt;
+    const int nother = kb - ibw;
+    for (int idx = tid; idx < nother * pc; idx += PNT) {
+      const int t = idx / pc, k = idx - t * pc;
+      const int col = (t < ib) ? (c0 + t) : (c0 + t + ibw);
+      s_tmp[k][t] = A[s_prow[k] + (long long)col * n];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < nother * pc; idx += PNT) {
+      const int t = idx / pc, k = idx - t * pc;
+      const int col = (t < ib) ? (c0 + t) : (c0 + t + ibw);
+      A[s_prow[k] + (long long)col * n] = s_tmp[s_psrc[k]][t];
+    }
+    const int nr = kb - ib - ibw;
+    if (tid < PIB) {
+#pragma unroll
+      for (int jj = 0; jj < PIB; ++jj) s_L[tid][jj] = (tid < ibw) ? v[0][jj] : 0.0;
+    }
+    __syncthreads();
+    if (nr > 0) {
+      // U strip of the columns right of this inner block (inside the panel)
+      for (int t = tid; t < nr; t += PNT) {
+        const int col = rb + ibw + t;
+        double u[PIB];
+#pragma unroll
+        for (int jj = 0; jj < PIB; ++jj) u[jj] = (jj < ibw) ? A[rb + jj + (long long)col * n] : 0.0;
+#pragma unroll
+        for (int jj = 1; jj < PIB; ++jj)
+#pragma unroll
+          for (int kk = 0; kk < jj; ++kk) u[jj] -= s_L[jj][kk] * u[kk];
+#pragma unroll
+        for (int jj = 0; jj < PIB; ++jj) {
+          if (jj < ibw) A[rb + jj + (long long)col * n] = u[jj];
+          s_U[jj][t] = (jj < ibw) ? u[jj] : 0.0;
+        }
+      }
+      __syncthreads();
+      // rank-ibw update of the rows below for those columns
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int r = rb + tid + PNT * q;
+        if (r >= rb + ibw && r < fs) {
+          for (int t0 = 0; t0 < nr; t0 += 8) {
+            double a[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+              a[u] = (t0 + u < nr) ? A[r + (long long)(rb + ibw + t0 + u) * n] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+              for (int kk = 0; kk < PIB; ++kk) a[u] -= v[q][kk] * s_U[kk][(t0 + u) & (TILE - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+              if (t0 + u < nr) A[r + (long long)(rb + ibw + t0 + u) * n] = a[u];
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int k = tid; k < kb; k += PNT) piv[fr.poff + c0 + k] = s_piv[k];
+}
+
+// ------------------------------------------------------------------------
+// After a panel: row swaps for every column outside the panel, the U strip
+// U12 = L11^-1 * A12 for columns right of the panel (unit lower TRSM), and
+// the ABL rows of the panel L21 = A21 * U11^-1 (rows [fs, n), right upper
+// TRSM).  Item: f, a = kind (0 left columns, 1 ABL rows, 2 right columns),
+// b = first column / row of the 64-wide chunk.
+// ------------------------------------------------------------------------
+constexpr int LDP = TILE + 1;
+
+__device__ inline void panel_perm(const int* s_p, int kb, int c0, int fs, int* s_prow, int* s_psrc,
+                                  int* s_cnt) {
+  // The kb strip rows first (in order), then the distinct swap targets
+  // below the strip.  content[k] = original row now at s_prow[k].
+  int cnt = kb;
+  int content[2 * TILE];
+  for (int k = 0; k < kb; ++k) {
+    s_prow[k] = c0 + k;
+    content[k] = c0 + k;
+  }
+  for (int s = 0; s < kb; ++s) {
+    const int ra = c0 + s, rt = c0 + s + s_p[s];
+    int it = -1;
+    if (rt < c0 + kb) it = rt - c0;
+    else
+      for (int k = kb; k < cnt; ++k)
+        if (s_prow[k] == rt) it = k;
+    if (it < 0) {
+      s_prow[cnt] = rt;
+      content[cnt] = rt;
+      it = cnt++;
+    }
+    const int ia = ra - c0;
+    const int t = content[ia];
+    content[ia] = content[it];
+    content[it] = t;
+  }
+  for (int k = 0; k < cnt; ++k) {
+    const int r = content[k];
+    int src;
+    if (r < c0 + kb) src = r - c0;
+    else {
+      src = k;
+      for (int k2 = kb; k2 < cnt; ++k2)
+        if (s_prow[k2] == r) src = k2;
+    }
+    s_psrc[k] = src;
+  }
+  *s_cnt = cnt;
+}
+
+__global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ F,
+                                                 const uhmk_item* __restrict__ items, int c0, int nb,
+                                                 double* __restrict__ arena, const int* __restrict__ piv) {
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs;
+  const int kb = min(nb, fs - c0);
+  double* __restrict__ A = arena + fr.off;
+  const int tid = threadIdx.x;
+  __shared__ double s_X[2 * TILE][TILE];  // permuted rows / strip
+  __shared__ double s_T[TILE][LDP];       // triangular block
+  __shared__ int s_p[TILE], s_prow[2 * TILE], s_psrc[2 * TILE];
+  __shared__ int s_cnt;
+
+  if (it.a == 1) {
+    // ABL rows [start, start+h): X <- X * U11^-1
+    const int i0 = it.b, h = min(TILE, n - i0);
+    double(*sx)[LDP] = reinterpret_cast<double(*)[LDP]>(&s_X[0][0]);
+    for (int idx = tid; idx < kb * kb; idx += 256) {
+      const int i = idx % kb, j = idx / kb;
+      s_T[i][j] = A[(c0 + i) + (long long)(c0 + j) * n];
+    }
+    for (int idx = tid; idx < h * kb; idx += 256) {
+      const int r = idx % h, k = idx / h;
+      sx[r][k] = A[(i0 + r) + (long long)(c0 + k) * n];
+    }
+    __syncthreads();
+    if (tid < h) {
+      for (int j = 0; j < kb; ++j) {
+        double x = sx[tid][j];
+        for (int k = 0; k < j; ++k) x -= sx[tid][k] * s_T[k][j];
+        sx[tid][j] = x / s_T[j][j];
+      }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < h * kb; idx += 256) {
+      const int r = idx % h, k = idx / h;
+      A[(i0 + r) + (long long)(c0 + k) * n] = sx[r][k];
+    }
+    return;
+  }
+
+  const int j0 = it.b;
+  const int width = min(TILE, (it.a == 0 ? c0 : n) - j0);
+  if (tid < kb) s_p[tid] = piv[fr.poff + c0 + tid];
+  __syncthreads();
+  if (tid == 0) panel_perm(s_p, kb, c0, fs, s_prow, s_psrc, &s_cnt);
+  __syncthreads();
+  const int pc = s_cnt;
+  for (int idx = tid; idx < width * pc; idx += 256) {
+    const int k = idx % pc, t = idx / pc;
+    s_X[k][t] = A[s_prow[k] + (long long)(j0 + t) * n];
+  }
+  __syncthreads();
+  if (it.a == 0) {
+    for (int idx = tid; idx < width * pc; idx += 256) {
+      const int k = idx % pc, t = idx / pc;
+      A[s_prow[k] + (long long)(j0 + t) * n] = s_X[s_psrc[k]][t];
+    }
+    return;
+  }
+  // right columns: rows moved out of the strip go back now; the strip is
+  // solved in LDS first.
+  for (int idx = tid; idx < width * (pc - kb); idx += 256) {
+    const int k = kb + idx % (pc - kb), t = idx / (pc - kb);
+    A[s_prow[k] + (long long)(j0 + t) * n] = s_X[s_psrc[k]][t];
+  }
+  for (int idx = tid; idx < kb * kb; idx += 256) {
+    const int i = idx % kb, j = idx / kb;
+    s_T[i][j] = A[(c0 + i) + (long long)(c0 + j) * n];
+  }
+  // Gather the permuted strip into rows [0, kb) of s_X.  Sources may sit
+  // anywhere in [0, pc), so every value is read before any is written.
+  {
+    double g[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int idx = tid + 256 * u;
+      if (idx < width * kb) {
+        const int k = idx % kb, t = idx / kb;
+        g[u] = s_X[s_psrc[k]][t];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int idx = tid + 256 * u;
+      if (idx < width * kb) {
+        const int k = idx % kb, t = idx / kb;
+        s_X[k][t] = g[u];
+      }
+    }
+  }
+  __syncthreads();
+  if (tid < width) {
+    for (int i = 1; i < kb; ++i) {
+      double x = s_X[i][tid];
+      for (int k = 0; k < i; ++k) x -= s_T[i][k] * s_X[k][tid];
+      s_X[i][tid] = x;
+    }
+  }
+  __syncthreads();
+  for (int idx = tid; idx < width * kb; idx += 256) {
+    const int k = idx % kb, t = idx / kb;
+    A[(c0 + k) + (long long)(j0 + t) * n] = s_X[k][t];
+  }
+}
+
+// ------------------------------------------------------------------------
+// Cholesky of the kb x kb diagonal block (lower), one wavefront per front.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_chol_diag(const uhmk_front* __restrict__ F,
+                                                  const int* __restrict__ flist, int c0, int nb,
+                                                  double* __restrict__ arena, int* __restrict__ info) {
+  const int f = flist[blockIdx.x];
+  const uhmk_front fr = F[f];
+  const int n = fr.n, fs = fr.fs;
+  const int kb = min(nb, fs - c0);
+  double* __restrict__ A = arena + fr.off;
+  __shared__ double s[TILE][LDP];
+  const int i = threadIdx.x;
+  for (int j = 0; j < kb; ++j)
+    if (i < kb) s[i][j] = A[(c0 + i) + (long long)(c0 + j) * n];
+  __syncthreads();
+  for (int k = 0; k < kb; ++k) {
+    const double d = s[k][k];
+    if (i == 0 && !(d > 0.0) && info[f] == 0) info[f] = c0 + k + 1;
+    const double lkk = sqrt(d);
+    __syncthreads();
+    if (i == k) s[k][k] = lkk;
+    if (i > k && i < kb) s[i][k] = s[i][k] / lkk;
+    __syncthreads();
+    if (i > k && i < kb) {
+      const double lik = s[i][k];
+      for (int j = k + 1; j <= i; ++j) s[i][j] -= lik * s[j][k];
+    }
+    __syncthreads();
+  }
+  for (int j = 0; j < kb; ++j)
+    if (i < kb && i >= j) A[(c0 + i) + (long long)(c0 + j) * n] = s[i][j];
+}
+
+// Cholesky panel rows [i0, i0+64) below the diagonal block: X <- X * L11^-T.
+__global__ __launch_bounds__(256) void k_chol_trsm(const uhmk_front* __restrict__ F,
+                                                   const uhmk_item* __restrict__ items, int c0, int nb,
+                                                   double* __restrict__ arena) {
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs;
+  const int kb = min(nb, fs - c0);
+  double* __restrict__ A = arena + fr.off;
+  const int tid = threadIdx.x;
+  __shared__ double s_T[TILE][LDP];
+  __shared__ double sx[TILE][LDP];
+  const int i0 = it.b, h = min(TILE, n - i0);
+  for (int idx = tid; idx < kb * kb; idx += 256) {
+    const int i = idx % kb, j = idx / kb;
+    s_T[i][j] = A[(c0 + i) + (long long)(c0 + j) * n];
+  }
+  for (int idx = tid; idx < h * kb; idx += 256) {
+    const int r = idx % h, k = idx / h;
+    sx[r][k] = A[(i0 + r) + (long long)(c0 + k) * n];
+  }
+  __syncthreads();
+  if (tid < h) {
+    for (int j = 0; j < kb; ++j) {
+      double x = sx[tid][j];
+      for (int k = 0; k < j; ++k) x -= sx[tid][k] * s_T[j][k];
+      sx[tid][j] = x / s_T[j][j];
+    }
+  }
+  __syncthreads();
+  for (int idx = tid; idx < h * kb; idx += 256) {
+    const int r = idx % h, k = idx / h;
+    A[(i0 + r) + (long long)(c0 + k) * n] = sx[r][k];
+  }
+}
+
+// ------------------------------------------------------------------------
+// Schur update C -= A * op(B) on 64 x 64 tiles with FP64 MFMA
+// (v_mfma_f64_16x16x4f64).  A = F[i, k], B = F[k, j] (NN) or F[j, k] (NT),
+// C = F[i, j].  K range: panel [c0, min(c0+nb, fs)) or, for the final Schur
+// complement (schur = 1), [0, fs).  A tile never straddles the fs boundary
+// (items are cut there), so its extent is min(i0+64, fs) or min(i0+64, n).
+// The MFMA computes the transposed tile so that each 16-lane group stores
+// 16 consecutive rows of a column.
+// ------------------------------------------------------------------------
+typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int KC = 16, LDK = TILE + 16;  // 160-word row stride: conflict-free half-wave reads
+
+template <bool TRANSB>
+__global__ __launch_bounds__(256) void k_gemm_f64(const uhmk_front* __restrict__ F,
+                                                  const uhmk_item* __restrict__ items, int c0, int nb,
+                                                  int schur, double* __restrict__ arena) {
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs;
+  const int kbeg = schur ? 0 : c0;
+  const int kend = schur ? fs : min(c0 + nb, fs);
+  const int i0 = it.a, j0 = it.b;
+  const int i1 = min(i0 + TILE, i0 < fs ? fs : n);
+  const int j1 = min(j0 + TILE, j0 < fs ? fs : n);
+  double* __restrict__ M = arena + fr.off;
+  __shared__ double sA[KC][LDK];
+  __shared__ double sB[KC][LDK];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int iw = (w & 1) * 32, jw = (w >> 1) * 32;
+  d4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+
+  for (int k0 = kbeg; k0 < kend; k0 += KC) {
+    {
+      const int i = tid & 63, kq = (tid >> 6) * 4;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = kq + u;
+        sA[k][i] = (i0 + i < i1 && k0 + k < kend) ? M[(i0 + i) + (long long)(k0 + k) * n] : 0.0;
+      }
+    }
+    if (TRANSB) {
+      const int j = tid & 63, kq = (tid >> 6) * 4;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = kq + u;
+        sB[k][j] = (j0 + j < j1 && k0 + k < kend) ? M[(j0 + j) + (long long)(k0 + k) * n] : 0.0;
+      }
+    } else {
+      const int k = tid & 15, jq = tid >> 4;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = jq + 16 * u;
+        sB[k][j] = (j0 + j < j1 && k0 + k < kend) ? M[(k0 + k) + (long long)(j0 + j) * n] : 0.0;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KC; kk += 4) {
+      const int kr = kk + (lane >> 4), cl = lane & 15;
+      double bo[2], ao[2];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) bo[b] = sB[kr][jw + b * 16 + cl];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) ao[a] = sA[kr][iw + a * 16 + cl];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bo[b], ao[a], acc[a][b], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int i = i0 + iw + a * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = j0 + jw + b * 16 + 4 * (lane >> 4) + r;
+        if (i < i1 && j < j1) M[i + (long long)j * n] -= acc[a][b][r];
+      }
+    }
+}
+
+inline int ret(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
+
+}  // namespace
+
+extern "C" {
+
+int uhmk_assemble(const uhmk_front* fronts, const uhmk_item* items, int n_items, const int* src_beg,
+                  const uhmk_source* srcs, const int* inv, double* arena, const double* stage,
+                  hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  hipLaunchKernelGGL(k_assemble, dim3(n_items), dim3(256), 0, stream, fronts, items, src_beg, srcs, inv,
+                     arena, stage);
+  return ret(hipGetLastError());
+}
+
+int uhmk_lu_panel(const uhmk_front* fronts, const int* flist, int n_fronts, int max_rows, int c0, int nb,
+                  double* arena, int* piv, int* info, hipStream_t stream) {
+  if (n_fronts <= 0) return 0;
+  if (nb > TILE || nb < 1) return -1000;
+  const int R = (max_rows + PNT - 1) / PNT;
+  dim3 g(n_fronts), b(PNT);
+  if (R <= 1) hipLaunchKernelGGL(k_lu_panel<1>, g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
+  else if (R <= 2) hipLaunchKernelGGL(k_lu_panel<2>, g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
+  else if (R <= 4) hipLaunchKernelGGL(k_lu_panel<4>, g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
+  else if (R <= 8) hipLaunchKernelGGL(k_lu_panel<8>, g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
+  else if (R <= 12) hipLaunchKernelGGL(k_lu_panel<12>, g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
+  else return -1001;  // more than 6144 pivot-candidate rows
+  return ret(hipGetLastError());
+}
+
+int uhmk_lu_rows(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb,
+                 double* arena, const int* piv, hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  if (nb > TILE || nb < 1) return -1000;
+  hipLaunchKernelGGL(k_lu_rows, dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb, arena, piv);
+  return ret(hipGetLastError());
+}
+
+int uhmk_chol_diag(const uhmk_front* fronts, const int* flist, int n_fronts, int c0, int nb, double* arena,
+                   int* info, hipStream_t stream) {
+  if (n_fronts <= 0) return 0;
+  if (nb > TILE || nb < 1) return -1000;
+  hipLaunchKernelGGL(k_chol_diag, dim3(n_fronts), dim3(64), 0, stream, fronts, flist, c0, nb, arena, info);
+  return ret(hipGetLastError());
+}
+
+int uhmk_chol_trsm(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb,
+                   double* arena, hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  if (nb > TILE || nb < 1) return -1000;
+  hipLaunchKernelGGL(k_chol_trsm, dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb, arena);
+  return ret(hipGetLastError());
+}
+
+int uhmk_gemm_update(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb,
+                     int schur, int transb, double* arena, hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  if (transb)
+    hipLaunchKernelGGL(k_gemm_f64<true>, dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb, schur, arena);
+  else
+    hipLaunchKernelGGL(k_gemm_f64<false>, dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb, schur, arena);
+  return ret(hipGetLastError());
+}
+
+}  // extern "C"
