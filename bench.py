@@ -1,0 +1,232 @@
+#!/usr/bin/env python3
+"""Benchmark of the multifrontal factorization on one MI355X.
+
+One step = one complete factorization of the synthetic workload (extend-add
+assembly of every front + partial factorization of every front, all tree
+levels, leaves first) with inputs already resident in HBM.  Reported value:
+algorithmic factor TFLOP/s (LAWN-41 count of the reference's
+Mesh_::estimate_cost, the same numerator whatever precision executes it).
+
+Default workload (the configuration the target is stated on): 3D unit cube,
+16^3 hexahedral Q4 elements (274,625 dofs, 8191 fronts, root separator
+4225), SPD element matrices K_e = (M^T M + 125 I) 10^u with u ~ U(-2, 2),
+Cholesky, one right-hand side.  `--workload lu` runs the 3D Q3 LU
+configuration (117,649 dofs, K_e = (M^T M + 64 I + 0.1 S) lognormal(0, 1)).
+
+Printed: one JSON line with the metric, the FP64 GPU path's throughput, the
+time from the start of factorization to a verified backward error <= 1e-12,
+the live roofline of the dominant kernel (the Schur-update GEMM) and the CPU
+oracle timed on a bounded sample of the same tree.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "factor TFLOP/s + time-to-1e-12 backward error, mixed-prec vs FP64 GPU"
+PEAK_FP64_TFLOPS = 78.6
+
+WORKLOADS = {
+    "chol": dict(kind=2, dims=(16, 16, 16), p=4, method=1, alpha=125.0, beta=0.0, scale_kind=2, p0=-2.0, p1=2.0,
+                 name="3D 16^3 hex Q4, SPD K_e=(M^T M+125 I)*10^U(-2,2), Cholesky, 1 RHS"),
+    "lu": dict(kind=2, dims=(16, 16, 16), p=3, method=3, alpha=64.0, beta=0.1, scale_kind=1, p0=1.0, p1=0.0,
+               name="3D 16^3 hex Q3, K_e=(M^T M+64 I+0.1 S)*lognormal(0,1), LU partial pivoting, 1 RHS"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def device_sync():
+    try:
+        ctypes.CDLL("libamdhip64.so").hipDeviceSynchronize()
+    except OSError:
+        pass
+
+
+def build_mesh(U, w, seed):
+    m = U.Mesh()
+    m.build_synthetic(w["kind"], w["dims"], w["p"], 0)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    m.create_matrix_buffer()
+    m.set_random_params(seed=seed, alpha=w["alpha"], beta=w["beta"], scale_kind=w["scale_kind"], p0=w["p0"],
+                        p1=w["p1"])
+    m.random_matrix()
+    m.set_rhs()
+    return m
+
+
+def cpu_baseline(U, m, w, target_gflop=60.0):
+    """The CPU oracle (C restatement of the reference algorithm, OpenMP over
+    the fronts of a level) on the bottom levels of the same tree, with fresh
+    element matrices of the same distribution drawn by numpy."""
+    import numpy as np
+    from oracle.oracle import OracleMesh, set_threads
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = max(1, min(threads, 16, os.cpu_count() or 1))
+    set_threads(threads)
+    c = m.export_connectivity()
+    om = OracleMesh.from_connectivity(c)
+    om.init_numeric(1)
+    levels = om.levels()
+    acc, L1 = 0.0, 0
+    for L in range(len(levels)):
+        fl = om.level_flops(w["method"], L, L)
+        if L > 0 and acc + fl > target_gflop * 1e9:
+            break
+        acc += fl
+        L1 = L
+    rng = np.random.default_rng(1)
+    eptr, enode = c["eptr"], c["enode"]
+    leaves = [e for e in range(len(c["parent"])) if eptr[e + 1] > eptr[e]]
+    for e in leaves:
+        nl = enode[eptr[e]:eptr[e + 1]]
+        k = len(nl)
+        M = rng.standard_normal((k, k))
+        K = M.T @ M + w["alpha"] * np.eye(k)
+        if w["beta"]:
+            K += w["beta"] * rng.standard_normal((k, k))
+        om.copy_in(e, nl, 1, K)
+    t0 = time.perf_counter()
+    info = om.factor_levels(w["method"], 256, 0, L1)
+    dt = time.perf_counter() - t0
+    return dict(value=round(acc / dt / 1e12, 5), unit="TFLOP/s", cores=threads, kind="port",
+                sample="oracle factorization of tree levels 0-%d (%d fronts, %.1f GFLOP of %.1f) of the same "
+                       "workload, tile b=256, %d OpenMP threads, %.2f s, info %d"
+                       % (L1, sum(len(levels[L]) for L in range(L1 + 1)), acc / 1e9,
+                          om.flops(w["method"]) / 1e9, threads, dt, info))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="chol")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--nb", type=int, default=64, help="panel width")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (one process, one MI355X); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import uhm_amd as U
+    w = WORKLOADS[args.workload]
+    chol = w["method"] == 1
+    t0 = time.perf_counter()
+    m = build_mesh(U, w, seed=3)
+    U.set_hier_block_size(args.nb)
+    ndof = m.get_n_dof()
+    flops, _, _, _ = m.estimate_cost(w["method"])
+    log("mesh %s: ndof %d, fronts %d, %.1f GFLOP, setup %.1f s"
+        % (args.workload, ndof, m.n_fronts(), flops / 1e9, time.perf_counter() - t0))
+    factor = m.chol_with_free if chol else m.lu_piv_with_free
+    solve = m.solve_chol if chol else m.solve_lu_piv
+    check = m.check_chol if chol else m.check_lu_piv
+
+    for i in range(args.warmup):
+        info = factor()
+        log("warmup %d: %.2f ms (info %d)" % (i, m.timing()[0], info))
+    device_sync()
+    t0 = time.perf_counter()
+    dev_ms = []
+    for i in range(args.steps):
+        info = factor()
+        dev_ms.append(m.timing()[0])
+    device_sync()
+    wall = time.perf_counter() - t0
+    ms_step = wall / args.steps * 1e3
+    log("timed %d steps: %.3f ms/step wall, device %.3f ms/step" % (args.steps, ms_step, sum(dev_ms) / len(dev_ms)))
+    if info:
+        raise SystemExit("zero / non-positive pivot in element %d" % (info - 1))
+
+    # time to a verified backward error <= 1e-12 (FP64 path: one solve)
+    solve()
+    check()  # builds the assembled CSR once (symbolic, host)
+    device_sync()
+    t0 = time.perf_counter()
+    factor()
+    solve()
+    check()
+    device_sync()
+    t_sol = (time.perf_counter() - t0) * 1e3
+    eta = m.backward_error()
+    f_ms, s_ms = m.timing()
+    log("factor+solve+check %.2f ms, solve %.2f ms, backward error %.3e" % (t_sol, s_ms, eta))
+
+    # live per-kernel timing (device events around each launch)
+    m.set_profile(True)
+    factor()
+    stats = m.kernel_stats()
+    m.set_profile(False)
+    g_ms = stats["gemm"][0] + stats["schur"][0]
+    g_fl = stats["gemm"][1] + stats["schur"][1]
+    g_n = stats["gemm"][3] + stats["schur"][3]
+    achieved = g_fl / (g_ms * 1e-3) / 1e12 if g_ms else 0.0
+    for k, (ms_k, fl_k, by_k, n_k) in stats.items():
+        if n_k:
+            log("  %-10s %5d launches %9.3f ms  %8.3f TFLOP/s  %8.1f GB/s"
+                % (k, n_k, ms_k, fl_k / ms_k / 1e9 if ms_k else 0, by_k / ms_k / 1e6 if ms_k else 0))
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(U, m, w)
+            log("cpu baseline:", cpu["value"], cpu["unit"], cpu["sample"])
+        except Exception as e:  # the baseline is reported, never the product
+            log("cpu baseline failed:", repr(e))
+
+    value = flops / (ms_step * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Philox element matrices generated on the device)",
+        "config": {
+            "workload": w["name"],
+            "ndof": ndof,
+            "fronts": m.n_fronts(),
+            "factor_gflop": round(flops / 1e9, 3),
+            "precision_path": "fp64 (plain FP64 GPU path)",
+            "panel_width": args.nb,
+        },
+        "time_to_1e-12_ms": round(t_sol, 3) if eta <= 1e-12 else None,
+        "backward_error": eta,
+        "solve_ms": round(s_ms, 3),
+        "roofline": {
+            "kernel": "k_gemm_f64 (Schur / trailing update, FP64 MFMA)",
+            "bound": "mfma",
+            "achieved": round(achieved, 4),
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+            "traffic": None,
+            "launches": g_n,
+            "avg_launch_us": round(g_ms / g_n * 1e3, 3) if g_n else None,
+        },
+        "kernels": {k: {"ms": round(v[0], 4), "gflop": round(v[1] / 1e9, 3), "gbytes": round(v[2] / 1e9, 4),
+                        "launches": v[3]} for k, v in stats.items() if v[3]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

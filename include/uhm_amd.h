@@ -141,6 +141,14 @@ void uhm_get_info_(uhm_fort_p* mesh, uhm_fort_int* info);
 void uhm_get_backward_error_(uhm_fort_p* mesh, uhm_fort_double* eta);
 /* Milliseconds of the last factorization and solve (device events). */
 void uhm_get_timing_(uhm_fort_p* mesh, uhm_fort_double* factor_ms, uhm_fort_double* solve_ms);
+/* Per-launch device timing of the next factorizations (on = 1) and the
+ * totals of the last profiled one, per kernel kind in the order assemble,
+ * lu_panel, lu_rows, chol_diag, chol_trsm, gemm (panel update), schur
+ * (final Schur complement): milliseconds, algorithmic flops, algorithmic
+ * bytes and launch counts (arrays of 7). */
+void uhm_set_profile_(uhm_fort_p* mesh, uhm_fort_int* on);
+void uhm_get_kernel_stats_(uhm_fort_p* mesh, uhm_fort_double* ms, uhm_fort_double* flops, uhm_fort_double* bytes,
+                           uhm_fort_int* launches);
 /* Last error message (empty when none); returns its length. */
 int uhm_get_last_error_(char* buf, int len);
 

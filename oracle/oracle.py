@@ -53,6 +53,10 @@ def load():
     lib.orc_set_rhs_front.argtypes = [C.c_void_p, C.c_int, _D]
     lib.orc_factor.restype = C.c_int
     lib.orc_factor.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.orc_factor_levels.restype = C.c_int
+    lib.orc_factor_levels.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+    lib.orc_level_flops.restype = C.c_double
+    lib.orc_level_flops.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
     lib.orc_solve.argtypes = [C.c_void_p, C.c_int]
     lib.orc_get_front.argtypes = [C.c_void_p, C.c_int, _D]
     lib.orc_get_pivots.argtypes = [C.c_void_p, C.c_int, _I]
@@ -179,6 +183,12 @@ class OracleMesh:
 
     def factor(self, method, b=256):
         return _lib.orc_factor(self.h, method, b)
+
+    def factor_levels(self, method, b, L0, L1):
+        return _lib.orc_factor_levels(self.h, method, b, L0, L1)
+
+    def level_flops(self, method, L0, L1):
+        return _lib.orc_level_flops(self.h, method, L0, L1)
 
     def solve(self, method):
         _lib.orc_solve(self.h, method)

@@ -411,14 +411,18 @@ static void alloc_numeric(orc* o, int nrhs) {
   o->piv = (int**)calloc(o->n_elem, sizeof(int*));
   o->X = (double**)calloc(o->n_elem, sizeof(double*));
   o->B = (double**)calloc(o->n_elem, sizeof(double*));
-  for (int e = 0; e < o->n_elem; ++e) {
-    size_t n = (size_t)(o->fs[e] + o->ss[e]);
-    o->F[e] = (double*)calloc(n * n + 1, sizeof(double));
-    o->K[e] = (double*)calloc(n * n + 1, sizeof(double));
-    o->piv[e] = (int*)calloc(o->fs[e] + 1, sizeof(int));
-    o->X[e] = (double*)calloc(n * nrhs + 1, sizeof(double));
-    o->B[e] = (double*)calloc(n * nrhs + 1, sizeof(double));
-  }
+}
+
+/* Per-front storage is allocated on first use, so a sample of levels of a
+ * large mesh fits in host memory. */
+static void ensure(orc* o, int e) {
+  if (o->F[e]) return;
+  size_t n = (size_t)(o->fs[e] + o->ss[e]);
+  o->K[e] = (double*)calloc(n * n + 1, sizeof(double));
+  o->piv[e] = (int*)calloc(o->fs[e] + 1, sizeof(int));
+  o->X[e] = (double*)calloc(n * o->nrhs + 1, sizeof(double));
+  o->B[e] = (double*)calloc(n * o->nrhs + 1, sizeof(double));
+  o->F[e] = (double*)calloc(n * n + 1, sizeof(double));
 }
 
 void orc_init_numeric(void* h, int nrhs) { alloc_numeric((orc*)h, nrhs); }
@@ -446,6 +450,7 @@ int orc_copy_in(void* h, int e, int nn, const int* nods, int side, int m, const 
     free(map);
     return -1;
   }
+  ensure(o, e);
   int n = o->fs[e] + o->ss[e];
   if (side == 1) {
     for (int j = 0; j < m; ++j)
@@ -462,11 +467,13 @@ int orc_copy_in(void* h, int e, int nn, const int* nods, int side, int m, const 
 void orc_set_front(void* h, int e, const double* F) {
   orc* o = (orc*)h;
   size_t n = (size_t)(o->fs[e] + o->ss[e]);
+  ensure(o, e);
   memcpy(o->K[e], F, n * n * sizeof(double));
 }
 void orc_set_rhs_front(void* h, int e, const double* b) {
   orc* o = (orc*)h;
   size_t n = (size_t)(o->fs[e] + o->ss[e]);
+  ensure(o, e);
   memcpy(o->B[e], b, n * o->nrhs * sizeof(double));
 }
 
@@ -661,11 +668,14 @@ static void merge_A(orc* o, int e) {
   }
 }
 
-int orc_factor(void* h, int method, int b) {
+/* Levels [L0, L1] (leaves are level 0); every child of those fronts must
+ * have been factored. */
+int orc_factor_levels(void* h, int method, int b, int L0, int L1) {
   orc* o = (orc*)h;
   o->info = 0;
-  for (int L = 0; L < o->n_levels; ++L) {
+  for (int L = L0; L <= L1 && L < o->n_levels; ++L) {
     const int lo = o->lvl_ptr[L], hi = o->lvl_ptr[L + 1];
+    for (int t = lo; t < hi; ++t) ensure(o, o->lvl_elem[t]);
     int first = hi;
 #pragma omp parallel for schedule(dynamic, 1) reduction(min : first)
     for (int t = lo; t < hi; ++t) {
@@ -680,9 +690,12 @@ int orc_factor(void* h, int method, int b) {
   return o->info;
 }
 
+int orc_factor(void* h, int method, int b) { return orc_factor_levels(h, method, b, 0, ((orc*)h)->n_levels - 1); }
+
 /* Forward (leaf -> root) then backward (root -> leaf) solves. */
 void orc_solve(void* h, int method) {
   orc* o = (orc*)h;
+  for (int e = 0; e < o->n_elem; ++e) ensure(o, e);
   for (int L = 0; L < o->n_levels; ++L) {
 #pragma omp parallel for schedule(dynamic, 1)
     for (int t = o->lvl_ptr[L]; t < o->lvl_ptr[L + 1]; ++t) {
@@ -771,16 +784,19 @@ void orc_solve(void* h, int method) {
 
 void orc_get_front(void* h, int e, double* out) {
   orc* o = (orc*)h;
+  ensure(o, e);
   size_t n = (size_t)(o->fs[e] + o->ss[e]);
   memcpy(out, o->F[e], n * n * sizeof(double));
 }
 void orc_get_pivots(void* h, int e, int* out) {
   orc* o = (orc*)h;
+  ensure(o, e);
   memcpy(out, o->piv[e], o->fs[e] * sizeof(int));
 }
 /* solution in the global numbering: x[offset(node) + d] */
 void orc_get_x_global(void* h, double* out, int ndof) {
   orc* o = (orc*)h;
+  for (int e = 0; e < o->n_elem; ++e) ensure(o, e);
   for (int e = 0; e < o->n_elem; ++e) {
     int n = o->fs[e] + o->ss[e];
     for (int q = 0; q < o->nfac[e]; ++q) {
@@ -810,6 +826,22 @@ double orc_flops(void* h, int method) {
       s += 2.0 * q * q * f;
     }
   }
+  return s;
+}
+
+/* algorithmic flops of the fronts in levels [L0, L1] */
+double orc_level_flops(void* h, int method, int L0, int L1) {
+  orc* o = (orc*)h;
+  double s = 0.0;
+  for (int L = L0; L <= L1 && L < o->n_levels; ++L)
+    for (int t = o->lvl_ptr[L]; t < o->lvl_ptr[L + 1]; ++t) {
+      const int e = o->lvl_elem[t];
+      const double f = o->fs[e], q = o->ss[e];
+      if (method == 1)
+        s += (f * f * f / 3.0 + f * f / 2.0 + f / 6.0) + q * f * f + f * q * (q + 1.0);
+      else
+        s += (2.0 * f * f * f / 3.0 - f * f / 2.0 + 5.0 * f / 6.0) + 2.0 * q * f * f + 2.0 * q * q * f;
+    }
   return s;
 }
 

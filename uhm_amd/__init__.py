@@ -83,7 +83,10 @@ _SIGS = {
     "uhm_fetch_x_global_": [_P, _D], "uhm_fetch_b_global_": [_P, _D],
     "uhm_get_csr_size_": [_P, _LL], "uhm_fetch_csr_": [_P, _LL, _I, _D],
     "uhm_export_connectivity_": [_P, _I, _I, _I, _I, _I, _I, _I, _I, _I],
+    "uhm_set_profile_": [_P, _I], "uhm_get_kernel_stats_": [_P, _D, _D, _D, _I],
 }
+
+KERNEL_KINDS = ("assemble", "lu_panel", "lu_rows", "chol_diag", "chol_trsm", "gemm", "schur")
 _RET = {"uhm_get_last_error_": C.c_int}
 
 
@@ -288,6 +291,17 @@ class Mesh:
         a, b = C.c_double(0), C.c_double(0)
         _call("uhm_get_timing_", C.byref(self.handle), C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def set_profile(self, on=True):
+        _call("uhm_set_profile_", C.byref(self.handle), _i(1 if on else 0))
+
+    def kernel_stats(self):
+        """{kind: (ms, algorithmic flops, algorithmic bytes, launches)} of the
+        last profiled factorization."""
+        ms, fl, by = (np.zeros(7) for _ in range(3))
+        ln = np.zeros(7, np.int32)
+        _call("uhm_get_kernel_stats_", C.byref(self.handle), _dptr(ms), _dptr(fl), _dptr(by), _iptr(ln))
+        return {k: (float(ms[i]), float(fl[i]), float(by[i]), int(ln[i])) for i, k in enumerate(KERNEL_KINDS)}
 
     # ---- queries
     def get_n_dof(self):

@@ -244,6 +244,21 @@ void uhm_get_timing_(uhm_fort_p* mesh, uhm_fort_double* fms, uhm_fort_double* sm
     *sms = S(M(mesh))->last_solve_ms();
   });
 }
+void uhm_set_profile_(uhm_fort_p* mesh, uhm_fort_int* on) {
+  guard("uhm_set_profile", [&] { S(M(mesh))->set_profile(*on != 0); });
+}
+void uhm_get_kernel_stats_(uhm_fort_p* mesh, uhm_fort_double* ms, uhm_fort_double* flops, uhm_fort_double* bytes,
+                           uhm_fort_int* launches) {
+  guard("uhm_get_kernel_stats", [&] {
+    const uhm::KernelStats& s = S(M(mesh))->kernel_stats();
+    for (int k = 0; k < uhm::N_LAUNCH_KINDS; ++k) {
+      ms[k] = s.ms[k];
+      flops[k] = s.flops[k];
+      bytes[k] = s.bytes[k];
+      launches[k] = s.launches[k];
+    }
+  });
+}
 int uhm_get_last_error_(char* buf, int len) {
   const int n = (int)g_last_error.size();
   if (buf && len > 0) {

@@ -564,7 +564,9 @@ __global__ __launch_bounds__(256) void k_chol_trsm(const uhmk_front* __restrict_
 // complement (schur = 1), [0, fs).  A tile never straddles the fs boundary
 // (items are cut there), so its extent is min(i0+64, fs) or min(i0+64, n).
 // The MFMA computes the transposed tile so that each 16-lane group stores
-// 16 consecutive rows of a column.
+// 16 consecutive rows of a column.  Operand layout of the f64 16x16x4
+// instruction: A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, and
+// accumulator register r of lane l holds D[4 r + l / 16][l % 16].
 // ------------------------------------------------------------------------
 typedef double d4 __attribute__((ext_vector_type(4)));
 constexpr int KC = 16, LDK = TILE + 16;  // 160-word row stride: conflict-free half-wave reads
@@ -639,7 +641,9 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const uhmk_front* __restrict__
       const int i = i0 + iw + a * 16 + (lane & 15);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int j = j0 + jw + b * 16 + 4 * (lane >> 4) + r;
+        // f64 16x16x4: accumulator register r holds row 4 r + lane / 16
+        // of the 16 x 16 block (column lane % 16).
+        const int j = j0 + jw + b * 16 + 4 * r + (lane >> 4);
         if (i < i1 && j < j1) M[i + (long long)j * n] -= acc[a][b][r];
       }
     }

@@ -403,6 +403,73 @@ void FrontSolver::build_schedule(int method) {
     ls.count = (int)(items_.size() - ls.off);
     if (ls.count) launches_.push_back(ls);
   }
+  // Algorithmic work of every launch: the flops / bytes the operation needs,
+  // not what a kernel happens to execute (diagonal SYRK tiles count their
+  // lower half, assembly counts each stored entry once).
+  auto tri = [](double m) { return 0.5 * m * (m + 1.0); };
+  for (Launch& l : launches_) {
+    double fl = 0, by = 0;
+    switch (l.kind) {
+      case Launch::ASSEMBLE:
+        for (int f : levels_[l.level]) {
+          const double n = fr_[f].n;
+          const double w = chol ? tri(n) : n * n;
+          double rd = fr_[f].soff >= 0 ? w : 0.0;
+          for (Element_* c : order_[f]->children) rd += chol ? tri(c->ss) : (double)c->ss * c->ss;
+          by += 8.0 * (w + rd);
+        }
+        break;
+      case Launch::LU_PANEL:
+      case Launch::CHOL_DIAG:
+        for (int q = 0; q < l.count; ++q) {
+          const int f = flists_[l.off + q];
+          const double m = fr_[f].fs - l.c0, kb = std::min<double>(l.nb, m);
+          if (l.kind == Launch::LU_PANEL) {
+            fl += flop_lu(m, kb);
+            by += 16.0 * m * kb;
+          } else {
+            fl += flop_chol(kb);
+            by += 16.0 * kb * kb;
+          }
+        }
+        break;
+      case Launch::LU_ROWS:
+      case Launch::CHOL_TRSM:
+        for (int q = 0; q < l.count; ++q) {
+          const uhmk_item& it = items_[l.off + q];
+          const int fs = fr_[it.f].fs, n = fr_[it.f].n;
+          const double kb = std::min(l.nb, fs - l.c0);
+          if (l.kind == Launch::CHOL_TRSM || it.a == 1) {
+            const double h = std::min(64, n - it.b);
+            fl += kb * kb * h;
+            by += 16.0 * kb * h;
+          } else {
+            const double w = std::min(64, (it.a == 0 ? l.c0 : n) - it.b);
+            if (it.a == 2) fl += kb * kb * w;
+            by += 16.0 * 2.0 * kb * w;
+          }
+        }
+        break;
+      case Launch::GEMM:
+      case Launch::SCHUR:
+        for (int q = 0; q < l.count; ++q) {
+          const uhmk_item& it = items_[l.off + q];
+          const int fs = fr_[it.f].fs, n = fr_[it.f].n;
+          const double K = (l.kind == Launch::SCHUR) ? fs : std::min(l.c0 + l.nb, fs) - l.c0;
+          const int i1 = tile_end(it.a, fs, n), j1 = tile_end(it.b, fs, n);
+          double cnt = 0;
+          if (chol)
+            for (int j = it.b; j < j1; ++j) cnt += std::max(0, i1 - std::max(it.a, j));
+          else
+            cnt = (double)(i1 - it.a) * (j1 - it.b);
+          fl += 2.0 * K * cnt;
+          by += 16.0 * cnt + 8.0 * K * ((i1 - it.a) + (j1 - it.b));
+        }
+        break;
+    }
+    l.flops = fl;
+    l.bytes = by;
+  }
   d_items_.upload(items_, stream_);
   d_flists_.upload(flists_, stream_);
   check_hip(hipStreamSynchronize(stream_), "schedule upload");
@@ -426,8 +493,16 @@ int FrontSolver::factor(int method) {
   check_hip(hipEventCreate(&e0), "event");
   check_hip(hipEventCreate(&e1), "event");
   check_hip(hipEventRecord(e0, stream_), "event");
-  for (const Launch& l : launches_) {
+  std::vector<hipEvent_t> pev;
+  if (profile_) {
+    stats_ = KernelStats();
+    pev.resize(2 * launches_.size());
+    for (auto& e : pev) check_hip(hipEventCreate(&e), "event");
+  }
+  for (size_t li = 0; li < launches_.size(); ++li) {
+    const Launch& l = launches_[li];
     int rc = 0;
+    if (profile_) check_hip(hipEventRecord(pev[2 * li], stream_), "event");
     switch (l.kind) {
       case Launch::ASSEMBLE:
         rc = uhmk_assemble(F, IT + l.off, l.count, d_srcbeg_.as<int>(), d_srcs_.as<uhmk_source>(), d_inv_.as<int>(),
@@ -453,11 +528,24 @@ int FrontSolver::factor(int method) {
         break;
     }
     check_k(rc, "factor");
+    if (profile_) check_hip(hipEventRecord(pev[2 * li + 1], stream_), "event");
   }
   check_hip(hipEventRecord(e1, stream_), "event");
   check_hip(hipEventSynchronize(e1), "factor");
   float ms = 0;
   check_hip(hipEventElapsedTime(&ms, e0, e1), "event");
+  if (profile_) {
+    for (size_t li = 0; li < launches_.size(); ++li) {
+      float t = 0;
+      check_hip(hipEventElapsedTime(&t, pev[2 * li], pev[2 * li + 1]), "event");
+      const int k = (int)launches_[li].kind;
+      stats_.ms[k] += t;
+      stats_.flops[k] += launches_[li].flops;
+      stats_.bytes[k] += launches_[li].bytes;
+      stats_.launches[k] += 1;
+    }
+    for (auto& e : pev) (void)hipEventDestroy(e);
+  }
   t_factor_ms_ = ms;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -47,10 +47,17 @@ struct Launch {
   int count = 0;
   int extra = 0;      // max pivot rows (panel), transb (gemm / schur)
   int level = 0;
+  double flops = 0;   // algorithmic flops of the launch
+  double bytes = 0;   // algorithmic HBM bytes of the launch
 };
+constexpr int N_LAUNCH_KINDS = 7;
 
-struct TimingReport {
-  double factor_ms = 0, solve_ms = 0;
+// Per-kind totals of a profiled factorization (device events per launch).
+struct KernelStats {
+  double ms[N_LAUNCH_KINDS] = {0};
+  double flops[N_LAUNCH_KINDS] = {0};
+  double bytes[N_LAUNCH_KINDS] = {0};
+  int launches[N_LAUNCH_KINDS] = {0};
 };
 
 class FrontSolver {
@@ -99,7 +106,12 @@ class FrontSolver {
   void csr(std::vector<long long>& rowptr, std::vector<int>& col, std::vector<double>& val);
 
   // tuning / timing
-  void set_block(int nb) { nb_ = nb; schedule_method_ = 0; }
+  void set_block(int nb) {
+    if (nb != nb_) schedule_method_ = 0;
+    nb_ = nb;
+  }
+  void set_profile(bool on) { profile_ = on; }
+  const KernelStats& kernel_stats() const { return stats_; }
   int block() const { return nb_; }
   double last_factor_ms() const { return t_factor_ms_; }
   double last_solve_ms() const { return t_solve_ms_; }
@@ -163,6 +175,8 @@ class FrontSolver {
   double t_factor_ms_ = 0, t_solve_ms_ = 0;
   int last_method_ = 0;
   bool factored_ = false;
+  bool profile_ = false;
+  KernelStats stats_;
 };
 
 }  // namespace uhm
