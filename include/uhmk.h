@@ -85,12 +85,13 @@ int uhmk_chol_diag(const uhmk_front* fronts, const int* flist, int n_fronts, int
 int uhmk_chol_trsm(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb,
                    double* arena, hipStream_t stream);
 
-/* C -= A op(B) on 64x64 tiles, FP64 MFMA: panel update (schur = 0, K =
- * [c0, c0+nb)) or final Schur complement ABR -= ABL ATR / ABL ABL^T
- * (schur = 1, K = [0, fs)).  Replaces linal::dense::gemm / syrk
- * (LINAL/src/dense/gemm.cxx:48-137, syrk.cxx:43-123).  items: {f, i0, j0, 0}. */
+/* C -= A op(B) on tile x tile blocks (tile 64 or 128), FP64 MFMA: panel
+ * update (schur = 0, K = [c0, c0+nb)) or final Schur complement
+ * ABR -= ABL ATR / ABL ABL^T (schur = 1, K = [0, fs)).  Replaces
+ * linal::dense::gemm / syrk (LINAL/src/dense/gemm.cxx:48-137,
+ * syrk.cxx:43-123).  items: {f, i0, j0, 0}, tiles cut at fs. */
 int uhmk_gemm_update(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb,
-                     int schur, int transb, double* arena, hipStream_t stream);
+                     int schur, int transb, int tile, double* arena, hipStream_t stream);
 
 /* ---- solve ---------------------------------------------------------------- */
 

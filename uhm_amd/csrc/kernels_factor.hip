@@ -16,6 +16,7 @@
 //   trsm / gemm / syrk  LINAL/src/dense/{trsm,gemm,syrk}.cxx
 #include <hip/hip_runtime.h>
 
+#include "dense_device.hpp"
 #include "uhmk.h"
 
 namespace {
@@ -379,6 +380,15 @@ __device__ inline void panel_perm(const int* s_p, int kb, int c0, int fs, int* s
   *s_cnt = cnt;
 }
 
+// ------------------------------------------------------------------------
+// After a panel: row swaps for every column outside the panel
+// (apply_pivots), the U strip U12 = L11^-1 A12 for the columns right of the
+// panel, and the ABL rows of the panel L21 = A21 U11^-1 (rows [fs, n)).
+// Item: f, a = kind (0 left columns, 1 ABL rows, 2 right columns),
+// b = first column / row of the 64-wide chunk.  The triangular solves use
+// 4 lanes per vector (tri_solve64), 64 vectors per workgroup.
+// ------------------------------------------------------------------------
+
 __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ F,
                                                  const uhmk_item* __restrict__ items, int c0, int nb,
                                                  double* __restrict__ arena, const int* __restrict__ piv) {
@@ -387,37 +397,36 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
   const int n = fr.n, fs = fr.fs;
   const int kb = min(nb, fs - c0);
   double* __restrict__ A = arena + fr.off;
-  const int tid = threadIdx.x;
-  __shared__ double s_X[2 * TILE][TILE];  // permuted rows / strip
-  __shared__ double s_T[TILE][LDP];       // triangular block
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, q = lane >> 4;
+  __shared__ double s_X[2 * TILE][TILE];  // rows touched by the swaps, per column
+  __shared__ double s_T[TILE * LDP];      // triangular panel block
+  __shared__ double s_rd[TILE];
   __shared__ int s_p[TILE], s_prow[2 * TILE], s_psrc[2 * TILE];
   __shared__ int s_cnt;
 
   if (it.a == 1) {
-    // ABL rows [start, start+h): X <- X * U11^-1
-    const int i0 = it.b, h = min(TILE, n - i0);
-    double(*sx)[LDP] = reinterpret_cast<double(*)[LDP]>(&s_X[0][0]);
     for (int idx = tid; idx < kb * kb; idx += 256) {
-      const int i = idx % kb, j = idx / kb;
-      s_T[i][j] = A[(c0 + i) + (long long)(c0 + j) * n];
-    }
-    for (int idx = tid; idx < h * kb; idx += 256) {
-      const int r = idx % h, k = idx / h;
-      sx[r][k] = A[(i0 + r) + (long long)(c0 + k) * n];
+      const int r = idx % kb, c = idx / kb;
+      s_T[r * LDP + c] = A[(c0 + r) + (long long)(c0 + c) * n];
     }
     __syncthreads();
-    if (tid < h) {
-      for (int j = 0; j < kb; ++j) {
-        double x = sx[tid][j];
-        for (int k = 0; k < j; ++k) x -= sx[tid][k] * s_T[k][j];
-        sx[tid][j] = x / s_T[j][j];
+    if (tid < kb) s_rd[tid] = 1.0 / s_T[tid * LDP + tid];
+    __syncthreads();
+    const int row = it.b + 16 * w + (lane & 15);
+    const bool live = row < n;
+    double x[16];
+#pragma unroll
+    for (int cr = 0; cr < 16; ++cr) {
+      const int c = 16 * q + cr;
+      x[cr] = (live && c < kb) ? A[row + (long long)(c0 + c) * n] : 0.0;
+    }
+    uhmd::tri_solve64<false, true>(x, s_T, LDP, s_rd, kb, lane);
+    if (live)
+#pragma unroll
+      for (int cr = 0; cr < 16; ++cr) {
+        const int c = 16 * q + cr;
+        if (c < kb) A[row + (long long)(c0 + c) * n] = x[cr];
       }
-    }
-    __syncthreads();
-    for (int idx = tid; idx < h * kb; idx += 256) {
-      const int r = idx % h, k = idx / h;
-      A[(i0 + r) + (long long)(c0 + k) * n] = sx[r][k];
-    }
     return;
   }
 
@@ -440,212 +449,30 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
     }
     return;
   }
-  // right columns: rows moved out of the strip go back now; the strip is
-  // solved in LDS first.
+  // right columns: rows moved below the strip go back now; the strip is
+  // solved in registers, one column per 4 lanes.
   for (int idx = tid; idx < width * (pc - kb); idx += 256) {
     const int k = kb + idx % (pc - kb), t = idx / (pc - kb);
     A[s_prow[k] + (long long)(j0 + t) * n] = s_X[s_psrc[k]][t];
   }
   for (int idx = tid; idx < kb * kb; idx += 256) {
-    const int i = idx % kb, j = idx / kb;
-    s_T[i][j] = A[(c0 + i) + (long long)(c0 + j) * n];
-  }
-  // Gather the permuted strip into rows [0, kb) of s_X.  Sources may sit
-  // anywhere in [0, pc), so every value is read before any is written.
-  {
-    double g[16];
-#pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int idx = tid + 256 * u;
-      if (idx < width * kb) {
-        const int k = idx % kb, t = idx / kb;
-        g[u] = s_X[s_psrc[k]][t];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int idx = tid + 256 * u;
-      if (idx < width * kb) {
-        const int k = idx % kb, t = idx / kb;
-        s_X[k][t] = g[u];
-      }
-    }
+    const int r = idx % kb, c = idx / kb;
+    s_T[r * LDP + c] = A[(c0 + r) + (long long)(c0 + c) * n];
   }
   __syncthreads();
-  if (tid < width) {
-    for (int i = 1; i < kb; ++i) {
-      double x = s_X[i][tid];
-      for (int k = 0; k < i; ++k) x -= s_T[i][k] * s_X[k][tid];
-      s_X[i][tid] = x;
-    }
+  const int t = 16 * w + (lane & 15);
+  double x[16];
+#pragma unroll
+  for (int cr = 0; cr < 16; ++cr) {
+    const int k = 16 * q + cr;
+    x[cr] = (t < width && k < kb) ? s_X[s_psrc[k]][t] : 0.0;
   }
-  __syncthreads();
-  for (int idx = tid; idx < width * kb; idx += 256) {
-    const int k = idx % kb, t = idx / kb;
-    A[(c0 + k) + (long long)(j0 + t) * n] = s_X[k][t];
-  }
-}
-
-// ------------------------------------------------------------------------
-// Cholesky of the kb x kb diagonal block (lower), one wavefront per front.
-// ------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_chol_diag(const uhmk_front* __restrict__ F,
-                                                  const int* __restrict__ flist, int c0, int nb,
-                                                  double* __restrict__ arena, int* __restrict__ info) {
-  const int f = flist[blockIdx.x];
-  const uhmk_front fr = F[f];
-  const int n = fr.n, fs = fr.fs;
-  const int kb = min(nb, fs - c0);
-  double* __restrict__ A = arena + fr.off;
-  __shared__ double s[TILE][LDP];
-  const int i = threadIdx.x;
-  for (int j = 0; j < kb; ++j)
-    if (i < kb) s[i][j] = A[(c0 + i) + (long long)(c0 + j) * n];
-  __syncthreads();
-  for (int k = 0; k < kb; ++k) {
-    const double d = s[k][k];
-    if (i == 0 && !(d > 0.0) && info[f] == 0) info[f] = c0 + k + 1;
-    const double lkk = sqrt(d);
-    __syncthreads();
-    if (i == k) s[k][k] = lkk;
-    if (i > k && i < kb) s[i][k] = s[i][k] / lkk;
-    __syncthreads();
-    if (i > k && i < kb) {
-      const double lik = s[i][k];
-      for (int j = k + 1; j <= i; ++j) s[i][j] -= lik * s[j][k];
-    }
-    __syncthreads();
-  }
-  for (int j = 0; j < kb; ++j)
-    if (i < kb && i >= j) A[(c0 + i) + (long long)(c0 + j) * n] = s[i][j];
-}
-
-// Cholesky panel rows [i0, i0+64) below the diagonal block: X <- X * L11^-T.
-__global__ __launch_bounds__(256) void k_chol_trsm(const uhmk_front* __restrict__ F,
-                                                   const uhmk_item* __restrict__ items, int c0, int nb,
-                                                   double* __restrict__ arena) {
-  const uhmk_item it = items[blockIdx.x];
-  const uhmk_front fr = F[it.f];
-  const int n = fr.n, fs = fr.fs;
-  const int kb = min(nb, fs - c0);
-  double* __restrict__ A = arena + fr.off;
-  const int tid = threadIdx.x;
-  __shared__ double s_T[TILE][LDP];
-  __shared__ double sx[TILE][LDP];
-  const int i0 = it.b, h = min(TILE, n - i0);
-  for (int idx = tid; idx < kb * kb; idx += 256) {
-    const int i = idx % kb, j = idx / kb;
-    s_T[i][j] = A[(c0 + i) + (long long)(c0 + j) * n];
-  }
-  for (int idx = tid; idx < h * kb; idx += 256) {
-    const int r = idx % h, k = idx / h;
-    sx[r][k] = A[(i0 + r) + (long long)(c0 + k) * n];
-  }
-  __syncthreads();
-  if (tid < h) {
-    for (int j = 0; j < kb; ++j) {
-      double x = sx[tid][j];
-      for (int k = 0; k < j; ++k) x -= sx[tid][k] * s_T[j][k];
-      sx[tid][j] = x / s_T[j][j];
-    }
-  }
-  __syncthreads();
-  for (int idx = tid; idx < h * kb; idx += 256) {
-    const int r = idx % h, k = idx / h;
-    A[(i0 + r) + (long long)(c0 + k) * n] = sx[r][k];
-  }
-}
-
-// ------------------------------------------------------------------------
-// Schur update C -= A * op(B) on 64 x 64 tiles with FP64 MFMA
-// (v_mfma_f64_16x16x4f64).  A = F[i, k], B = F[k, j] (NN) or F[j, k] (NT),
-// C = F[i, j].  K range: panel [c0, min(c0+nb, fs)) or, for the final Schur
-// complement (schur = 1), [0, fs).  A tile never straddles the fs boundary
-// (items are cut there), so its extent is min(i0+64, fs) or min(i0+64, n).
-// The MFMA computes the transposed tile so that each 16-lane group stores
-// 16 consecutive rows of a column.  Operand layout of the f64 16x16x4
-// instruction: A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, and
-// accumulator register r of lane l holds D[4 r + l / 16][l % 16].
-// ------------------------------------------------------------------------
-typedef double d4 __attribute__((ext_vector_type(4)));
-constexpr int KC = 16, LDK = TILE + 16;  // 160-word row stride: conflict-free half-wave reads
-
-template <bool TRANSB>
-__global__ __launch_bounds__(256) void k_gemm_f64(const uhmk_front* __restrict__ F,
-                                                  const uhmk_item* __restrict__ items, int c0, int nb,
-                                                  int schur, double* __restrict__ arena) {
-  const uhmk_item it = items[blockIdx.x];
-  const uhmk_front fr = F[it.f];
-  const int n = fr.n, fs = fr.fs;
-  const int kbeg = schur ? 0 : c0;
-  const int kend = schur ? fs : min(c0 + nb, fs);
-  const int i0 = it.a, j0 = it.b;
-  const int i1 = min(i0 + TILE, i0 < fs ? fs : n);
-  const int j1 = min(j0 + TILE, j0 < fs ? fs : n);
-  double* __restrict__ M = arena + fr.off;
-  __shared__ double sA[KC][LDK];
-  __shared__ double sB[KC][LDK];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int iw = (w & 1) * 32, jw = (w >> 1) * 32;
-  d4 acc[2][2];
+  uhmd::tri_solve64<true, false>(x, s_T, LDP, s_rd, kb, lane);
+  if (t < width)
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
-
-  for (int k0 = kbeg; k0 < kend; k0 += KC) {
-    {
-      const int i = tid & 63, kq = (tid >> 6) * 4;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int k = kq + u;
-        sA[k][i] = (i0 + i < i1 && k0 + k < kend) ? M[(i0 + i) + (long long)(k0 + k) * n] : 0.0;
-      }
-    }
-    if (TRANSB) {
-      const int j = tid & 63, kq = (tid >> 6) * 4;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int k = kq + u;
-        sB[k][j] = (j0 + j < j1 && k0 + k < kend) ? M[(j0 + j) + (long long)(k0 + k) * n] : 0.0;
-      }
-    } else {
-      const int k = tid & 15, jq = tid >> 4;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int j = jq + 16 * u;
-        sB[k][j] = (j0 + j < j1 && k0 + k < kend) ? M[(k0 + k) + (long long)(j0 + j) * n] : 0.0;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < KC; kk += 4) {
-      const int kr = kk + (lane >> 4), cl = lane & 15;
-      double bo[2], ao[2];
-#pragma unroll
-      for (int b = 0; b < 2; ++b) bo[b] = sB[kr][jw + b * 16 + cl];
-#pragma unroll
-      for (int a = 0; a < 2; ++a) ao[a] = sA[kr][iw + a * 16 + cl];
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bo[b], ao[a], acc[a][b], 0, 0, 0);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int i = i0 + iw + a * 16 + (lane & 15);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        // f64 16x16x4: accumulator register r holds row 4 r + lane / 16
-        // of the 16 x 16 block (column lane % 16).
-        const int j = j0 + jw + b * 16 + 4 * r + (lane >> 4);
-        if (i < i1 && j < j1) M[i + (long long)j * n] -= acc[a][b][r];
-      }
+    for (int cr = 0; cr < 16; ++cr) {
+      const int k = 16 * q + cr;
+      if (k < kb) A[(c0 + k) + (long long)(j0 + t) * n] = x[cr];
     }
 }
 
@@ -684,32 +511,6 @@ int uhmk_lu_rows(const uhmk_front* fronts, const uhmk_item* items, int n_items, 
   if (n_items <= 0) return 0;
   if (nb > TILE || nb < 1) return -1000;
   hipLaunchKernelGGL(k_lu_rows, dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb, arena, piv);
-  return ret(hipGetLastError());
-}
-
-int uhmk_chol_diag(const uhmk_front* fronts, const int* flist, int n_fronts, int c0, int nb, double* arena,
-                   int* info, hipStream_t stream) {
-  if (n_fronts <= 0) return 0;
-  if (nb > TILE || nb < 1) return -1000;
-  hipLaunchKernelGGL(k_chol_diag, dim3(n_fronts), dim3(64), 0, stream, fronts, flist, c0, nb, arena, info);
-  return ret(hipGetLastError());
-}
-
-int uhmk_chol_trsm(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb,
-                   double* arena, hipStream_t stream) {
-  if (n_items <= 0) return 0;
-  if (nb > TILE || nb < 1) return -1000;
-  hipLaunchKernelGGL(k_chol_trsm, dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb, arena);
-  return ret(hipGetLastError());
-}
-
-int uhmk_gemm_update(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb,
-                     int schur, int transb, double* arena, hipStream_t stream) {
-  if (n_items <= 0) return 0;
-  if (transb)
-    hipLaunchKernelGGL(k_gemm_f64<true>, dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb, schur, arena);
-  else
-    hipLaunchKernelGGL(k_gemm_f64<false>, dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb, schur, arena);
   return ret(hipGetLastError());
 }
 

@@ -322,29 +322,35 @@ __global__ __launch_bounds__(256) void k_residual(int nrow, const long long* __r
                                                   const double* __restrict__ xg,
                                                   const double* __restrict__ bg, double* __restrict__ r,
                                                   double* __restrict__ out) {
-  const int wpb = 4;
-  const int row = blockIdx.x * wpb + (threadIdx.x >> 6);
+  // one wavefront per row, grid-stride over rows; maxima kept per wave and
+  // published with one atomic each at the end
   const int lane = threadIdx.x & 63;
-  double ax = 0.0, an = 0.0;
-  if (row < nrow) {
+  const int nwaves = gridDim.x * 4;
+  double mr = 0.0, ma = 0.0, mx = 0.0, mb = 0.0;
+  for (int row = blockIdx.x * 4 + (threadIdx.x >> 6); row < nrow; row += nwaves) {
+    double ax = 0.0, an = 0.0;
     for (long long p = rowptr[row] + lane; p < rowptr[row + 1]; p += 64) {
       const double a = val[p];
       ax += a * xg[colidx[p]];
       an += fabs(a);
     }
-  }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    ax += __shfl_xor(ax, o);
-    an += __shfl_xor(an, o);
-  }
-  if (row < nrow && lane == 0) {
+    for (int o = 32; o > 0; o >>= 1) {
+      ax += __shfl_xor(ax, o);
+      an += __shfl_xor(an, o);
+    }
     const double ri = bg[row] - ax;
-    r[row] = ri;
-    atomic_max_nonneg(out + 0, fabs(ri));
-    atomic_max_nonneg(out + 1, an);
-    atomic_max_nonneg(out + 2, fabs(xg[row]));
-    atomic_max_nonneg(out + 3, fabs(bg[row]));
+    if (lane == 0) r[row] = ri;
+    mr = fmax(mr, fabs(ri));
+    ma = fmax(ma, an);
+    mx = fmax(mx, fabs(xg[row]));
+    mb = fmax(mb, fabs(bg[row]));
+  }
+  if (lane == 0) {
+    atomic_max_nonneg(out + 0, mr);
+    atomic_max_nonneg(out + 1, ma);
+    atomic_max_nonneg(out + 2, mx);
+    atomic_max_nonneg(out + 3, mb);
   }
 }
 
@@ -438,8 +444,8 @@ int uhmk_generate_elements(const uhmk_front* fronts, const uhmk_item* items, int
 int uhmk_residual(int nrow, const long long* rowptr, const int* colidx, const double* val, const double* xg,
                   const double* bg, double* r, double* out4, hipStream_t stream) {
   if (nrow <= 0) return 0;
-  hipLaunchKernelGGL(k_residual, dim3((nrow + 3) / 4), dim3(256), 0, stream, nrow, rowptr, colidx, val, xg, bg,
-                     r, out4);
+  const int blocks = (nrow + 3) / 4 < 2048 ? (nrow + 3) / 4 : 2048;
+  hipLaunchKernelGGL(k_residual, dim3(blocks), dim3(256), 0, stream, nrow, rowptr, colidx, val, xg, bg, r, out4);
   return ret(hipGetLastError());
 }
 

@@ -293,10 +293,18 @@ void FrontSolver::generate(unsigned long long seed, double alpha, double beta, i
 }
 
 // ------------------------------------------------------------ schedule
-static void tile_starts(int a, int b, int cut, std::vector<int>& out) {
+static void tile_starts(int a, int b, int cut, int T, std::vector<int>& out) {
   out.clear();
-  for (int s = a; s < std::min(b, cut); s += 64) out.push_back(s);
-  for (int s = std::max(a, cut); s < b; s += 64) out.push_back(s);
+  for (int s = a; s < std::min(b, cut); s += T) out.push_back(s);
+  for (int s = std::max(a, cut); s < b; s += T) out.push_back(s);
+}
+
+// GEMM tile edge for a level: 128 x 128 tiles once the level's fronts are
+// large enough to give several tiles per front, 64 x 64 below.
+static int gemm_tile_for(const std::vector<uhmk_front>& fr, const std::vector<int>& lv) {
+  int maxn = 0;
+  for (int f : lv) maxn = std::max(maxn, fr[f].n);
+  return maxn >= 768 ? 128 : 64;
 }
 
 void FrontSolver::build_schedule(int method) {
@@ -306,9 +314,10 @@ void FrontSolver::build_schedule(int method) {
   launches_.clear();
   const int nb = nb_;
   std::vector<int> R, C;
-  auto tile_end = [](int s, int fs, int n) { return std::min(s + 64, s < fs ? fs : n); };
+  auto tile_end = [](int s, int fs, int n, int T) { return std::min(s + T, s < fs ? fs : n); };
   for (int L = 0; L < (int)levels_.size(); ++L) {
     const auto& lv = levels_[L];
+    const int T = gemm_tile_for(fr_, lv);
     Launch la;
     la.kind = Launch::ASSEMBLE;
     la.level = L;
@@ -370,16 +379,17 @@ void FrontSolver::build_schedule(int method) {
       lg.level = L;
       lg.kind = Launch::GEMM;
       lg.extra = chol ? 1 : 0;
+      lg.tile = T;
       lg.off = (long long)items_.size();
       for (int f : act) {
         const int fs = fr_[f].fs, n = fr_[f].n;
         const int c1 = std::min(c0 + nb, fs);
-        tile_starts(c1, n, fs, R);
-        tile_starts(c1, chol ? fs : n, fs, C);
+        tile_starts(c1, n, fs, T, R);
+        tile_starts(c1, chol ? fs : n, fs, T, C);
         for (int j0 : C)
           for (int i0 : R) {
             if (!chol && i0 >= fs && j0 >= fs) continue;
-            if (chol && tile_end(i0, fs, n) - 1 < j0) continue;
+            if (chol && tile_end(i0, fs, n, T) - 1 < j0) continue;
             items_.push_back(uhmk_item{f, i0, j0, 0});
           }
       }
@@ -390,13 +400,14 @@ void FrontSolver::build_schedule(int method) {
     ls.kind = Launch::SCHUR;
     ls.level = L;
     ls.extra = chol ? 1 : 0;
+    ls.tile = T;
     ls.off = (long long)items_.size();
     for (int f : lv) {
       const int fs = fr_[f].fs, n = fr_[f].n;
       if (!fs || n == fs) continue;
-      for (int j0 = fs; j0 < n; j0 += 64)
-        for (int i0 = fs; i0 < n; i0 += 64) {
-          if (chol && std::min(i0 + 64, n) - 1 < j0) continue;
+      for (int j0 = fs; j0 < n; j0 += T)
+        for (int i0 = fs; i0 < n; i0 += T) {
+          if (chol && std::min(i0 + T, n) - 1 < j0) continue;
           items_.push_back(uhmk_item{f, i0, j0, 0});
         }
     }
@@ -456,7 +467,7 @@ void FrontSolver::build_schedule(int method) {
           const uhmk_item& it = items_[l.off + q];
           const int fs = fr_[it.f].fs, n = fr_[it.f].n;
           const double K = (l.kind == Launch::SCHUR) ? fs : std::min(l.c0 + l.nb, fs) - l.c0;
-          const int i1 = tile_end(it.a, fs, n), j1 = tile_end(it.b, fs, n);
+          const int i1 = tile_end(it.a, fs, n, l.tile), j1 = tile_end(it.b, fs, n, l.tile);
           double cnt = 0;
           if (chol)
             for (int j = it.b; j < j1; ++j) cnt += std::max(0, i1 - std::max(it.a, j));
@@ -521,10 +532,10 @@ int FrontSolver::factor(int method) {
         rc = uhmk_chol_trsm(F, IT + l.off, l.count, l.c0, l.nb, A, stream_);
         break;
       case Launch::GEMM:
-        rc = uhmk_gemm_update(F, IT + l.off, l.count, l.c0, l.nb, 0, l.extra, A, stream_);
+        rc = uhmk_gemm_update(F, IT + l.off, l.count, l.c0, l.nb, 0, l.extra, l.tile, A, stream_);
         break;
       case Launch::SCHUR:
-        rc = uhmk_gemm_update(F, IT + l.off, l.count, 0, 0, 1, l.extra, A, stream_);
+        rc = uhmk_gemm_update(F, IT + l.off, l.count, 0, 0, 1, l.extra, l.tile, A, stream_);
         break;
     }
     check_k(rc, "factor");

@@ -47,6 +47,7 @@ struct Launch {
   int count = 0;
   int extra = 0;      // max pivot rows (panel), transb (gemm / schur)
   int level = 0;
+  int tile = 64;      // gemm / schur tile edge
   double flops = 0;   // algorithmic flops of the launch
   double bytes = 0;   // algorithmic HBM bytes of the launch
 };
