@@ -5,35 +5,118 @@
 
 namespace uhmd {
 
-// Triangular solve on 64-long vectors, 16 vectors per wavefront: lane
-// l = v + 16 q holds entries [16 q, 16 q + 16) of vector v in x[0..16).
-// For k = 0 .. kb-1 in order:
+// Broadcast of a double from lane (l & ~3) + KQ to the four lanes of its
+// quad (DPP quad_perm, no LDS round trip).
+template <int KQ>
+__device__ __forceinline__ double quad_bcast(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), KQ * 85, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), KQ * 85, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Value of v in lane L (uniform result, v_readlane).
+template <int L>
+__device__ __forceinline__ double readlane_d(double v) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), L),
+                          __builtin_amdgcn_readlane(__double2loint(v), L));
+}
+
+// 1 / sqrt(d) from v_rsq_f64 refined by two Newton steps (full FP64
+// accuracy for positive normal d; NaN / Inf otherwise).
+__device__ __forceinline__ double rsqrt_d(double d) {
+  double r = __builtin_amdgcn_rsq(d);
+  const double h = 0.5 * d;
+  r = r * fma(-h * r, r, 1.5);
+  r = r * fma(-h * r, r, 1.5);
+  return r;
+}
+
+// Triangular solve on 64-long vectors, 16 vectors per wavefront, quad
+// layout: lane l = 4 v + q holds entries [16 q, 16 q + 16) of vector v in
+// x[0..16).  For k = 0 .. kb-1 in order:
 //     x_k <- x_k * rdiag[k]                 (skipped when UNIT)
 //     x_c <- x_c - x_k * T(c, k)   for k < c < kb
 // with T(c, k) = s[c * ld + k] (TRANS = false) or s[k * ld + c] (TRANS = true)
-// read from LDS.  This is the right-looking (column) form of:
-//   * X <- X L^-T          rows of X, T = L   (Cholesky panel rows)
-//   * X <- X U^-1          rows of X, T = U^T (LU rows of ABL: TRANS)
-//   * x <- L^-1 x, unit    columns,   T = L   (LU U strip)
-// Every vector is independent; the owner of x_k broadcasts it to the other
-// three lanes of its vector with a shuffle, so no barrier is needed.
-template <bool UNIT, bool TRANS>
-__device__ inline void tri_solve64(double (&x)[16], const double* __restrict__ s, int ld,
-                                   const double* __restrict__ rdiag, int kb, int lane) {
-  const int q = lane >> 4, v = lane & 15;
-#pragma unroll
-  for (int k = 0; k < 64; ++k) {
-    if (k < kb) {
-      const int kq = k >> 4, kr = k & 15;
-      if (!UNIT && q == kq) x[kr] *= rdiag[k];
-      const double xk = __shfl(x[kr], v + 16 * kq);
+// read from LDS.  This is the right-looking (column) form of
+//   * X <- X L^-T          rows of X, T = L    (Cholesky panel rows)
+//   * X <- X U^-1          rows of X, T = U^T  (LU rows of ABL, TRANS)
+//   * x <- L^-1 x, unit    columns,   T = L    (LU U strip)
+// Each vector is independent; the owner of x_k broadcasts it inside its
+// quad with DPP.  The 64 steps are expanded at compile time so every
+// register index is static.  The caller stores T with zeros outside the
+// strict triangle that is used (and outside kb), so every update is an
+// unconditional FMA.
+template <int K, bool UNIT, bool TRANS, bool FULL>
+struct TriStep {
+  __device__ __forceinline__ static void run(double (&x)[16], const double* __restrict__ s, int ld,
+                                             const double* __restrict__ rdiag, int kb, int q) {
+    if (FULL || K < kb) {
+      constexpr int KQ = K >> 4, KR = K & 15;
+      if (!UNIT) {
+        const double xo = x[KR];
+        x[KR] = (q == KQ) ? xo * rdiag[K] : xo;
+      }
+      const double xk = quad_bcast<KQ>(x[KR]);
 #pragma unroll
       for (int cr = 0; cr < 16; ++cr) {
         const int c = 16 * q + cr;
-        if (c > k && c < kb) x[cr] -= xk * (TRANS ? s[k * ld + c] : s[c * ld + k]);
+        x[cr] = fma(-xk, TRANS ? s[K * ld + c] : s[c * ld + K], x[cr]);
       }
+      TriStep<K + 1, UNIT, TRANS, FULL>::run(x, s, ld, rdiag, kb, q);
     }
   }
+};
+template <bool UNIT, bool TRANS, bool FULL>
+struct TriStep<64, UNIT, TRANS, FULL> {
+  __device__ __forceinline__ static void run(double (&)[16], const double*, int, const double*, int, int) {}
+};
+
+// s must hold T(c, k) for k < c < kb and zero everywhere else in the
+// 64 x 64 block (the diagonal lives in rdiag).
+template <bool UNIT, bool TRANS>
+__device__ __forceinline__ void tri_solve64(double (&x)[16], const double* __restrict__ s, int ld,
+                                            const double* __restrict__ rdiag, int kb, int lane) {
+  if (kb == 64) TriStep<0, UNIT, TRANS, true>::run(x, s, ld, rdiag, kb, lane & 3);
+  else TriStep<0, UNIT, TRANS, false>::run(x, s, ld, rdiag, kb, lane & 3);
 }
+
+// Fill a 64 x 64 LDS block (row stride ld) with the strict lower (LOWER)
+// or strict upper triangle of the kb x kb block at A (leading dimension n),
+// zeros elsewhere, and rdiag[k] = 1 / A[k][k] (k < kb).
+template <bool LOWER>
+__device__ __forceinline__ void load_tri64(double* __restrict__ s, int ld, double* __restrict__ rdiag,
+                                           const double* __restrict__ A, long long n, int kb, int tid, int nthreads) {
+  for (int idx = tid; idx < 64 * 64; idx += nthreads) {
+    const int r = idx & 63, c = idx >> 6;
+    const bool in = r < kb && c < kb && (LOWER ? r > c : r < c);
+    s[r * ld + c] = in ? A[r + c * n] : 0.0;
+    if (r == c && r < kb) rdiag[r] = 1.0 / A[r + c * n];
+  }
+}
+
+// Right-looking Cholesky of a 64 x 64 block held one row per lane
+// (a[j] = A[i][j], lane i).  Column k is read from the other lanes with
+// v_readlane; updates run on the whole row (entries above the diagonal are
+// scratch and never stored).  first <- k + 1 of the first non-positive pivot.
+template <int K>
+struct CholStep {
+  __device__ __forceinline__ static void run(double (&a)[64], int i, int kb, int& first) {
+    if (K < kb) {
+      const double d = readlane_d<K>(a[K]);
+      if (!(d > 0.0) && first == 0) first = K + 1;
+      const double r = rsqrt_d(d);
+      a[K] = (i == K) ? d * r : a[K] * r;
+#pragma unroll
+      for (int j = K + 1; j < 64; ++j) a[j] = fma(-a[K], __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(a[K]), j),
+                                                                          __builtin_amdgcn_readlane(__double2loint(a[K]), j)),
+                                                  a[j]);
+      CholStep<K + 1>::run(a, i, kb, first);
+    }
+  }
+};
+template <>
+struct CholStep<64> {
+  __device__ __forceinline__ static void run(double (&)[64], int, int, int&) {}
+};
 
 }  // namespace uhmd

@@ -18,9 +18,8 @@ constexpr int TILE = 64;
 constexpr int LDP = TILE + 1;
 
 // ------------------------------------------------------------------------
-// Cholesky of the kb x kb diagonal panel block, one wavefront per front.
-// Lane i holds row i of the block in registers; column k is broadcast
-// through LDS after it is scaled (right-looking, unblocked).
+// Cholesky of the kb x kb diagonal panel block, one wavefront per front:
+// lane i holds row i in registers (uhmd::CholStep, fully unrolled).
 // ------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_chol_diag(const uhmk_front* __restrict__ F,
                                                   const int* __restrict__ flist, int c0, int nb,
@@ -30,27 +29,13 @@ __global__ __launch_bounds__(64) void k_chol_diag(const uhmk_front* __restrict__
   const int n = fr.n, fs = fr.fs;
   const int kb = min(nb, fs - c0);
   double* __restrict__ A = arena + fr.off;
-  __shared__ double s_col[TILE];
   const int i = threadIdx.x;
   double a[TILE];
 #pragma unroll
   for (int j = 0; j < TILE; ++j) a[j] = (i < kb && j < kb && j <= i) ? A[(c0 + i) + (long long)(c0 + j) * n] : 0.0;
-#pragma unroll
-  for (int k = 0; k < TILE; ++k) {
-    if (k < kb) {
-      const double d = __shfl(a[k], k);
-      if (i == 0 && !(d > 0.0) && info[f] == 0) info[f] = c0 + k + 1;
-      const double l = sqrt(d);
-      if (i == k) a[k] = l;
-      else if (i > k) a[k] = a[k] / l;
-      s_col[i] = a[k];
-      __syncthreads();
-#pragma unroll
-      for (int j = k + 1; j < TILE; ++j)
-        if (j <= i && j < kb) a[j] -= a[k] * s_col[j];
-      __syncthreads();
-    }
-  }
+  int first = 0;
+  uhmd::CholStep<0>::run(a, i, kb, first);
+  if (i == 0 && first && info[f] == 0) info[f] = c0 + first;
 #pragma unroll
   for (int j = 0; j < TILE; ++j)
     if (i < kb && j < kb && j <= i) A[(c0 + i) + (long long)(c0 + j) * n] = a[j];
@@ -69,14 +54,9 @@ __global__ __launch_bounds__(256) void k_chol_trsm(const uhmk_front* __restrict_
   __shared__ double s_T[TILE * LDP];
   __shared__ double s_rd[TILE];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  for (int idx = tid; idx < kb * kb; idx += 256) {
-    const int r = idx % kb, c = idx / kb;
-    s_T[r * LDP + c] = A[(c0 + r) + (long long)(c0 + c) * n];
-  }
+  uhmd::load_tri64<true>(s_T, LDP, s_rd, A + c0 + (long long)c0 * n, n, kb, tid, 256);
   __syncthreads();
-  if (tid < kb) s_rd[tid] = 1.0 / s_T[tid * LDP + tid];
-  __syncthreads();
-  const int row = it.b + 16 * w + (lane & 15), q = lane >> 4;
+  const int row = it.b + 16 * w + (lane >> 2), q = lane & 3;
   const bool live = row < n;
   double x[16];
 #pragma unroll

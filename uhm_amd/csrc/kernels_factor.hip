@@ -397,7 +397,7 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
   const int n = fr.n, fs = fr.fs;
   const int kb = min(nb, fs - c0);
   double* __restrict__ A = arena + fr.off;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, q = lane >> 4;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, q = lane & 3;
   __shared__ double s_X[2 * TILE][TILE];  // rows touched by the swaps, per column
   __shared__ double s_T[TILE * LDP];      // triangular panel block
   __shared__ double s_rd[TILE];
@@ -405,14 +405,9 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
   __shared__ int s_cnt;
 
   if (it.a == 1) {
-    for (int idx = tid; idx < kb * kb; idx += 256) {
-      const int r = idx % kb, c = idx / kb;
-      s_T[r * LDP + c] = A[(c0 + r) + (long long)(c0 + c) * n];
-    }
+    uhmd::load_tri64<false>(s_T, LDP, s_rd, A + c0 + (long long)c0 * n, n, kb, tid, 256);
     __syncthreads();
-    if (tid < kb) s_rd[tid] = 1.0 / s_T[tid * LDP + tid];
-    __syncthreads();
-    const int row = it.b + 16 * w + (lane & 15);
+    const int row = it.b + 16 * w + (lane >> 2);
     const bool live = row < n;
     double x[16];
 #pragma unroll
@@ -455,12 +450,9 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
     const int k = kb + idx % (pc - kb), t = idx / (pc - kb);
     A[s_prow[k] + (long long)(j0 + t) * n] = s_X[s_psrc[k]][t];
   }
-  for (int idx = tid; idx < kb * kb; idx += 256) {
-    const int r = idx % kb, c = idx / kb;
-    s_T[r * LDP + c] = A[(c0 + r) + (long long)(c0 + c) * n];
-  }
+  uhmd::load_tri64<true>(s_T, LDP, s_rd, A + c0 + (long long)c0 * n, n, kb, tid, 256);
   __syncthreads();
-  const int t = 16 * w + (lane & 15);
+  const int t = 16 * w + (lane >> 2);
   double x[16];
 #pragma unroll
   for (int cr = 0; cr < 16; ++cr) {

@@ -184,6 +184,7 @@ __global__ __launch_bounds__(256) void k_solve_bwd_chol(const uhmk_front* __rest
   const double* __restrict__ A = arena + fr.off;
   for (int i = wid; i < fs; i += 4) {
     double acc = 0.0;
+#pragma unroll 8
     for (int k = lane; k < ss; k += 64) acc += A[(fs + k) + (long long)i * n] * sx[fs + k];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
@@ -195,6 +196,7 @@ __global__ __launch_bounds__(256) void k_solve_bwd_chol(const uhmk_front* __rest
     const int k0 = bk * BLK, kb = min(BLK, fs - k0), k1 = k0 + kb;
     for (int c = wid; c < kb; c += 4) {
       double acc = 0.0;
+#pragma unroll 8
       for (int i = k1 + lane; i < fs; i += 64) acc += A[i + (long long)(k0 + c) * n] * sx[i];
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);

@@ -299,12 +299,11 @@ static void tile_starts(int a, int b, int cut, int T, std::vector<int>& out) {
   for (int s = std::max(a, cut); s < b; s += T) out.push_back(s);
 }
 
-// GEMM tile edge for a level: 128 x 128 tiles once the level's fronts are
-// large enough to give several tiles per front, 64 x 64 below.
-static int gemm_tile_for(const std::vector<uhmk_front>& fr, const std::vector<int>& lv) {
-  int maxn = 0;
-  for (int f : lv) maxn = std::max(maxn, fr[f].n);
-  return maxn >= 768 ? 128 : 64;
+// GEMM tile edge of one launch from the mean extent of the updated blocks
+// (rows x columns): 128 x 128 tiles when both average at least 256, so the
+// ragged edge tiles stay a small share, 64 x 64 otherwise.
+static int gemm_tile_for(double mean_rows, double mean_cols) {
+  return (mean_rows >= 256.0 && mean_cols >= 256.0) ? 128 : 64;
 }
 
 void FrontSolver::build_schedule(int method) {
@@ -317,7 +316,6 @@ void FrontSolver::build_schedule(int method) {
   auto tile_end = [](int s, int fs, int n, int T) { return std::min(s + T, s < fs ? fs : n); };
   for (int L = 0; L < (int)levels_.size(); ++L) {
     const auto& lv = levels_[L];
-    const int T = gemm_tile_for(fr_, lv);
     Launch la;
     la.kind = Launch::ASSEMBLE;
     la.level = L;
@@ -379,7 +377,16 @@ void FrontSolver::build_schedule(int method) {
       lg.level = L;
       lg.kind = Launch::GEMM;
       lg.extra = chol ? 1 : 0;
-      lg.tile = T;
+      {
+        double sr = 0, sc = 0;
+        for (int f : act) {
+          const int c1 = std::min(c0 + nb, fr_[f].fs);
+          sr += fr_[f].n - c1;
+          sc += (chol ? fr_[f].fs : fr_[f].n) - c1;
+        }
+        lg.tile = gemm_tile_for(sr / act.size(), sc / act.size());
+      }
+      const int T = lg.tile;
       lg.off = (long long)items_.size();
       for (int f : act) {
         const int fs = fr_[f].fs, n = fr_[f].n;
@@ -400,7 +407,17 @@ void FrontSolver::build_schedule(int method) {
     ls.kind = Launch::SCHUR;
     ls.level = L;
     ls.extra = chol ? 1 : 0;
-    ls.tile = T;
+    {
+      double sss = 0;
+      int cnt = 0;
+      for (int f : lv)
+        if (fr_[f].fs && fr_[f].n > fr_[f].fs) {
+          sss += fr_[f].n - fr_[f].fs;
+          ++cnt;
+        }
+      ls.tile = cnt ? gemm_tile_for(sss / cnt, sss / cnt) : 64;
+    }
+    const int T = ls.tile;
     ls.off = (long long)items_.size();
     for (int f : lv) {
       const int fs = fr_[f].fs, n = fr_[f].n;
