@@ -26,8 +26,18 @@
  * merges the b part during factorization and the update during the solve,
  * which is the same sum in another order.
  *
- * Parity status of each part is stated in DESIGN.md and pinned by the
- * tests in tests/test_oracle.py.
+ * This file is test infrastructure: only the tests, the smoke check and the
+ * CPU-baseline leg of the benchmark load it.  Parity status, each pinned by
+ * tests/test_oracle.py:
+ *   front LU        LAPACK getrf golden factors and pivots (scipy, committed)
+ *   front Cholesky  numpy.linalg.cholesky golden factors (committed)
+ *   lock / levels   independent lowest-common-ancestor formulation, hand-
+ *                   derived maps of the two-element example mesh
+ *   mapper          run-partition invariants, hand-derived example maps
+ *   solve           dense numpy.linalg.solve of the assembled matrix (committed)
+ *   flops           LAWN-41 closed forms
+ * Not pinnable here: libflame's blocked-panel rounding (the reference's
+ * tests store no values); the restatement follows LAPACK's unblocked GEPP.
  */
 #include <math.h>
 #include <stdlib.h>

@@ -183,7 +183,7 @@ def test_oracle_solution_matches_dense_solve(oracle_lib, N, method):
 
 
 def test_oracle_flops_closed_form(oracle_lib):
-    # cfg2 structure: 3D Q3 on 16^3 -> 177.777 GFLOP (LU); cfg1 Cholesky
+    # LAWN-41 closed forms summed over an independently computed lock of 3D Q3 on 4^3
     spec = qp_spec((4, 4, 4), 3)
     om = spec.oracle()
     res, _, _ = lca_lock(spec)
