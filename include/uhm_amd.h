@@ -149,6 +149,26 @@ void uhm_get_timing_(uhm_fort_p* mesh, uhm_fort_double* factor_ms, uhm_fort_doub
 void uhm_set_profile_(uhm_fort_p* mesh, uhm_fort_int* on);
 void uhm_get_kernel_stats_(uhm_fort_p* mesh, uhm_fort_double* ms, uhm_fort_double* flops, uhm_fort_double* bytes,
                            uhm_fort_int* launches);
+/* Precision of the fronts (call after uhm_create_matrix_without_buffer_,
+ * before factorizing).  mode 0: every front FP64 with FP64 MFMA updates
+ * (default).  mode 1: mixed -- the fronts of the top `top_levels` tree
+ * levels stay FP64 (their pivot blocks factorize in FP64), all other fronts
+ * are stored in FP32, and every Schur / trailing update is an FP32 MFMA
+ * product; the solution is then brought to FP64 accuracy with
+ * uhm_improve_*_. */
+void uhm_set_precision_(uhm_fort_p* mesh, uhm_fort_int* mode, uhm_fort_int* top_levels);
+/* Iterative refinement after uhm_solve_*_: x <- x + M^-1 (b - A x), the
+ * residual in FP64 against the assembled matrix, M the last factorization,
+ * until the normwise backward error of every right-hand side is <= tol or
+ * max_iter corrections were applied; n_iter = corrections applied.  Fills
+ * the slot of Mesh_::improve_chol / improve_lu_piv
+ * (UHM/src/mesh/chol.cxx:136-149, UHM/src/mesh/lu_piv.cxx:232-245), whose
+ * reference bodies are stubs. */
+void uhm_improve_chol_(uhm_fort_p* mesh, uhm_fort_double* tol, uhm_fort_int* max_iter, uhm_fort_int* n_iter);
+void uhm_improve_lu_piv_(uhm_fort_p* mesh, uhm_fort_double* tol, uhm_fort_int* max_iter, uhm_fort_int* n_iter);
+/* Backward error before each correction of the last refinement (n in:
+ * capacity of eta, out: number of entries) and its wall time in ms. */
+void uhm_get_refine_history_(uhm_fort_p* mesh, uhm_fort_double* eta, uhm_fort_int* n, uhm_fort_double* ms);
 /* Last error message (empty when none); returns its length. */
 int uhm_get_last_error_(char* buf, int len);
 

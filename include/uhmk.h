@@ -11,7 +11,11 @@
  * level (a "batch"), bucketed by work item.
  *
  * Front layout (shared by every kernel): front f is one column-major
- * n x n matrix, n = fs + ss, leading dimension n, stored at arena + off.
+ * n x n matrix, n = fs + ss, leading dimension n, stored at arena + off in
+ * the FP64 arena (prec 0) or the FP32 arena (prec 1); the fronts of one
+ * tree level share a storage type.  Kernels that take `void* arena, int
+ * prec` read and write that level's arena; arithmetic outside the MFMA
+ * products is FP64.
  * Its quadrants are the reference's ATL (fs x fs), ABL (ss x fs),
  * ATR (fs x ss) and ABR (ss x ss) (UHM/src/matrix/uhm/fla/matrix.cxx:75-91).
  * Pivots are int32, libflame's relative format: p[k] = pivot row - k.
@@ -35,6 +39,8 @@ typedef struct uhmk_front {
   int parent;      /* parent front index, -1 for a root                   */
   int emap;        /* offset of the ss-long map child schur -> parent idx */
   int id;          /* element id, reported on a zero pivot / non-SPD      */
+  int prec;        /* storage of the front: 0 FP64 arena, 1 FP32 arena   */
+  int rsv;         /* reserved, 0                                         */
 } uhmk_front;
 
 /* Assembly source of a front: its own stage (src = -1) or the ABR block of
@@ -58,56 +64,89 @@ typedef struct uhmk_item {
  * UHM/include/uhm/matrix/uhm/helper.hxx:191-222,
  * UHM/src/matrix/uhm/fla/matrix.cxx:284-303).  items: {f, row tile, col tile}. */
 int uhmk_assemble(const uhmk_front* fronts, const uhmk_item* items, int n_items, const int* src_beg,
-                  const uhmk_source* srcs, const int* inv, double* arena, const double* stage,
-                  hipStream_t stream);
+                  const uhmk_source* srcs, const int* inv, double* arena64, float* arena32, int prec,
+                  const double* stage, hipStream_t stream);
 
 /* Panel LU with partial pivoting restricted to ATL rows [c0, fs) for the
  * fronts in flist; writes relative pivots.  Replaces FLA_LU_piv on the
  * panel inside linal::dense::lu_piv (LINAL/src/dense/lu_piv.cxx:79).
  * info[f] <- first zero-pivot row + 1. */
 int uhmk_lu_panel(const uhmk_front* fronts, const int* flist, int n_fronts, int max_rows, int c0, int nb,
-                  double* arena, int* piv, int* info, hipStream_t stream);
+                  void* arena, int prec, int* piv, int* info, hipStream_t stream);
 
 /* Row swaps outside the panel (apply_pivots, LINAL/src/dense/pivot.cxx:46-91),
  * U12 = L11^-1 A12 (trsm left lower unit) and L21 = A21 U11^-1 on the ABL
  * rows (trsm right upper, UHM/src/matrix/uhm/fla/lu_piv/decompose.cxx:136-146).
  * items: {f, kind (0 left cols, 1 ABL rows, 2 right cols), start, 0}. */
-int uhmk_lu_rows(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb,
-                 double* arena, const int* piv, hipStream_t stream);
+int uhmk_lu_rows(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, void* arena,
+                 int prec, const int* piv, hipStream_t stream);
 
 /* Cholesky of the diagonal panel block (FLA_Chol in linal::dense::chol,
  * LINAL/src/dense/chol.cxx:67).  info[f] <- first non-positive pivot + 1. */
-int uhmk_chol_diag(const uhmk_front* fronts, const int* flist, int n_fronts, int c0, int nb, double* arena,
+int uhmk_chol_diag(const uhmk_front* fronts, const int* flist, int n_fronts, int c0, int nb, void* arena, int prec,
                    int* info, hipStream_t stream);
 
 /* Panel rows below the diagonal block: X <- X L11^-T (trsm right lower
  * transpose, LINAL/src/dense/chol.cxx:80-84).  items: {f, 0, first row, 0}. */
-int uhmk_chol_trsm(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb,
-                   double* arena, hipStream_t stream);
+int uhmk_chol_trsm(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, void* arena,
+                   int prec, hipStream_t stream);
 
-/* C -= A op(B) on tile x tile blocks (tile 64 or 128), FP64 MFMA: panel
+/* C -= A op(B) on tile x tile blocks (tile 64 or 128) on MFMA: panel
  * update (schur = 0, K = [c0, c0+nb)) or final Schur complement
- * ABR -= ABL ATR / ABL ABL^T (schur = 1, K = [0, fs)).  Replaces
- * linal::dense::gemm / syrk (LINAL/src/dense/gemm.cxx:48-137,
+ * ABR -= ABL ATR / ABL ABL^T (schur = 1, K = [0, fs)).  mma 0: FP64
+ * products (v_mfma_f64_16x16x4f64, FP64 fronts only); mma 1: FP32
+ * products and accumulation (v_mfma_f32_16x16x4f32) on FP32 or FP64
+ * fronts.  Replaces linal::dense::gemm / syrk (LINAL/src/dense/gemm.cxx:48-137,
  * syrk.cxx:43-123).  items: {f, i0, j0, 0}, tiles cut at fs. */
-int uhmk_gemm_update(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb,
-                     int schur, int transb, int tile, double* arena, hipStream_t stream);
+int uhmk_gemm_update(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, int schur,
+                     int transb, int tile, void* arena, int prec, int mma, hipStream_t stream);
 
 /* ---- solve ---------------------------------------------------------------- */
 
-/* Forward solve of one rhs column for one tree level: gather the children's
- * xb, pivots, L solve, xb -= ABL xt (op_solve level 1 + solve_*_1_hier,
- * UHM/src/operation/element.cxx:243-294, lu_piv/solve.cxx:161-187,
- * chol/solve.cxx:152-168). */
-int uhmk_solve_fwd(const uhmk_front* fronts, const int* flist, int n_fronts, int max_n, int chol,
-                   const int* src_beg, const uhmk_source* srcs, const int* inv, const double* arena,
-                   const int* piv, double* x, const double* b, int col, hipStream_t stream);
+/* Composed row permutation of each LU front (perm[i] = row of the
+ * unpivoted right-hand side that the relative pivots bring to position i,
+ * swaps k <-> k + p[k] in order k = 0 .. fs-1, as FLA_Apply_pivots does in
+ * solve_lu_piv_1_hier, UHM/src/matrix/uhm/fla/lu_piv/solve.cxx:161-187).
+ * Computed once per factorization; perm has the layout of piv. */
+int uhmk_lu_perm(const uhmk_front* fronts, const int* flist, int n_fronts, const int* piv, int* perm,
+                 hipStream_t stream);
 
-/* Backward solve of one rhs column for one tree level: branch the parent's
- * solution into xb, xt -= ATR xb (ABL^T xb), U (L^T) solve
- * (lu_piv/solve.cxx:211-231, chol/solve.cxx:186-202, helper.hxx:302-331). */
-int uhmk_solve_bwd(const uhmk_front* fronts, const int* flist, int n_fronts, int max_n, int chol,
-                   const int* emap, const double* arena, double* x, int col, hipStream_t stream);
+/* Forward-solve gather for one tree level: x_f = b_f + sum of the
+ * children's xb in merge order (Helper_::_merge_rhs, helper.hxx:253-301),
+ * the factor rows read through perm for LU.  items: {f, 0, first row, 0},
+ * 256 rows each; all n_rhs columns. */
+int uhmk_solve_gather(const uhmk_front* fronts, const uhmk_item* items, int n_items, int lu, const int* src_beg,
+                      const uhmk_source* srcs, const int* inv, const int* perm, double* x, const double* b, int nrhs,
+                      hipStream_t stream);
+
+/* Triangular solve of the diagonal super-block [sb * sb_size,
+ * min((sb + 1) * sb_size, fs)) of ATL for every front in flist (one
+ * workgroup per front; fronts with fs <= sb * sb_size are skipped;
+ * sb_size a multiple of 64, at most 1024): fwd = 1: x_t <- L^-1 x_t (unit
+ * lower for LU); fwd = 0: x_t <- U^-1 x_t (LU) or L^-T x_t (Cholesky).
+ * With sb_size >= fs this is the whole ATL solve; otherwise the couplings
+ * between super-blocks go through uhmk_solve_gemv.  arena holds the fronts
+ * in FP64 (prec 0) or FP32 (prec 1); arithmetic is FP64.  Replaces the
+ * triangular parts of solve_{lu_piv,chol}_{1,2}_hier
+ * (lu_piv/solve.cxx:161-231, chol/solve.cxx:152-202). */
+int uhmk_solve_trsv(const uhmk_front* fronts, const int* flist, int n_fronts, int max_fs, int chol, int fwd,
+                    int sb, int sb_size, const void* arena, int prec, double* x, int nrhs, hipStream_t stream);
+
+/* Off-diagonal solve updates.  cols = 0: x[rows] -= F[rows, a:c) x[a:c)
+ * over row chunks, items {f, a, first row, c}, rows [b, min(b + RT,
+ * b < fs ? fs : n)), RT = 16 (FP64) or 32 (FP32): xb -= ABL x_t, x_t -=
+ * ATR xb and super-block couplings.  cols = 1: x[col] -= sum over rows
+ * [a, c) of F(row, col) x[row] for columns [b, min(b + 16, fs)), items
+ * {f, a, first column, c}: x_t -= ABL^T xb and the Cholesky backward
+ * couplings. */
+int uhmk_solve_gemv(const uhmk_front* fronts, const uhmk_item* items, int n_items, int cols, const void* arena,
+                    int prec, double* x, int nrhs, hipStream_t stream);
+
+/* Backward branch: xb <- the parent's solution rows through emap
+ * (Helper_::_branch_rhs, helper.hxx:302-331).  items {f, 0, first row, 0},
+ * 256 rows each. */
+int uhmk_solve_branch(const uhmk_front* fronts, const uhmk_item* items, int n_items, const int* emap, double* x,
+                      int nrhs, hipStream_t stream);
 
 /* ---- inputs and residuals -------------------------------------------------- */
 
@@ -135,9 +174,10 @@ int uhmk_gather_rhs(long long nrow, const long long* bptr, const long long* bidx
 int uhmk_residual(int nrow, const long long* rowptr, const int* colidx, const double* val, const double* xg,
                   const double* bg, double* r, double* out4, hipStream_t stream);
 
-/* Copy between per-front solution blocks and a global vector (dir 0:
- * fronts -> global, 1: global -> fronts) over the factor rows; the analogue
- * of copy_out for the whole mesh (UHM/src/mesh/interf.cxx:183-199). */
+/* Between per-front solution blocks and a global vector over the factor
+ * rows: dir 0 g = xf (fronts -> global), dir 1 xf = g (global -> fronts),
+ * dir 2 g += xf (refinement update); the analogue of copy_out for the
+ * whole mesh (UHM/src/mesh/interf.cxx:183-199). */
 int uhmk_front_global(const uhmk_front* fronts, int n_fronts, const long long* goff, double* xf, double* g,
                       int col, long long nglob, int dir, hipStream_t stream);
 

@@ -52,5 +52,6 @@ class DeviceArray:
 
 
 FRONT_DTYPE = np.dtype([("off", "<i8"), ("xoff", "<i8"), ("soff", "<i8"), ("n", "<i4"), ("fs", "<i4"),
-                        ("poff", "<i4"), ("parent", "<i4"), ("emap", "<i4"), ("id", "<i4")])
-assert FRONT_DTYPE.itemsize == 48
+                        ("poff", "<i4"), ("parent", "<i4"), ("emap", "<i4"), ("id", "<i4"), ("prec", "<i4"),
+                        ("rsv", "<i4")])
+assert FRONT_DTYPE.itemsize == 56

@@ -63,10 +63,9 @@ def _compare_fronts(m, om, method):
     return worst
 
 
-def x_tolerance(m):
-    """Forward-error bar for two backward-stable FP64 solutions of the same
-    system: max(1e-12, 10 kappa_1(A) eps), kappa_1 estimated from the
-    assembled matrix (scipy splu + onenormest)."""
+def kappa_1(m):
+    """1-norm condition number of the assembled matrix (scipy splu +
+    onenormest)."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spl
     rp, col, val = m.fetch_csr()
@@ -74,7 +73,13 @@ def x_tolerance(m):
     A = sp.csr_matrix((val, col, rp), shape=(n, n)).tocsc()
     lu = spl.splu(A)
     op = spl.LinearOperator((n, n), matvec=lu.solve, rmatvec=lambda v: lu.solve(v, trans="T"), dtype=float)
-    kappa = spl.onenormest(A) * spl.onenormest(op)
+    return spl.onenormest(A) * spl.onenormest(op)
+
+
+def x_tolerance(m):
+    """Forward-error bar for two backward-stable FP64 solutions of the same
+    system: max(1e-12, 10 kappa_1(A) eps)."""
+    kappa = kappa_1(m)
     tol = max(X_TOL, 10.0 * kappa * np.finfo(float).eps)
     say("  kappa_1 ~ %.3e -> x tolerance %.2e" % (kappa, tol))
     return tol
@@ -133,6 +138,9 @@ CASES = [
     # fronts above 768 rows: 128-wide GEMM tiles and multi-panel pivoting
     ("q3-3d-chol-8cube", 2, (8, 8, 8), 3, U.UHM_CHOL, 64.0, 0.0, 1, 0.5, 0.0),
     ("q3-3d-lu-8cube", 2, (8, 8, 8), 3, U.UHM_LU_PIV, 64.0, 0.1, 1, 1.0, 0.0),
+    # root fronts above 1024 rows: super-block triangular solves
+    ("q4-3d-chol-8cube", 2, (8, 8, 8), 4, U.UHM_CHOL, 125.0, 0.0, 2, -2.0, 2.0),
+    ("q3-3d-lu-11cube", 2, (11, 11, 11), 3, U.UHM_LU_PIV, 64.0, 0.1, 1, 1.0, 0.0),
 ]
 
 
@@ -199,6 +207,40 @@ def test_solution_matches_oracle_device_rhs(lib, oracle_lib):
     # a second solve on the same factors reproduces the bits
     m.solve_lu_piv()
     assert np.array_equal(m.fetch_x()[:, 0], xg)
+
+
+@pytest.mark.parametrize("method", [U.UHM_LU_PIV, U.UHM_CHOL])
+def test_multiple_rhs_vs_oracle(lib, oracle_lib, method):
+    """Three right-hand sides solved together match the oracle column by
+    column, and each column's backward error is at FP64 level."""
+    m = U.Mesh()
+    m.build_synthetic(2, (5, 4, 4), 3, 0)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 3)
+    m.create_matrix_buffer()
+    m.set_random_params(seed=5, alpha=64.0, beta=0.1 if method == U.UHM_LU_PIV else 0.0, scale_kind=1, p0=0.5)
+    m.random_matrix()
+    m.set_rhs()
+    chol = method == U.UHM_CHOL
+    assert (m.chol_with_free() if chol else m.lu_piv_with_free()) == 0
+    (m.solve_chol if chol else m.solve_lu_piv)()
+    (m.check_chol if chol else m.check_lu_piv)()
+    assert m.backward_error() <= ETA_TOL
+    xg = m.fetch_x()
+    assert xg.shape[1] == 3
+    spec, om = _oracle_from_product(m, method)
+    for f in range(m.n_fronts()):
+        info = m.front_info(f)
+        if info["leaf"]:
+            om.set_rhs_front(info["id"], m.fetch_rhs_front(f))
+    assert om.factor(method, 64) == 0
+    om.solve(method)
+    xo = om.x_global(m.get_n_dof())
+    for c in range(3):
+        err = np.abs(xg[:, c] - xo[:, c]).max() / np.abs(xo[:, c]).max()
+        say("  rhs", c, "x rel err vs oracle", err)
+        assert err <= X_TOL
+    assert not np.array_equal(xg[:, 0], xg[:, 1])
 
 
 def test_refactorization_reproducible(lib):

@@ -84,10 +84,16 @@ _SIGS = {
     "uhm_get_csr_size_": [_P, _LL], "uhm_fetch_csr_": [_P, _LL, _I, _D],
     "uhm_export_connectivity_": [_P, _I, _I, _I, _I, _I, _I, _I, _I, _I],
     "uhm_set_profile_": [_P, _I], "uhm_get_kernel_stats_": [_P, _D, _D, _D, _I],
+    "uhm_set_precision_": [_P, _I, _I],
+    "uhm_improve_chol_": [_P, _D, _I, _I], "uhm_improve_lu_piv_": [_P, _D, _I, _I],
+    "uhm_get_refine_history_": [_P, _D, _I, _D],
 }
 
 KERNEL_KINDS = ("assemble", "lu_panel", "lu_rows", "chol_diag", "chol_trsm", "gemm", "schur")
 _RET = {"uhm_get_last_error_": C.c_int}
+
+
+PREC_FP64, PREC_MIXED = 0, 1
 
 
 class UHMError(RuntimeError):
@@ -291,6 +297,26 @@ class Mesh:
         a, b = C.c_double(0), C.c_double(0)
         _call("uhm_get_timing_", C.byref(self.handle), C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def set_precision(self, mode, top_levels=2):
+        """mode PREC_FP64 (0) or PREC_MIXED (1: FP32 fronts below the top
+        `top_levels` levels, FP32 MFMA Schur updates)."""
+        _call("uhm_set_precision_", C.byref(self.handle), _i(mode), _i(top_levels))
+
+    def improve(self, method, tol=1e-12, max_iter=20):
+        """Iterative refinement to backward error tol; returns the number of
+        corrections applied."""
+        n = C.c_int(0)
+        name = "uhm_improve_chol_" if method == UHM_CHOL else "uhm_improve_lu_piv_"
+        _call(name, C.byref(self.handle), C.byref(C.c_double(tol)), _i(max_iter), C.byref(n))
+        return n.value
+
+    def refine_history(self):
+        """(backward error before each correction and at the end, ms)."""
+        eta = np.zeros(256)
+        n, ms = C.c_int(len(eta)), C.c_double(0)
+        _call("uhm_get_refine_history_", C.byref(self.handle), _dptr(eta), C.byref(n), C.byref(ms))
+        return eta[:min(n.value, len(eta))].copy(), ms.value
 
     def set_profile(self, on=True):
         _call("uhm_set_profile_", C.byref(self.handle), _i(1 if on else 0))

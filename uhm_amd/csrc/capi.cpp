@@ -244,6 +244,24 @@ void uhm_get_timing_(uhm_fort_p* mesh, uhm_fort_double* fms, uhm_fort_double* sm
     *sms = S(M(mesh))->last_solve_ms();
   });
 }
+void uhm_set_precision_(uhm_fort_p* mesh, uhm_fort_int* mode, uhm_fort_int* top_levels) {
+  guard("uhm_set_precision", [&] { S(M(mesh))->set_precision(*mode, *top_levels); });
+}
+void uhm_improve_chol_(uhm_fort_p* mesh, uhm_fort_double* tol, uhm_fort_int* max_iter, uhm_fort_int* n_iter) {
+  guard("uhm_improve_chol", [&] { *n_iter = S(M(mesh))->refine(uhm::UHM_CHOL, *tol, *max_iter); });
+}
+void uhm_improve_lu_piv_(uhm_fort_p* mesh, uhm_fort_double* tol, uhm_fort_int* max_iter, uhm_fort_int* n_iter) {
+  guard("uhm_improve_lu_piv", [&] { *n_iter = S(M(mesh))->refine(uhm::UHM_LU_PIV, *tol, *max_iter); });
+}
+void uhm_get_refine_history_(uhm_fort_p* mesh, uhm_fort_double* eta, uhm_fort_int* n, uhm_fort_double* ms) {
+  guard("uhm_get_refine_history", [&] {
+    const std::vector<double>& h = S(M(mesh))->refine_history();
+    const int cap = *n;
+    for (int k = 0; k < cap && k < (int)h.size(); ++k) eta[k] = h[k];
+    *n = (int)h.size();
+    *ms = S(M(mesh))->last_refine_ms();
+  });
+}
 void uhm_set_profile_(uhm_fort_p* mesh, uhm_fort_int* on) {
   guard("uhm_set_profile", [&] { S(M(mesh))->set_profile(*on != 0); });
 }

@@ -83,14 +83,14 @@ __device__ __forceinline__ void tri_solve64(double (&x)[16], const double* __res
 // Fill a 64 x 64 LDS block (row stride ld) with the strict lower (LOWER)
 // or strict upper triangle of the kb x kb block at A (leading dimension n),
 // zeros elsewhere, and rdiag[k] = 1 / A[k][k] (k < kb).
-template <bool LOWER>
+template <bool LOWER, typename T>
 __device__ __forceinline__ void load_tri64(double* __restrict__ s, int ld, double* __restrict__ rdiag,
-                                           const double* __restrict__ A, long long n, int kb, int tid, int nthreads) {
+                                           const T* __restrict__ A, long long n, int kb, int tid, int nthreads) {
   for (int idx = tid; idx < 64 * 64; idx += nthreads) {
     const int r = idx & 63, c = idx >> 6;
     const bool in = r < kb && c < kb && (LOWER ? r > c : r < c);
-    s[r * ld + c] = in ? A[r + c * n] : 0.0;
-    if (r == c && r < kb) rdiag[r] = 1.0 / A[r + c * n];
+    s[r * ld + c] = in ? (double)A[r + c * n] : 0.0;
+    if (r == c && r < kb) rdiag[r] = 1.0 / (double)A[r + c * n];
   }
 }
 

@@ -30,16 +30,19 @@ constexpr int TILE = 64;
 // then children in insertion order) into a zero start, which is the exact
 // operation sequence of op_merge on a freshly zeroed front
 // (UHM/src/operation/element.cxx:135-185).  Item: f, a = row tile start,
-// b = column tile start.
+// b = column tile start.  The sum is formed in FP64 whatever the storage
+// types: T of the assembled front, FP64 or FP32 per child.
 // ------------------------------------------------------------------------
 constexpr int MAX_SRC = 16;
 
+template <typename T>
 __global__ __launch_bounds__(256) void k_assemble(const uhmk_front* __restrict__ F,
                                                   const uhmk_item* __restrict__ items,
                                                   const int* __restrict__ src_beg,
                                                   const uhmk_source* __restrict__ srcs,
                                                   const int* __restrict__ inv,
-                                                  double* __restrict__ arena,
+                                                  double* __restrict__ arena64,
+                                                  float* __restrict__ arena32,
                                                   const double* __restrict__ stage) {
   const uhmk_item it = items[blockIdx.x];
   const int f = it.f;
@@ -57,7 +60,7 @@ __global__ __launch_bounds__(256) void k_assemble(const uhmk_front* __restrict__
       s_inv[threadIdx.x] = -1;
     } else {
       const uhmk_front ch = F[sc.src];
-      s_kind[threadIdx.x] = 1;
+      s_kind[threadIdx.x] = 1 + ch.prec;
       s_off[threadIdx.x] = ch.off + ch.fs + (long long)ch.fs * ch.n;
       s_ld[threadIdx.x] = ch.n;
       s_inv[threadIdx.x] = sc.inv;
@@ -71,7 +74,7 @@ __global__ __launch_bounds__(256) void k_assemble(const uhmk_front* __restrict__
 #pragma unroll
   for (int s = 0; s < MAX_SRC; ++s)
     if (s < ns) rmap[s] = s_kind[s] ? inv[s_inv[s] + row] : row;
-  double* A = arena + off;
+  T* A = (sizeof(T) == 8 ? (T*)arena64 : (T*)arena32) + off;
   for (int c = ty; c < TILE; c += 4) {
     const int col = it.b + c;
     if (col >= n) break;
@@ -82,12 +85,13 @@ __global__ __launch_bounds__(256) void k_assemble(const uhmk_front* __restrict__
         const int cj = s_kind[s] ? inv[s_inv[s] + col] : col;
         const int ri = rmap[s];
         if (ri >= 0 && cj >= 0) {
-          const double* src = s_kind[s] ? arena : stage;
-          acc += src[s_off[s] + ri + (long long)cj * s_ld[s]];
+          const long long o = s_off[s] + ri + (long long)cj * s_ld[s];
+          const int k = s_kind[s];
+          acc += k == 0 ? stage[o] : (k == 1 ? arena64[o] : (double)arena32[o]);
         }
       }
     }
-    A[row + (long long)col * n] = acc;
+    A[row + (long long)col * n] = (T)acc;
   }
 }
 
@@ -103,16 +107,16 @@ __global__ __launch_bounds__(256) void k_assemble(const uhmk_front* __restrict__
 // ------------------------------------------------------------------------
 constexpr int PNT = 512, PIB = 8, PNW = PNT / 64;
 
-template <int R>
+template <int R, typename T>
 __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__ F,
                                                   const int* __restrict__ flist, int c0, int nb,
-                                                  double* __restrict__ arena, int* __restrict__ piv,
+                                                  T* __restrict__ arena, int* __restrict__ piv,
                                                   int* __restrict__ info) {
   const int f = flist[blockIdx.x];
   const uhmk_front fr = F[f];
   const int n = fr.n, fs = fr.fs;
   const int kb = min(nb, fs - c0);
-  double* __restrict__ A = arena + fr.off;
+  T* __restrict__ A = arena + fr.off;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 
   __shared__ double s_redv[PNW];
@@ -134,7 +138,7 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__
     for (int q = 0; q < R; ++q) {
       const int r = rb + tid + PNT * q;
 #pragma unroll
-      for (int j = 0; j < PIB; ++j) v[q][j] = (r < fs && j < ibw) ? A[r + (long long)(rb + j) * n] : 0.0;
+      for (int j = 0; j < PIB; ++j) v[q][j] = (r < fs && j < ibw) ? (double)A[r + (long long)(rb + j) * n] : 0.0;
     }
 #pragma unroll
     for (int j = 0; j < PIB; ++j) {
@@ -229,7 +233,7 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__
       if (r < fs)
 #pragma unroll
         for (int jj = 0; jj < PIB; ++jj)
-          if (jj < ibw) A[r + (long long)(rb + jj) * n] = v[q][jj];
+          if (jj < ibw) A[r + (long long)(rb + jj) * n] = (T)v[q][jj];
     }
     __syncthreads();
     // Net row permutation of this inner block's swaps (rows listed once;
@@ -272,13 +276,13 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__
     for (int idx = tid; idx < nother * pc; idx += PNT) {
       const int t = idx / pc, k = idx - t * pc;
       const int col = (t < ib) ? (c0 + t) : (c0 + t + ibw);
-      s_tmp[k][t] = A[s_prow[k] + (long long)col * n];
+      s_tmp[k][t] = (double)A[s_prow[k] + (long long)col * n];
     }
     __syncthreads();
     for (int idx = tid; idx < nother * pc; idx += PNT) {
       const int t = idx / pc, k = idx - t * pc;
       const int col = (t < ib) ? (c0 + t) : (c0 + t + ibw);
-      A[s_prow[k] + (long long)col * n] = s_tmp[s_psrc[k]][t];
+      A[s_prow[k] + (long long)col * n] = (T)s_tmp[s_psrc[k]][t];
     }
     const int nr = kb - ib - ibw;
     if (tid < PIB) {
@@ -292,14 +296,14 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__
         const int col = rb + ibw + t;
         double u[PIB];
 #pragma unroll
-        for (int jj = 0; jj < PIB; ++jj) u[jj] = (jj < ibw) ? A[rb + jj + (long long)col * n] : 0.0;
+        for (int jj = 0; jj < PIB; ++jj) u[jj] = (jj < ibw) ? (double)A[rb + jj + (long long)col * n] : 0.0;
 #pragma unroll
         for (int jj = 1; jj < PIB; ++jj)
 #pragma unroll
           for (int kk = 0; kk < jj; ++kk) u[jj] -= s_L[jj][kk] * u[kk];
 #pragma unroll
         for (int jj = 0; jj < PIB; ++jj) {
-          if (jj < ibw) A[rb + jj + (long long)col * n] = u[jj];
+          if (jj < ibw) A[rb + jj + (long long)col * n] = (T)u[jj];
           s_U[jj][t] = (jj < ibw) ? u[jj] : 0.0;
         }
       }
@@ -313,14 +317,14 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__
             double a[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u)
-              a[u] = (t0 + u < nr) ? A[r + (long long)(rb + ibw + t0 + u) * n] : 0.0;
+              a[u] = (t0 + u < nr) ? (double)A[r + (long long)(rb + ibw + t0 + u) * n] : 0.0;
 #pragma unroll
             for (int u = 0; u < 8; ++u)
 #pragma unroll
               for (int kk = 0; kk < PIB; ++kk) a[u] -= v[q][kk] * s_U[kk][(t0 + u) & (TILE - 1)];
 #pragma unroll
             for (int u = 0; u < 8; ++u)
-              if (t0 + u < nr) A[r + (long long)(rb + ibw + t0 + u) * n] = a[u];
+              if (t0 + u < nr) A[r + (long long)(rb + ibw + t0 + u) * n] = (T)a[u];
           }
         }
       }
@@ -389,14 +393,15 @@ __device__ inline void panel_perm(const int* s_p, int kb, int c0, int fs, int* s
 // 4 lanes per vector (tri_solve64), 64 vectors per workgroup.
 // ------------------------------------------------------------------------
 
+template <typename T>
 __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ F,
                                                  const uhmk_item* __restrict__ items, int c0, int nb,
-                                                 double* __restrict__ arena, const int* __restrict__ piv) {
+                                                 T* __restrict__ arena, const int* __restrict__ piv) {
   const uhmk_item it = items[blockIdx.x];
   const uhmk_front fr = F[it.f];
   const int n = fr.n, fs = fr.fs;
   const int kb = min(nb, fs - c0);
-  double* __restrict__ A = arena + fr.off;
+  T* __restrict__ A = arena + fr.off;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, q = lane & 3;
   __shared__ double s_X[2 * TILE][TILE];  // rows touched by the swaps, per column
   __shared__ double s_T[TILE * LDP];      // triangular panel block
@@ -413,14 +418,14 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
 #pragma unroll
     for (int cr = 0; cr < 16; ++cr) {
       const int c = 16 * q + cr;
-      x[cr] = (live && c < kb) ? A[row + (long long)(c0 + c) * n] : 0.0;
+      x[cr] = (live && c < kb) ? (double)A[row + (long long)(c0 + c) * n] : 0.0;
     }
     uhmd::tri_solve64<false, true>(x, s_T, LDP, s_rd, kb, lane);
     if (live)
 #pragma unroll
       for (int cr = 0; cr < 16; ++cr) {
         const int c = 16 * q + cr;
-        if (c < kb) A[row + (long long)(c0 + c) * n] = x[cr];
+        if (c < kb) A[row + (long long)(c0 + c) * n] = (T)x[cr];
       }
     return;
   }
@@ -434,13 +439,13 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
   const int pc = s_cnt;
   for (int idx = tid; idx < width * pc; idx += 256) {
     const int k = idx % pc, t = idx / pc;
-    s_X[k][t] = A[s_prow[k] + (long long)(j0 + t) * n];
+    s_X[k][t] = (double)A[s_prow[k] + (long long)(j0 + t) * n];
   }
   __syncthreads();
   if (it.a == 0) {
     for (int idx = tid; idx < width * pc; idx += 256) {
       const int k = idx % pc, t = idx / pc;
-      A[s_prow[k] + (long long)(j0 + t) * n] = s_X[s_psrc[k]][t];
+      A[s_prow[k] + (long long)(j0 + t) * n] = (T)s_X[s_psrc[k]][t];
     }
     return;
   }
@@ -448,7 +453,7 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
   // solved in registers, one column per 4 lanes.
   for (int idx = tid; idx < width * (pc - kb); idx += 256) {
     const int k = kb + idx % (pc - kb), t = idx / (pc - kb);
-    A[s_prow[k] + (long long)(j0 + t) * n] = s_X[s_psrc[k]][t];
+    A[s_prow[k] + (long long)(j0 + t) * n] = (T)s_X[s_psrc[k]][t];
   }
   uhmd::load_tri64<true>(s_T, LDP, s_rd, A + c0 + (long long)c0 * n, n, kb, tid, 256);
   __syncthreads();
@@ -464,45 +469,70 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
 #pragma unroll
     for (int cr = 0; cr < 16; ++cr) {
       const int k = 16 * q + cr;
-      if (k < kb) A[(c0 + k) + (long long)(j0 + t) * n] = x[cr];
+      if (k < kb) A[(c0 + k) + (long long)(j0 + t) * n] = (T)x[cr];
     }
 }
 
 inline int ret(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
+
+template <typename T>
+int launch_panel(const uhmk_front* fronts, const int* flist, int n_fronts, int max_rows, int c0, int nb, T* arena,
+                 int* piv, int* info, hipStream_t stream) {
+  const int R = (max_rows + PNT - 1) / PNT;
+  dim3 g(n_fronts), b(PNT);
+  if (R <= 1) hipLaunchKernelGGL((k_lu_panel<1, T>), g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
+  else if (R <= 2) hipLaunchKernelGGL((k_lu_panel<2, T>), g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
+  else if (R <= 4) hipLaunchKernelGGL((k_lu_panel<4, T>), g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
+  else if (R <= 8) hipLaunchKernelGGL((k_lu_panel<8, T>), g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
+  else if (R <= 12) hipLaunchKernelGGL((k_lu_panel<12, T>), g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
+  else return -1001;  // more than 6144 pivot-candidate rows
+  return ret(hipGetLastError());
+}
 
 }  // namespace
 
 extern "C" {
 
 int uhmk_assemble(const uhmk_front* fronts, const uhmk_item* items, int n_items, const int* src_beg,
-                  const uhmk_source* srcs, const int* inv, double* arena, const double* stage,
-                  hipStream_t stream) {
+                  const uhmk_source* srcs, const int* inv, double* arena64, float* arena32, int prec,
+                  const double* stage, hipStream_t stream) {
   if (n_items <= 0) return 0;
-  hipLaunchKernelGGL(k_assemble, dim3(n_items), dim3(256), 0, stream, fronts, items, src_beg, srcs, inv,
-                     arena, stage);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_assemble<double>, dim3(n_items), dim3(256), 0, stream, fronts, items, src_beg, srcs, inv,
+                       arena64, arena32, stage);
+  else if (prec == 1)
+    hipLaunchKernelGGL(k_assemble<float>, dim3(n_items), dim3(256), 0, stream, fronts, items, src_beg, srcs, inv,
+                       arena64, arena32, stage);
+  else
+    return -1006;
   return ret(hipGetLastError());
 }
 
 int uhmk_lu_panel(const uhmk_front* fronts, const int* flist, int n_fronts, int max_rows, int c0, int nb,
-                  double* arena, int* piv, int* info, hipStream_t stream) {
+                  void* arena, int prec, int* piv, int* info, hipStream_t stream) {
   if (n_fronts <= 0) return 0;
   if (nb > TILE || nb < 1) return -1000;
-  const int R = (max_rows + PNT - 1) / PNT;
-  dim3 g(n_fronts), b(PNT);
-  if (R <= 1) hipLaunchKernelGGL(k_lu_panel<1>, g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
-  else if (R <= 2) hipLaunchKernelGGL(k_lu_panel<2>, g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
-  else if (R <= 4) hipLaunchKernelGGL(k_lu_panel<4>, g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
-  else if (R <= 8) hipLaunchKernelGGL(k_lu_panel<8>, g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
-  else if (R <= 12) hipLaunchKernelGGL(k_lu_panel<12>, g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
-  else return -1001;  // more than 6144 pivot-candidate rows
-  return ret(hipGetLastError());
+  if (prec == 0)
+    return launch_panel<double>(fronts, flist, n_fronts, max_rows, c0, nb, static_cast<double*>(arena), piv, info,
+                                stream);
+  if (prec == 1)
+    return launch_panel<float>(fronts, flist, n_fronts, max_rows, c0, nb, static_cast<float*>(arena), piv, info,
+                               stream);
+  return -1006;
 }
 
-int uhmk_lu_rows(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb,
-                 double* arena, const int* piv, hipStream_t stream) {
+int uhmk_lu_rows(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, void* arena,
+                 int prec, const int* piv, hipStream_t stream) {
   if (n_items <= 0) return 0;
   if (nb > TILE || nb < 1) return -1000;
-  hipLaunchKernelGGL(k_lu_rows, dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb, arena, piv);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_lu_rows<double>, dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb,
+                       static_cast<double*>(arena), piv);
+  else if (prec == 1)
+    hipLaunchKernelGGL(k_lu_rows<float>, dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb,
+                       static_cast<float*>(arena), piv);
+  else
+    return -1006;
   return ret(hipGetLastError());
 }
 

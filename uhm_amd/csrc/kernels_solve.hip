@@ -12,217 +12,328 @@
 //   rhs merge Helper_::_merge_rhs (helper.hxx:253-301)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "uhmk.h"
 
 namespace {
 
-constexpr int BLK = 64;
+// ------------------------------------------------------------------------
+// Level-by-level front solves.  Per tree level the forward pass runs
+//   gather   x_f = b_f + sum of the children's xb (merge order, through inv);
+//            LU: the factor rows are read through the composed pivot
+//            permutation, so P is applied while gathering
+//   trsv     x_t <- L^-1 x_t on ATL, one workgroup per front
+//   gemv     xb -= ABL x_t, 16-row (FP64) / 32-row (FP32) chunks of ABL
+// and the backward pass, root first,
+//   branch   xb <- the parent's solution rows (emap)
+//   gemv     LU: x_t -= ATR xb (row chunks); Cholesky: x_t -= ABL^T xb
+//            (column chunks, one wavefront per column)
+//   trsv     x_t <- U^-1 x_t (LU) or L^-T x_t (Cholesky)
+// Every kernel handles all n_rhs columns (x and b are n x n_rhs per front,
+// leading dimension n).  T is the storage type of the level's fronts; all
+// arithmetic is FP64.
+// ------------------------------------------------------------------------
+constexpr int PERM_LDS = 12288;
 
-// ------------------------------------------------------------------------
-// Forward solve of one right-hand-side column for every front in flist.
-// x_f = b_f + sum_children xb_c (children gathered through inv, in merge
-// order), then P (LU), then elimination with the unit (LU) or non-unit
-// (Cholesky) lower factor over all n rows, which also forms
-// xb -= ABL * xt.  Blocks of 64 columns: the diagonal triangle is solved by
-// one wavefront with shuffles, the rows below by the whole workgroup.
-// ------------------------------------------------------------------------
-template <bool CHOL>
-__global__ __launch_bounds__(256) void k_solve_fwd(const uhmk_front* __restrict__ F,
-                                                   const int* __restrict__ flist,
+// perm[i] = row of the unpivoted right-hand side that LAPACK-style swaps
+// (k <-> k + p[k], k = 0 .. fs-1) bring to position i.
+__global__ __launch_bounds__(64) void k_lu_perm(const uhmk_front* __restrict__ F, const int* __restrict__ flist,
+                                                const int* __restrict__ piv, int* __restrict__ perm) {
+  __shared__ int s[PERM_LDS];
+  const uhmk_front fr = F[flist[blockIdx.x]];
+  const int fs = fr.fs;
+  const int* __restrict__ p = piv + fr.poff;
+  int* __restrict__ out = perm + fr.poff;
+  const bool lds = fs <= PERM_LDS;
+  int* w = lds ? s : out;
+  for (int i = threadIdx.x; i < fs; i += 64) w[i] = i;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 0; k < fs; ++k) {
+      const int d = p[k];
+      if (d) {
+        const int t = w[k];
+        w[k] = w[k + d];
+        w[k + d] = t;
+      }
+    }
+  __syncthreads();
+  if (lds)
+    for (int i = threadIdx.x; i < fs; i += 64) out[i] = s[i];
+}
+
+// item {f, 0, first row, 0}: 256 rows of front f.
+template <bool LU>
+__global__ __launch_bounds__(256) void k_sv_gather(const uhmk_front* __restrict__ F,
+                                                   const uhmk_item* __restrict__ items,
                                                    const int* __restrict__ src_beg,
                                                    const uhmk_source* __restrict__ srcs,
-                                                   const int* __restrict__ inv,
-                                                   const double* __restrict__ arena,
-                                                   const int* __restrict__ piv, double* __restrict__ x,
-                                                   const double* __restrict__ b, int col) {
-  extern __shared__ double sx[];
-  const int f = flist[blockIdx.x];
-  const uhmk_front fr = F[f];
+                                                   const int* __restrict__ inv, const int* __restrict__ perm,
+                                                   double* __restrict__ x, const double* __restrict__ b, int nrhs) {
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
   const int n = fr.n, fs = fr.fs;
-  const int tid = threadIdx.x, lane = tid & 63;
-  const long long xo = fr.xoff + (long long)col * n;
-  const int s0 = src_beg[f], s1 = src_beg[f + 1];
-  for (int i = tid; i < n; i += 256) {
-    double acc = b[xo + i];
-    for (int s = s0; s < s1; ++s) {
-      const uhmk_source sc = srcs[s];
-      if (sc.src < 0) continue;
-      const int ri = inv[sc.inv + i];
-      if (ri >= 0) {
-        const uhmk_front ch = F[sc.src];
-        acc += x[ch.xoff + (long long)col * ch.n + ch.fs + ri];
-      }
-    }
-    sx[i] = acc;
+  const int i = it.b + threadIdx.x;
+  if (i >= n) return;
+  const int r = (LU && i < fs) ? perm[fr.poff + i] : i;
+  double* __restrict__ xo = x + fr.xoff;
+  for (int c = 0; c < nrhs; ++c) xo[i + (long long)c * n] = b[fr.xoff + r + (long long)c * n];
+  for (int s = src_beg[it.f]; s < src_beg[it.f + 1]; ++s) {
+    const uhmk_source sc = srcs[s];
+    if (sc.src < 0) continue;
+    const int ri = inv[sc.inv + r];
+    if (ri < 0) continue;
+    const uhmk_front ch = F[sc.src];
+    const double* __restrict__ xc = x + ch.xoff + ch.fs + ri;
+    for (int c = 0; c < nrhs; ++c) xo[i + (long long)c * n] += xc[(long long)c * ch.n];
   }
-  __syncthreads();
-  if (!CHOL && tid == 0) {
-    for (int k = 0; k < fs; ++k) {
-      const int p = piv[fr.poff + k];
-      if (p) {
-        const double t = sx[k];
-        sx[k] = sx[k + p];
-        sx[k + p] = t;
-      }
-    }
-  }
-  __syncthreads();
-  const double* __restrict__ A = arena + fr.off;
-  for (int k0 = 0; k0 < fs; k0 += BLK) {
-    const int kb = min(BLK, fs - k0);
-    if (tid < 64) {
-      const int j = lane;
-      double xj = (j < kb) ? sx[k0 + j] : 0.0;
-      double Lr[BLK];
-#pragma unroll
-      for (int k = 0; k < BLK; ++k)
-        Lr[k] = (j < kb && k < kb && k <= j) ? A[(k0 + j) + (long long)(k0 + k) * n] : 0.0;
-#pragma unroll
-      for (int k = 0; k < BLK; ++k) {
-        if (k < kb) {
-          if (CHOL && j == k) xj = xj / Lr[k];
-          const double xk = __shfl(xj, k);
-          if (j > k) xj -= Lr[k] * xk;
-        }
-      }
-      if (j < kb) sx[k0 + j] = xj;
-    }
-    __syncthreads();
-    for (int i = k0 + kb + tid; i < n; i += 256) {
-      double acc = 0.0;
-#pragma unroll 8
-      for (int k = 0; k < kb; ++k) acc += A[i + (long long)(k0 + k) * n] * sx[k0 + k];
-      sx[i] -= acc;
-    }
-    __syncthreads();
-  }
-  for (int i = tid; i < n; i += 256) x[xo + i] = sx[i];
 }
 
-// Backward solve (LU): xb <- parent's solution (branch), xt -= ATR * xb,
-// xt <- U^-1 xt.
-__global__ __launch_bounds__(256) void k_solve_bwd_lu(const uhmk_front* __restrict__ F,
-                                                      const int* __restrict__ flist,
-                                                      const int* __restrict__ emap,
-                                                      const double* __restrict__ arena,
-                                                      double* __restrict__ x, int col) {
-  extern __shared__ double sx[];
-  const int f = flist[blockIdx.x];
-  const uhmk_front fr = F[f];
-  const int n = fr.n, fs = fr.fs, ss = n - fs;
-  const int tid = threadIdx.x, lane = tid & 63;
-  const long long xo = fr.xoff + (long long)col * n;
-  for (int i = tid; i < fs; i += 256) sx[i] = x[xo + i];
-  if (fr.parent >= 0) {
-    const uhmk_front P = F[fr.parent];
-    for (int i = tid; i < ss; i += 256) sx[fs + i] = x[P.xoff + (long long)col * P.n + emap[fr.emap + i]];
-  } else {
-    for (int i = tid; i < ss; i += 256) sx[fs + i] = x[xo + fs + i];
-  }
-  __syncthreads();
-  const double* __restrict__ A = arena + fr.off;
-  for (int i = tid; i < fs; i += 256) {
-    double acc = 0.0;
-#pragma unroll 8
-    for (int k = 0; k < ss; ++k) acc += A[i + (long long)(fs + k) * n] * sx[fs + k];
-    sx[i] -= acc;
-  }
-  __syncthreads();
-  const int nblk = (fs + BLK - 1) / BLK;
-  for (int bk = nblk - 1; bk >= 0; --bk) {
-    const int k0 = bk * BLK, kb = min(BLK, fs - k0);
-    if (tid < 64) {
-      const int j = lane;
-      double xj = (j < kb) ? sx[k0 + j] : 0.0;
-      double Ur[BLK];
-#pragma unroll
-      for (int k = 0; k < BLK; ++k)
-        Ur[k] = (j < kb && k < kb && k >= j) ? A[(k0 + j) + (long long)(k0 + k) * n] : 0.0;
-#pragma unroll
-      for (int k = BLK - 1; k >= 0; --k) {
-        if (k < kb) {
-          if (j == k) xj = xj / Ur[k];
-          const double xk = __shfl(xj, k);
-          if (j < k) xj -= Ur[k] * xk;
-        }
-      }
-      if (j < kb) sx[k0 + j] = xj;
-    }
-    __syncthreads();
-    for (int i = tid; i < k0; i += 256) {
-      double acc = 0.0;
-#pragma unroll 8
-      for (int k = 0; k < kb; ++k) acc += A[i + (long long)(k0 + k) * n] * sx[k0 + k];
-      sx[i] -= acc;
-    }
-    __syncthreads();
-  }
-  for (int i = tid; i < n; i += 256) x[xo + i] = sx[i];
+// Broadcast of lane L's double to the wavefront (v_readlane).
+template <int L>
+__device__ __forceinline__ double bcast(double v) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), L),
+                          __builtin_amdgcn_readlane(__double2loint(v), L));
 }
 
-// Backward solve (Cholesky): xb <- parent's solution, xt -= ABL^T xb,
-// xt <- L^-T xt.
-__global__ __launch_bounds__(256) void k_solve_bwd_chol(const uhmk_front* __restrict__ F,
-                                                        const int* __restrict__ flist,
-                                                        const int* __restrict__ emap,
-                                                        const double* __restrict__ arena,
-                                                        double* __restrict__ x, int col) {
-  extern __shared__ double sx[];
-  const int f = flist[blockIdx.x];
-  const uhmk_front fr = F[f];
-  const int n = fr.n, fs = fr.fs, ss = n - fs;
+// One wavefront solves a 64 x 64 triangle held one entry vector per lane:
+// step K finalises x_K (times rd when SCALE) in its owner lane, broadcasts it
+// and updates x_j -= e[K] x_K in the other lanes, where e holds the lane's
+// strict-triangle entries (zero elsewhere).  ASC runs K = 0 .. 63, else
+// 63 .. 0; steps with K >= kb are skipped.
+template <int K, bool ASC, bool SCALE>
+struct TrsvStep {
+  __device__ __forceinline__ static void run(double& xj, const double (&e)[64], double rd, int lane, int kb) {
+    constexpr int KK = ASC ? K : 63 - K;
+    if (KK < kb) {
+      if (SCALE && lane == KK) xj *= rd;
+      const double xk = bcast<KK>(xj);
+      xj = fma(-e[KK], xk, xj);
+    }
+    TrsvStep<K + 1, ASC, SCALE>::run(xj, e, rd, lane, kb);
+  }
+};
+template <bool ASC, bool SCALE>
+struct TrsvStep<64, ASC, SCALE> {
+  __device__ __forceinline__ static void run(double&, const double (&)[64], double, int, int) {}
+};
+
+template <typename T>
+__device__ __forceinline__ double ld(const T* p) {
+  return (double)*p;
+}
+
+// Triangular solve of the diagonal super-block [s0, s1) of ATL, s0 = sb * SB,
+// s1 = min(s0 + SB, fs), SB <= 1024; fronts with fs <= s0 are skipped.
+// Forward: x <- L^-1 x (LU: unit lower; Cholesky: lower).  Backward:
+// x <- U^-1 x (LU) or L^-T x (Cholesky).  The couplings with the rest of
+// ATL are applied by the gemv kernels between super-blocks.  One workgroup
+// of NT threads per front; the super-block's part of x lives in LDS and is
+// processed in 64-row blocks: one wavefront solves the 64 x 64 triangle
+// (TrsvStep), the workgroup updates the rest of the super-block.
+constexpr int SB_MAX = 1024;
+
+template <typename T, bool CHOL, bool FWD, int NT>
+__global__ __launch_bounds__(NT) void k_sv_trsv(const uhmk_front* __restrict__ F, const int* __restrict__ flist,
+                                                const T* __restrict__ arena, double* __restrict__ x, int nrhs,
+                                                int sb, int SB) {
+  __shared__ double sx[SB_MAX];
+  __shared__ double sL[(!FWD && CHOL) ? 64 * 65 : 1];
+  const uhmk_front fr = F[flist[blockIdx.x]];
+  const int n = fr.n, fs = fr.fs;
+  const int s0 = sb * SB;
+  if (s0 >= fs) return;
+  const int s1 = min(s0 + SB, fs), m = s1 - s0;
+  const T* __restrict__ A = arena + fr.off;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const long long xo = fr.xoff + (long long)col * n;
-  for (int i = tid; i < fs; i += 256) sx[i] = x[xo + i];
-  if (fr.parent >= 0) {
-    const uhmk_front P = F[fr.parent];
-    for (int i = tid; i < ss; i += 256) sx[fs + i] = x[P.xoff + (long long)col * P.n + emap[fr.emap + i]];
-  } else {
-    for (int i = tid; i < ss; i += 256) sx[fs + i] = x[xo + fs + i];
-  }
-  __syncthreads();
-  const double* __restrict__ A = arena + fr.off;
-  for (int i = wid; i < fs; i += 4) {
-    double acc = 0.0;
-#pragma unroll 8
-    for (int k = lane; k < ss; k += 64) acc += A[(fs + k) + (long long)i * n] * sx[fs + k];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-    if (lane == 0) sx[i] -= acc;
-  }
-  __syncthreads();
-  const int nblk = (fs + BLK - 1) / BLK;
-  for (int bk = nblk - 1; bk >= 0; --bk) {
-    const int k0 = bk * BLK, kb = min(BLK, fs - k0), k1 = k0 + kb;
-    for (int c = wid; c < kb; c += 4) {
-      double acc = 0.0;
-#pragma unroll 8
-      for (int i = k1 + lane; i < fs; i += 64) acc += A[i + (long long)(k0 + c) * n] * sx[i];
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-      if (lane == 0) sx[k0 + c] -= acc;
-    }
+  constexpr int NW = NT / 64;
+  const int nblk = (m + 63) / 64;
+  for (int c = 0; c < nrhs; ++c) {
+    double* __restrict__ xg = x + fr.xoff + (long long)c * n + s0;
+    for (int i = tid; i < m; i += NT) sx[i] = xg[i];
     __syncthreads();
-    if (tid < 64) {
-      const int j = lane;
-      double xj = (j < kb) ? sx[k0 + j] : 0.0;
-      double Lc[BLK];
+    for (int bq = 0; bq < nblk; ++bq) {
+      const int bk = FWD ? bq : nblk - 1 - bq;
+      const int l0 = 64 * bk, kb = min(64, m - l0), l1 = l0 + kb;  // local rows
+      const int k0 = s0 + l0;                                       // absolute
+      if (!FWD && CHOL) {
+        // the diagonal block goes to LDS (transposed reads below); its loads
+        // are issued first so that they overlap the dot products
+        constexpr int NPRE = 64 * 64 / NT;
+        double pre[NPRE];
 #pragma unroll
-      for (int i = 0; i < BLK; ++i)
-        Lc[i] = (j < kb && i < kb && i >= j) ? A[(k0 + i) + (long long)(k0 + j) * n] : 0.0;
-#pragma unroll
-      for (int i = BLK - 1; i >= 0; --i) {
-        if (i < kb) {
-          if (j == i) xj = xj / Lc[i];
-          const double xi = __shfl(xj, i);
-          if (j < i) xj -= Lc[i] * xi;
+        for (int u = 0; u < NPRE; ++u) {
+          const int idx = tid + NT * u, r = idx & 63, cc = idx >> 6;
+          pre[u] = (r < kb && cc < kb) ? ld(A + (k0 + r) + (long long)(k0 + cc) * n) : 0.0;
         }
+        // x_q -= sum_{i in [l1, m)} L(i, q) x_i, NW columns at a time per wave
+        double acc[64 / NW];
+#pragma unroll
+        for (int u = 0; u < 64 / NW; ++u) acc[u] = 0.0;
+#pragma unroll 4
+        for (int i = l1 + lane; i < m; i += 64) {
+          const double xi = sx[i];
+#pragma unroll
+          for (int u = 0; u < 64 / NW; ++u) {
+            const int q = wid + NW * u;
+            if (q < kb) acc[u] = fma(ld(A + (s0 + i) + (long long)(k0 + q) * n), xi, acc[u]);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 64 / NW; ++u) {
+          double a = acc[u];
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+          const int q = wid + NW * u;
+          if (lane == 0 && q < kb) sx[l0 + q] -= a;
+        }
+        // stage the diagonal block so that lane j can read column j
+#pragma unroll
+        for (int u = 0; u < NPRE; ++u) {
+          const int idx = tid + NT * u, r = idx & 63, cc = idx >> 6;
+          sL[r * 65 + cc] = pre[u];
+        }
+        __syncthreads();
       }
-      if (j < kb) sx[k0 + j] = xj;
+      if (tid < 64) {
+        double e[64];
+        double rd = 1.0;
+        const int j = lane;
+        if (FWD) {
+          // row j of the block, strict lower part
+#pragma unroll
+          for (int q = 0; q < 64; ++q) e[q] = (j < kb && q < j) ? ld(A + (k0 + j) + (long long)(k0 + q) * n) : 0.0;
+          if (CHOL && j < kb) rd = 1.0 / ld(A + (k0 + j) + (long long)(k0 + j) * n);
+        } else if (CHOL) {
+          // column j of the block below the diagonal: e[i] = L(i, j), i > j
+#pragma unroll
+          for (int q = 0; q < 64; ++q) e[q] = (q > j) ? sL[q * 65 + j] : 0.0;
+          if (j < kb) rd = 1.0 / sL[j * 65 + j];
+        } else {
+          // row j of U, strict upper part
+#pragma unroll
+          for (int q = 0; q < 64; ++q)
+            e[q] = (j < kb && q > j && q < kb) ? ld(A + (k0 + j) + (long long)(k0 + q) * n) : 0.0;
+          if (j < kb) rd = 1.0 / ld(A + (k0 + j) + (long long)(k0 + j) * n);
+        }
+        double xj = (j < kb) ? sx[l0 + j] : 0.0;
+        if (FWD) {
+          if (CHOL) TrsvStep<0, true, true>::run(xj, e, rd, lane, kb);
+          else TrsvStep<0, true, false>::run(xj, e, rd, lane, kb);
+        } else {
+          TrsvStep<0, false, true>::run(xj, e, rd, lane, kb);
+        }
+        if (j < kb) sx[l0 + j] = xj;
+      }
+      __syncthreads();
+      if (FWD || !CHOL) {
+        // FWD: rows below the block; LU backward: rows above it
+        const int r0 = FWD ? l1 : 0, r1 = FWD ? m : l0;
+        for (int i = r0 + tid; i < r1; i += NT) {
+          const T* __restrict__ Ai = A + (s0 + i) + (long long)k0 * n;
+          double acc = 0.0;
+          if (kb == 64) {
+#pragma unroll
+            for (int q = 0; q < 64; ++q) acc = fma(ld(Ai + (long long)q * n), sx[l0 + q], acc);
+          } else {
+            for (int q = 0; q < kb; ++q) acc = fma(ld(Ai + (long long)q * n), sx[l0 + q], acc);
+          }
+          sx[i] -= acc;
+        }
+        __syncthreads();
+      }
     }
+    for (int i = tid; i < m; i += NT) xg[i] = sx[i];
     __syncthreads();
   }
-  for (int i = tid; i < n; i += 256) x[xo + i] = sx[i];
+}
+
+// Row-chunk GEMV: x[rows] -= A[rows, cbeg:cend) x[cbeg:cend) over RT rows
+// per item (one 128-byte line of each column), rows [r0, min(r0 + RT,
+// r0 < fs ? fs : n)).  item {f, cbeg, r0, cend}.  Serves xb -= ABL x_t,
+// x_t -= ATR xb and the super-block couplings inside ATL.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sv_gemv_rows(const uhmk_front* __restrict__ F,
+                                                      const uhmk_item* __restrict__ items,
+                                                      const T* __restrict__ arena, double* __restrict__ x, int nrhs) {
+  constexpr int RT = 128 / (int)sizeof(T);
+  constexpr int KG = 64 / RT;
+  __shared__ double red[4][RT];
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs;
+  const T* __restrict__ A = arena + fr.off;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int r = lane % RT, kg = lane / RT;
+  const int rend = min(it.b + RT, it.b < fs ? fs : n);
+  const int cbeg = it.a, cend = it.c;
+  const int i = it.b + r;
+  const bool live = i < rend;
+  for (int c = 0; c < nrhs; ++c) {
+    double* __restrict__ xv = x + fr.xoff + (long long)c * n;
+    double acc = 0.0;
+    if (live) {
+#pragma unroll 16
+      for (int k = cbeg + w * KG + kg; k < cend; k += 4 * KG) acc = fma(ld(A + i + (long long)k * n), xv[k], acc);
+    }
+#pragma unroll
+    for (int o = RT; o < 64; o <<= 1) acc += __shfl_xor(acc, o);
+    if (kg == 0) red[w][r] = acc;
+    __syncthreads();
+    if (tid < RT && it.b + tid < rend) xv[it.b + tid] -= (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+    __syncthreads();
+  }
+}
+
+// Column-chunk transposed GEMV: x[col] -= sum_{i in [rbeg, rend)} A(i, col)
+// x[i] for 16 columns [c0, min(c0 + 16, fs)), four per wavefront at once.
+// item {f, rbeg, c0, rend}.  Serves x_t -= ABL^T xb and the super-block
+// couplings of the Cholesky backward solve.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sv_gemv_cols(const uhmk_front* __restrict__ F,
+                                                      const uhmk_item* __restrict__ items,
+                                                      const T* __restrict__ arena, double* __restrict__ x, int nrhs) {
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs;
+  const T* __restrict__ A = arena + fr.off;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int cend = min(it.b + 16, fs);
+  for (int c = 0; c < nrhs; ++c) {
+    double* __restrict__ xv = x + fr.xoff + (long long)c * n;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = it.a + lane; i < it.c; i += 64) {
+      const double xi = xv[i];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int col = it.b + w + 4 * u;
+        if (col < cend) acc[u] = fma(ld(A + i + (long long)col * n), xi, acc[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      double a = acc[u];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+      const int col = it.b + w + 4 * u;
+      if (lane == 0 && col < cend) xv[col] -= a;
+    }
+  }
+}
+
+// Branch: xb <- the parent's solution at emap.  item {f, 0, first row, 0}.
+__global__ __launch_bounds__(256) void k_sv_branch(const uhmk_front* __restrict__ F,
+                                                   const uhmk_item* __restrict__ items,
+                                                   const int* __restrict__ emap, double* __restrict__ x, int nrhs) {
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs, ss = n - fs;
+  const int i = it.b + threadIdx.x;
+  if (i >= ss || fr.parent < 0) return;
+  const uhmk_front P = F[fr.parent];
+  const int pi = emap[fr.emap + i];
+  for (int c = 0; c < nrhs; ++c) x[fr.xoff + fs + i + (long long)c * n] = x[P.xoff + pi + (long long)c * P.n];
 }
 
 // ------------------------------------------------------------------------
@@ -356,8 +467,9 @@ __global__ __launch_bounds__(256) void k_residual(int nrow, const long long* __r
   }
 }
 
-// Global <-> front vectors: g[gidx[k]] = xf[k] (dir 0) or xf[k] = g[gidx[k]]
-// (dir 1) over the factor rows of every front; item list = fronts.
+// Global <-> front vectors over the factor rows of every front: dir 0
+// g = xf, dir 1 xf = g, dir 2 g += xf (refinement update); one workgroup
+// per front.
 __global__ __launch_bounds__(256) void k_front_global(const uhmk_front* __restrict__ F, int nfront,
                                                       const long long* __restrict__ goff,
                                                       double* __restrict__ xf, double* __restrict__ g,
@@ -369,7 +481,8 @@ __global__ __launch_bounds__(256) void k_front_global(const uhmk_front* __restri
   const long long go = goff[f] + (long long)col * nglob;
   for (int i = threadIdx.x; i < fr.fs; i += 256) {
     if (dir == 0) g[go + i] = xf[xo + i];
-    else xf[xo + i] = g[go + i];
+    else if (dir == 1) xf[xo + i] = g[go + i];
+    else g[go + i] += xf[xo + i];
   }
 }
 
@@ -399,34 +512,91 @@ __global__ __launch_bounds__(256) void k_gather_rhs(long long nrow, const long l
 
 inline int ret(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
 
+template <typename T, bool CHOL, bool FWD>
+int launch_trsv(const uhmk_front* fronts, const int* flist, int n_fronts, int max_fs, const T* arena, double* x,
+                int nrhs, int sb, int SB, hipStream_t stream) {
+  if (SB > SB_MAX || SB < 64 || SB % 64) return -1008;
+  if (std::min(max_fs, SB) > 256)
+    hipLaunchKernelGGL((k_sv_trsv<T, CHOL, FWD, 512>), dim3(n_fronts), dim3(512), 0, stream, fronts, flist, arena, x,
+                       nrhs, sb, SB);
+  else
+    hipLaunchKernelGGL((k_sv_trsv<T, CHOL, FWD, 256>), dim3(n_fronts), dim3(256), 0, stream, fronts, flist, arena, x,
+                       nrhs, sb, SB);
+  return ret(hipGetLastError());
+}
+
+template <typename T>
+int trsv_any(int chol, int fwd, const uhmk_front* fronts, const int* flist, int n_fronts, int max_fs,
+             const T* arena, double* x, int nrhs, int sb, int SB, hipStream_t stream) {
+  if (chol)
+    return fwd ? launch_trsv<T, true, true>(fronts, flist, n_fronts, max_fs, arena, x, nrhs, sb, SB, stream)
+               : launch_trsv<T, true, false>(fronts, flist, n_fronts, max_fs, arena, x, nrhs, sb, SB, stream);
+  return fwd ? launch_trsv<T, false, true>(fronts, flist, n_fronts, max_fs, arena, x, nrhs, sb, SB, stream)
+             : launch_trsv<T, false, false>(fronts, flist, n_fronts, max_fs, arena, x, nrhs, sb, SB, stream);
+}
 }  // namespace
 
 extern "C" {
 
-int uhmk_solve_fwd(const uhmk_front* fronts, const int* flist, int n_fronts, int max_n, int chol,
-                   const int* src_beg, const uhmk_source* srcs, const int* inv, const double* arena,
-                   const int* piv, double* x, const double* b, int col, hipStream_t stream) {
+int uhmk_lu_perm(const uhmk_front* fronts, const int* flist, int n_fronts, const int* piv, int* perm,
+                 hipStream_t stream) {
   if (n_fronts <= 0) return 0;
-  const size_t sh = (size_t)max_n * sizeof(double);
-  if (sh > 160 * 1024) return -1002;
-  if (chol)
-    hipLaunchKernelGGL(k_solve_fwd<true>, dim3(n_fronts), dim3(256), sh, stream, fronts, flist, src_beg, srcs,
-                       inv, arena, piv, x, b, col);
-  else
-    hipLaunchKernelGGL(k_solve_fwd<false>, dim3(n_fronts), dim3(256), sh, stream, fronts, flist, src_beg, srcs,
-                       inv, arena, piv, x, b, col);
+  hipLaunchKernelGGL(k_lu_perm, dim3(n_fronts), dim3(64), 0, stream, fronts, flist, piv, perm);
   return ret(hipGetLastError());
 }
 
-int uhmk_solve_bwd(const uhmk_front* fronts, const int* flist, int n_fronts, int max_n, int chol,
-                   const int* emap, const double* arena, double* x, int col, hipStream_t stream) {
-  if (n_fronts <= 0) return 0;
-  const size_t sh = (size_t)max_n * sizeof(double);
-  if (sh > 160 * 1024) return -1002;
-  if (chol)
-    hipLaunchKernelGGL(k_solve_bwd_chol, dim3(n_fronts), dim3(256), sh, stream, fronts, flist, emap, arena, x, col);
+int uhmk_solve_gather(const uhmk_front* fronts, const uhmk_item* items, int n_items, int lu, const int* src_beg,
+                      const uhmk_source* srcs, const int* inv, const int* perm, double* x, const double* b, int nrhs,
+                      hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  if (lu)
+    hipLaunchKernelGGL(k_sv_gather<true>, dim3(n_items), dim3(256), 0, stream, fronts, items, src_beg, srcs, inv,
+                       perm, x, b, nrhs);
   else
-    hipLaunchKernelGGL(k_solve_bwd_lu, dim3(n_fronts), dim3(256), sh, stream, fronts, flist, emap, arena, x, col);
+    hipLaunchKernelGGL(k_sv_gather<false>, dim3(n_items), dim3(256), 0, stream, fronts, items, src_beg, srcs, inv,
+                       perm, x, b, nrhs);
+  return ret(hipGetLastError());
+}
+
+
+int uhmk_solve_trsv(const uhmk_front* fronts, const int* flist, int n_fronts, int max_fs, int chol, int fwd,
+                    int sb, int sb_size, const void* arena, int prec, double* x, int nrhs, hipStream_t stream) {
+  if (n_fronts <= 0 || max_fs <= 0) return 0;
+  if (prec == 0)
+    return trsv_any<double>(chol, fwd, fronts, flist, n_fronts, max_fs, static_cast<const double*>(arena), x, nrhs, sb,
+                            sb_size, stream);
+  if (prec == 1)
+    return trsv_any<float>(chol, fwd, fronts, flist, n_fronts, max_fs, static_cast<const float*>(arena), x, nrhs, sb,
+                           sb_size, stream);
+  return -1006;
+}
+
+int uhmk_solve_gemv(const uhmk_front* fronts, const uhmk_item* items, int n_items, int cols, const void* arena,
+                    int prec, double* x, int nrhs, hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  if (prec != 0 && prec != 1) return -1006;
+  if (!cols) {
+    if (prec == 0)
+      hipLaunchKernelGGL(k_sv_gemv_rows<double>, dim3(n_items), dim3(256), 0, stream, fronts, items,
+                         static_cast<const double*>(arena), x, nrhs);
+    else
+      hipLaunchKernelGGL(k_sv_gemv_rows<float>, dim3(n_items), dim3(256), 0, stream, fronts, items,
+                         static_cast<const float*>(arena), x, nrhs);
+  } else {
+    if (prec == 0)
+      hipLaunchKernelGGL(k_sv_gemv_cols<double>, dim3(n_items), dim3(256), 0, stream, fronts, items,
+                         static_cast<const double*>(arena), x, nrhs);
+    else
+      hipLaunchKernelGGL(k_sv_gemv_cols<float>, dim3(n_items), dim3(256), 0, stream, fronts, items,
+                         static_cast<const float*>(arena), x, nrhs);
+  }
+  return ret(hipGetLastError());
+}
+
+int uhmk_solve_branch(const uhmk_front* fronts, const uhmk_item* items, int n_items, const int* emap, double* x,
+                      int nrhs, hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  hipLaunchKernelGGL(k_sv_branch, dim3(n_items), dim3(256), 0, stream, fronts, items, emap, x, nrhs);
   return ret(hipGetLastError());
 }
 

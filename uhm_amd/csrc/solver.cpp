@@ -2,6 +2,8 @@
 // multifrontal factorization, solve and residual check.
 #include "solver.hpp"
 
+#include <chrono>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -57,6 +59,51 @@ void FrontSolver::check_k(int rc, const char* what) const {
 
 static inline long long align_up(long long v, long long a) { return (v + a - 1) / a * a; }
 
+void FrontSolver::layout_arenas() {
+  // Fronts of FP64 levels are packed into the FP64 arena, the others into
+  // the FP32 arena; each offset is 512-byte aligned.
+  const int nl = (int)levels_.size();
+  level_prec_.assign(nl, 0);
+  if (prec_mode_ == 1)
+    for (int L = 0; L < nl - top_fp64_; ++L) level_prec_[L] = 1;
+  long long a64 = 0, a32 = 0;
+  for (int f = 0; f < (int)fr_.size(); ++f) {
+    uhmk_front& F = fr_[f];
+    F.prec = level_prec_[front_level_[f]];
+    const long long sz = (long long)F.n * F.n;
+    if (F.prec) {
+      F.off = a32;
+      a32 = align_up(a32 + sz, 128);
+    } else {
+      F.off = a64;
+      a64 = align_up(a64 + sz, 64);
+    }
+  }
+  arena_elems_ = a64;
+  arena32_elems_ = a32;
+}
+
+void FrontSolver::set_precision(int mode, int top_levels) {
+  if (mode != 0 && mode != 1) throw std::runtime_error("set_precision: mode must be 0 (FP64) or 1 (mixed)");
+  if (top_levels < 0) throw std::runtime_error("set_precision: top_levels must be >= 0");
+  if (mode == prec_mode_ && top_levels == top_fp64_) return;
+  prec_mode_ = mode;
+  top_fp64_ = top_levels;
+  layout_arenas();
+  schedule_method_ = 0;
+  solve_method_ = 0;
+  factored_ = false;
+  if (created_) {
+    check_hip(hipStreamSynchronize(stream_), "set_precision");
+    dF_.upload(fr_, stream_);
+    d_arena_.release();
+    d_arena32_.release();
+    d_arena_.alloc(std::max(1LL, arena_elems_) * sizeof(double));
+    d_arena32_.alloc(std::max(1LL, arena32_elems_) * sizeof(float));
+    check_hip(hipStreamSynchronize(stream_), "set_precision");
+  }
+}
+
 void FrontSolver::build_symbolic() {
   // Front order = scheduler order: generations leaf-first, element id order
   // inside a generation (scheduler.cxx:157-178, 306-328).
@@ -73,15 +120,13 @@ void FrontSolver::build_symbolic() {
     }
   const int nf = (int)order_.size();
   fr_.assign(nf, uhmk_front{});
-  long long aoff = 0, xoff = 0, soff = 0, g = 0;
+  long long xoff = 0, soff = 0, g = 0;
   int poff = 0;
   for (int f = 0; f < nf; ++f) {
     Element_* e = order_[f];
     uhmk_front& F = fr_[f];
     F.fs = e->fs;
     F.n = e->fs + e->ss;
-    F.off = aoff;
-    aoff = align_up(aoff + (long long)F.n * F.n, 64);
     F.xoff = xoff;
     xoff = align_up(xoff + (long long)F.n * nrhs_, 8);
     F.poff = poff;
@@ -95,7 +140,7 @@ void FrontSolver::build_symbolic() {
     }
     F.emap = -1;
   }
-  arena_elems_ = aoff;
+  layout_arenas();
   x_elems_ = xoff;
   stage_elems_ = soff;
   piv_elems_ = poff;
@@ -175,7 +220,7 @@ void FrontSolver::build_symbolic() {
 }
 
 size_t FrontSolver::device_bytes() const {
-  return d_arena_.bytes + d_stage_.bytes + d_x_.bytes + d_b_.bytes + d_piv_.bytes + d_items_.bytes +
+  return d_arena_.bytes + d_arena32_.bytes + d_rf_.bytes + d_stage_.bytes + d_x_.bytes + d_b_.bytes + d_piv_.bytes + d_items_.bytes +
          d_val_.bytes + d_colidx_.bytes + d_cidx_.bytes + d_inv_.bytes + d_emap_.bytes;
 }
 
@@ -197,8 +242,19 @@ void FrontSolver::create_buffers() {
   level_lists_off_.assign(levels_.size(), {});
   for (auto& l : levels_) lv.insert(lv.end(), l.begin(), l.end());
   d_lvl_.upload(lv, stream_);
-  d_arena_.alloc(arena_elems_ * sizeof(double));
+  d_arena_.alloc(std::max(1LL, arena_elems_) * sizeof(double));
+  d_arena32_.alloc(std::max(1LL, arena32_elems_) * sizeof(float));
+  d_rf_.alloc(std::max(1LL, x_elems_) * sizeof(double));
+  check_hip(hipMemsetAsync(d_rf_.p, 0, d_rf_.bytes, stream_), "memset rf");
   d_piv_.alloc(std::max(1, piv_elems_) * sizeof(int));
+  d_perm_.alloc(std::max(1, piv_elems_) * sizeof(int));
+  {
+    std::vector<int> fsl;
+    for (int f = 0; f < (int)fr_.size(); ++f)
+      if (fr_[f].fs > 0) fsl.push_back(f);
+    n_fsl_ = (int)fsl.size();
+    d_fsl_.upload(fsl, stream_);
+  }
   d_x_.alloc(x_elems_ * sizeof(double));
   d_b_.alloc(x_elems_ * sizeof(double));
   d_stage_.alloc(std::max(1LL, stage_elems_) * sizeof(double));
@@ -223,7 +279,7 @@ void FrontSolver::stage_lhs(int f, int m, const int* map, const double* K) {
   for (int j = 0; j < m; ++j)
     for (int i = 0; i < m; ++i) S[map[i] + (size_t)map[j] * n] += K[i + (size_t)j * m];
   stage_dirty_ = true;
-  csr_built_ = false;
+  vals_dirty_ = true;
 }
 
 void FrontSolver::stage_rhs(int f, int m, int nr, const int* map, const double* b) {
@@ -237,6 +293,7 @@ void FrontSolver::stage_rhs(int f, int m, int nr, const int* map, const double* 
   for (int r = 0; r < nr; ++r)
     for (int i = 0; i < m; ++i) B[map[i] + (size_t)r * n] += b[i + (size_t)r * m];
   rhs_dirty_ = true;
+  bg_dirty_ = true;
 }
 
 void FrontSolver::upload_stage() {
@@ -284,11 +341,12 @@ void FrontSolver::generate(unsigned long long seed, double alpha, double beta, i
   if (with_lhs) {
     for (auto& s : hstage_) s.clear();
     stage_dirty_ = false;
-    csr_built_ = false;
+    vals_dirty_ = true;
   }
   if (with_rhs) {
     for (auto& s : hrhs_) s.clear();
     rhs_dirty_ = false;
+    bg_dirty_ = true;
   }
 }
 
@@ -431,6 +489,10 @@ void FrontSolver::build_schedule(int method) {
     ls.count = (int)(items_.size() - ls.off);
     if (ls.count) launches_.push_back(ls);
   }
+  for (Launch& l : launches_) {
+    l.prec = level_prec_[l.level];
+    l.mma = (prec_mode_ == 0) ? 0 : 1;
+  }
   // Algorithmic work of every launch: the flops / bytes the operation needs,
   // not what a kernel happens to execute (diagonal SYRK tiles count their
   // lower half, assembly counts each stored entry once).
@@ -496,7 +558,7 @@ void FrontSolver::build_schedule(int method) {
         break;
     }
     l.flops = fl;
-    l.bytes = by;
+    l.bytes = l.prec ? 0.5 * by : by;
   }
   d_items_.upload(items_, stream_);
   d_flists_.upload(flists_, stream_);
@@ -513,7 +575,6 @@ int FrontSolver::factor(int method) {
   const uhmk_front* F = dF_.as<uhmk_front>();
   const uhmk_item* IT = d_items_.as<uhmk_item>();
   const int* FL = d_flists_.as<int>();
-  double* A = d_arena_.as<double>();
   int* P = d_piv_.as<int>();
   int* info = d_info_.as<int>();
   check_hip(hipMemsetAsync(info, 0, d_info_.bytes, stream_), "memset info");
@@ -531,33 +592,36 @@ int FrontSolver::factor(int method) {
     const Launch& l = launches_[li];
     int rc = 0;
     if (profile_) check_hip(hipEventRecord(pev[2 * li], stream_), "event");
+    void* A = arena_ptr(l.prec);
     switch (l.kind) {
       case Launch::ASSEMBLE:
         rc = uhmk_assemble(F, IT + l.off, l.count, d_srcbeg_.as<int>(), d_srcs_.as<uhmk_source>(), d_inv_.as<int>(),
-                           A, d_stage_.as<double>(), stream_);
+                           d_arena_.as<double>(), d_arena32_.as<float>(), l.prec, d_stage_.as<double>(), stream_);
         break;
       case Launch::LU_PANEL:
-        rc = uhmk_lu_panel(F, FL + l.off, l.count, l.extra, l.c0, l.nb, A, P, info, stream_);
+        rc = uhmk_lu_panel(F, FL + l.off, l.count, l.extra, l.c0, l.nb, A, l.prec, P, info, stream_);
         break;
       case Launch::LU_ROWS:
-        rc = uhmk_lu_rows(F, IT + l.off, l.count, l.c0, l.nb, A, P, stream_);
+        rc = uhmk_lu_rows(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, P, stream_);
         break;
       case Launch::CHOL_DIAG:
-        rc = uhmk_chol_diag(F, FL + l.off, l.count, l.c0, l.nb, A, info, stream_);
+        rc = uhmk_chol_diag(F, FL + l.off, l.count, l.c0, l.nb, A, l.prec, info, stream_);
         break;
       case Launch::CHOL_TRSM:
-        rc = uhmk_chol_trsm(F, IT + l.off, l.count, l.c0, l.nb, A, stream_);
+        rc = uhmk_chol_trsm(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, stream_);
         break;
       case Launch::GEMM:
-        rc = uhmk_gemm_update(F, IT + l.off, l.count, l.c0, l.nb, 0, l.extra, l.tile, A, stream_);
+        rc = uhmk_gemm_update(F, IT + l.off, l.count, l.c0, l.nb, 0, l.extra, l.tile, A, l.prec, l.mma, stream_);
         break;
       case Launch::SCHUR:
-        rc = uhmk_gemm_update(F, IT + l.off, l.count, 0, 0, 1, l.extra, l.tile, A, stream_);
+        rc = uhmk_gemm_update(F, IT + l.off, l.count, 0, 0, 1, l.extra, l.tile, A, l.prec, l.mma, stream_);
         break;
     }
     check_k(rc, "factor");
     if (profile_) check_hip(hipEventRecord(pev[2 * li + 1], stream_), "event");
   }
+  if (method == UHM_LU_PIV)
+    check_k(uhmk_lu_perm(F, d_fsl_.as<int>(), n_fsl_, P, d_perm_.as<int>(), stream_), "lu perm");
   check_hip(hipEventRecord(e1, stream_), "event");
   check_hip(hipEventSynchronize(e1), "factor");
   float ms = 0;
@@ -586,32 +650,161 @@ int FrontSolver::factor(int method) {
   return 0;
 }
 
+void FrontSolver::build_solve_schedule(int method) {
+  // Step lists of the level solves, built once per pattern and method.
+  // Fronts with fs <= 1024 solve ATL in one workgroup; larger ones in
+  // 512-row super-blocks whose couplings go through the gemv kernels, so
+  // that a level's big fronts are read by many workgroups.
+  const bool chol = (method == UHM_CHOL);
+  sitems_.clear();
+  sflist_.clear();
+  sfwd_.clear();
+  sbwd_.clear();
+  auto rows = [&](int f, int a, int b, int cut, int step, int ca, int cb) {
+    for (int r = a; r < std::min(b, cut); r += step) sitems_.push_back(uhmk_item{f, ca, r, cb});
+    for (int r = std::max(a, cut); r < b; r += step) sitems_.push_back(uhmk_item{f, ca, r, cb});
+  };
+  auto step = [&](SolveStep::Kind kind, int off) {
+    SolveStep st;
+    st.kind = kind;
+    st.off = off;
+    st.cnt = (int)sitems_.size() - off;
+    return st;
+  };
+  const int nl = (int)levels_.size();
+  std::vector<int> tr_off(nl), tr_cnt(nl), mfs(nl), sbs(nl), nsb(nl);
+  for (int L = 0; L < nl; ++L) {
+    tr_off[L] = (int)sflist_.size();
+    mfs[L] = 0;
+    for (int f : levels_[L])
+      if (fr_[f].fs > 0) {
+        sflist_.push_back(f);
+        mfs[L] = std::max(mfs[L], fr_[f].fs);
+      }
+    tr_cnt[L] = (int)sflist_.size() - tr_off[L];
+    sbs[L] = mfs[L] <= 1024 ? 1024 : 512;
+    nsb[L] = (mfs[L] + sbs[L] - 1) / sbs[L];
+  }
+  auto trsv = [&](int L, int sb) {
+    SolveStep st;
+    st.prec = level_prec_[L];
+    st.kind = SolveStep::TRSV;
+    st.off = tr_off[L];
+    st.cnt = tr_cnt[L];
+    st.max_fs = mfs[L];
+    st.sb = sb;
+    st.sb_size = sbs[L];
+    return st;
+  };
+  auto rt_of = [&](int L) { return level_prec_[L] ? 32 : 16; };  // gemv row chunk: one 128-byte line
+  for (int L = 0; L < nl; ++L) {
+    const int rt = rt_of(L);
+    int o = (int)sitems_.size();
+    for (int f : levels_[L])
+      for (int r = 0; r < fr_[f].n; r += 256) sitems_.push_back(uhmk_item{f, 0, r, 0});
+    sfwd_.push_back(step(SolveStep::GATHER, o));
+    for (int sb = 0; sb < nsb[L]; ++sb) {
+      sfwd_.push_back(trsv(L, sb));
+      sfwd_.back().fwd = 1;
+      o = (int)sitems_.size();
+      for (int f : levels_[L]) {
+        const int fs = fr_[f].fs, n = fr_[f].n, s0 = sb * sbs[L];
+        if (fs <= s0) continue;
+        const int s1 = std::min(s0 + sbs[L], fs);
+        rows(f, s1, n, fs, rt, s0, s1);
+      }
+      SolveStep g = step(SolveStep::GEMV, o);
+      g.cols = 0;
+      g.prec = level_prec_[L];
+      if (g.cnt) sfwd_.push_back(g);
+    }
+  }
+  for (int L = nl - 1; L >= 0; --L) {
+    const int rt = rt_of(L);
+    int o = (int)sitems_.size();
+    for (int f : levels_[L])
+      if (fr_[f].parent >= 0)
+        for (int r = 0; r < fr_[f].n - fr_[f].fs; r += 256) sitems_.push_back(uhmk_item{f, 0, r, 0});
+    SolveStep br = step(SolveStep::BRANCH, o);
+    if (br.cnt) sbwd_.push_back(br);
+    o = (int)sitems_.size();
+    for (int f : levels_[L]) {
+      const int fs = fr_[f].fs, n = fr_[f].n;
+      if (fs == 0 || n == fs) continue;
+      if (chol)
+        for (int c0 = 0; c0 < fs; c0 += 16) sitems_.push_back(uhmk_item{f, fs, c0, n});
+      else
+        rows(f, 0, fs, fs, rt, fs, n);
+    }
+    SolveStep g = step(SolveStep::GEMV, o);
+    g.cols = chol ? 1 : 0;
+    g.prec = level_prec_[L];
+    if (g.cnt) sbwd_.push_back(g);
+    for (int sb = nsb[L] - 1; sb >= 0; --sb) {
+      sbwd_.push_back(trsv(L, sb));
+      if (sb == 0) continue;
+      o = (int)sitems_.size();
+      for (int f : levels_[L]) {
+        const int fs = fr_[f].fs, s0 = sb * sbs[L];
+        if (fs <= s0) continue;
+        const int s1 = std::min(s0 + sbs[L], fs);
+        if (chol)
+          for (int c0 = 0; c0 < s0; c0 += 16) sitems_.push_back(uhmk_item{f, s0, c0, s1});
+        else
+          rows(f, 0, s0, fs, rt, s0, s1);
+      }
+      SolveStep u = step(SolveStep::GEMV, o);
+      u.cols = chol ? 1 : 0;
+      u.prec = level_prec_[L];
+      if (u.cnt) sbwd_.push_back(u);
+    }
+  }
+  d_sitems_.upload(sitems_, stream_);
+  d_sflist_.upload(sflist_, stream_);
+  check_hip(hipStreamSynchronize(stream_), "solve schedule upload");
+  solve_method_ = method;
+}
+
+void FrontSolver::run_solve_steps(const std::vector<SolveStep>& steps, bool chol, const double* rhs) {
+  const uhmk_front* F = dF_.as<uhmk_front>();
+  const uhmk_item* SI = d_sitems_.as<uhmk_item>();
+  const int* SF = d_sflist_.as<int>();
+  double* X = d_x_.as<double>();
+  for (const SolveStep& st : steps) {
+    const void* A = arena_ptr(st.prec);
+    int rc = 0;
+    switch (st.kind) {
+      case SolveStep::GATHER:
+        rc = uhmk_solve_gather(F, SI + st.off, st.cnt, chol ? 0 : 1, d_srcbeg_.as<int>(), d_srcs_.as<uhmk_source>(),
+                               d_inv_.as<int>(), d_perm_.as<int>(), X, rhs, nrhs_, stream_);
+        break;
+      case SolveStep::TRSV:
+        rc = uhmk_solve_trsv(F, SF + st.off, st.cnt, st.max_fs, chol, st.fwd, st.sb, st.sb_size, A, st.prec, X, nrhs_,
+                             stream_);
+        break;
+      case SolveStep::GEMV:
+        rc = uhmk_solve_gemv(F, SI + st.off, st.cnt, st.cols, A, st.prec, X, nrhs_, stream_);
+        break;
+      case SolveStep::BRANCH:
+        rc = uhmk_solve_branch(F, SI + st.off, st.cnt, d_emap_.as<int>(), X, nrhs_, stream_);
+        break;
+    }
+    check_k(rc, "solve");
+  }
+}
+
 void FrontSolver::solve(int method) {
   if (!factored_) throw std::runtime_error("solve: not factorized");
+  if (method != last_method_) throw std::runtime_error("solve: method differs from the factorization's");
   upload_rhs();
+  if (solve_method_ != method) build_solve_schedule(method);
   const bool chol = (method == UHM_CHOL);
-  const uhmk_front* F = dF_.as<uhmk_front>();
-  const int* LV = d_lvl_.as<int>();
-  std::vector<int> loff(levels_.size() + 1, 0), maxn(levels_.size(), 0);
-  for (size_t L = 0; L < levels_.size(); ++L) {
-    loff[L + 1] = loff[L] + (int)levels_[L].size();
-    for (int f : levels_[L]) maxn[L] = std::max(maxn[L], fr_[f].n);
-  }
   hipEvent_t e0, e1;
   check_hip(hipEventCreate(&e0), "event");
   check_hip(hipEventCreate(&e1), "event");
   check_hip(hipEventRecord(e0, stream_), "event");
-  for (int col = 0; col < nrhs_; ++col) {
-    for (size_t L = 0; L < levels_.size(); ++L)
-      check_k(uhmk_solve_fwd(F, LV + loff[L], (int)levels_[L].size(), maxn[L], chol, d_srcbeg_.as<int>(),
-                             d_srcs_.as<uhmk_source>(), d_inv_.as<int>(), d_arena_.as<double>(), d_piv_.as<int>(),
-                             d_x_.as<double>(), d_b_.as<double>(), col, stream_),
-              "solve fwd");
-    for (int L = (int)levels_.size() - 1; L >= 0; --L)
-      check_k(uhmk_solve_bwd(F, LV + loff[L], (int)levels_[L].size(), maxn[L], chol, d_emap_.as<int>(),
-                             d_arena_.as<double>(), d_x_.as<double>(), col, stream_),
-              "solve bwd");
-  }
+  run_solve_steps(sfwd_, chol, d_b_.as<double>());
+  run_solve_steps(sbwd_, chol, d_b_.as<double>());
   check_hip(hipEventRecord(e1, stream_), "event");
   check_hip(hipEventSynchronize(e1), "solve");
   float ms = 0;
@@ -709,22 +902,110 @@ void FrontSolver::build_csr() {
   d_bptr_.upload(bptr, stream_);
   d_bidx_.upload(bidx, stream_);
   std::vector<int> bn32(bn.begin(), bn.end());
-  DevBuf dbn;
-  dbn.upload(bn32, stream_);
+  d_bn_.upload(bn32, stream_);
   d_val_.alloc(std::max(1LL, nnz_) * sizeof(double));
   d_xg_.alloc(std::max(1LL, N * nrhs_) * sizeof(double));
   d_bg_.alloc(std::max(1LL, N * nrhs_) * sizeof(double));
-  check_k(uhmk_csr_values(nnz_, d_cptr_.as<long long>(), d_cidx_.as<long long>(), d_stage_.as<double>(),
-                          d_val_.as<double>(), stream_),
-          "csr values");
-  for (int col = 0; col < nrhs_; ++col)
-    check_k(uhmk_gather_rhs(N, d_bptr_.as<long long>(), d_bidx_.as<long long>(), dbn.as<int>(), d_b_.as<double>(),
-                            d_bg_.as<double>() + (long long)col * N, col, stream_),
-            "gather rhs");
-  d_r_.alloc(std::max(1LL, N) * sizeof(double));
+  d_r_.alloc(std::max(1LL, N * nrhs_) * sizeof(double));
   d_norms_.alloc(4 * sizeof(double) * std::max(1, nrhs_));
   check_hip(hipStreamSynchronize(stream_), "build_csr");
   csr_built_ = true;
+  vals_dirty_ = bg_dirty_ = true;
+  refresh_csr();
+}
+
+void FrontSolver::refresh_csr() {
+  // Values and the assembled right-hand side follow the current inputs; the
+  // pattern is built once.
+  if (!csr_built_) {
+    build_csr();
+    return;
+  }
+  upload_stage();
+  upload_rhs();
+  const long long N = ndof_;
+  if (vals_dirty_)
+    check_k(uhmk_csr_values(nnz_, d_cptr_.as<long long>(), d_cidx_.as<long long>(), d_stage_.as<double>(),
+                            d_val_.as<double>(), stream_),
+            "csr values");
+  if (bg_dirty_)
+    for (int col = 0; col < nrhs_; ++col)
+      check_k(uhmk_gather_rhs(N, d_bptr_.as<long long>(), d_bidx_.as<long long>(), d_bn_.as<int>(), d_b_.as<double>(),
+                              d_bg_.as<double>() + (long long)col * N, col, stream_),
+              "gather rhs");
+  vals_dirty_ = bg_dirty_ = false;
+}
+
+void FrontSolver::residual_norms(int col, double* h) {
+  // r(:, col) = b - A x(:, col) from the global x; h = {max|r|, max row sum
+  // |A|, max|x|, max|b|}
+  const long long N = ndof_;
+  double* nrm = d_norms_.as<double>() + 4 * col;
+  check_hip(hipMemsetAsync(nrm, 0, 4 * sizeof(double), stream_), "memset norms");
+  check_k(uhmk_residual((int)N, d_rowptr_.as<long long>(), d_colidx_.as<int>(), d_val_.as<double>(),
+                        d_xg_.as<double>() + (long long)col * N, d_bg_.as<double>() + (long long)col * N,
+                        d_r_.as<double>() + (long long)col * N, nrm, stream_),
+          "residual");
+  check_hip(hipMemcpyAsync(h, nrm, 4 * sizeof(double), hipMemcpyDeviceToHost, stream_), "norms");
+  check_hip(hipStreamSynchronize(stream_), "residual");
+}
+
+int FrontSolver::refine(int method, double tol, int max_iter) {
+  // Iterative refinement with FP64 residuals on the assembled matrix and
+  // corrections through the (possibly low-precision) factors: the slot of
+  // Mesh_::improve_{chol,lu_piv} (UHM/src/mesh/chol.cxx:136-149,
+  // UHM/src/mesh/lu_piv.cxx:232-245), whose reference body is a stub.
+  if (!factored_) throw std::runtime_error("refine: not factorized");
+  if (method != last_method_) throw std::runtime_error("refine: method differs from the factorization's");
+  refresh_csr();
+  if (solve_method_ != method) build_solve_schedule(method);
+  const bool chol = (method == UHM_CHOL);
+  const long long N = ndof_;
+  const uhmk_front* F = dF_.as<uhmk_front>();
+  const int nf = (int)fr_.size();
+  const auto t0 = std::chrono::steady_clock::now();
+  refine_hist_.clear();
+  for (int col = 0; col < nrhs_; ++col)
+    check_k(uhmk_front_global(F, nf, d_goff_.as<long long>(), d_x_.as<double>(), d_xg_.as<double>(), col, N, 0,
+                              stream_),
+            "gather x");
+  int it = 0;
+  double h[4];
+  for (;; ++it) {
+    double worst = 0.0;
+    for (int col = 0; col < nrhs_; ++col) {
+      residual_norms(col, h);
+      worst = std::max(worst, h[0] / (h[1] * h[2] + h[3]));
+    }
+    refine_hist_.push_back(worst);
+    eta_ = worst;
+    if (worst <= tol || it >= max_iter || !(worst == worst)) break;
+    // correction: d = M^-1 r, right-hand side r on the factor rows of the
+    // owning fronts (the schur rows of the refinement buffer stay zero)
+    for (int col = 0; col < nrhs_; ++col)
+      check_k(uhmk_front_global(F, nf, d_goff_.as<long long>(), d_rf_.as<double>(), d_r_.as<double>(), col, N, 1,
+                                stream_),
+              "scatter r");
+    run_solve_steps(sfwd_, chol, d_rf_.as<double>());
+    run_solve_steps(sbwd_, chol, d_rf_.as<double>());
+    for (int col = 0; col < nrhs_; ++col)
+      check_k(uhmk_front_global(F, nf, d_goff_.as<long long>(), d_x_.as<double>(), d_xg_.as<double>(), col, N, 2,
+                                stream_),
+              "x += d");
+  }
+  // the refined solution back into the fronts (factor rows, then the
+  // branch pass for the schur rows) so that copy_out sees it
+  for (int col = 0; col < nrhs_; ++col)
+    check_k(uhmk_front_global(F, nf, d_goff_.as<long long>(), d_x_.as<double>(), d_xg_.as<double>(), col, N, 1,
+                              stream_),
+            "scatter x");
+  std::vector<SolveStep> br;
+  for (const SolveStep& st : sbwd_)
+    if (st.kind == SolveStep::BRANCH) br.push_back(st);
+  run_solve_steps(br, chol, d_b_.as<double>());
+  check_hip(hipStreamSynchronize(stream_), "refine");
+  t_refine_ms_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return it;
 }
 
 void FrontSolver::check(int) {
@@ -733,7 +1014,7 @@ void FrontSolver::check(int) {
   // hierarchical check multiplies through a no-op trmm,
   // LINAL/src/dense/trmm.cxx:104-130).
   if (!factored_) throw std::runtime_error("check: not factorized");
-  if (!csr_built_) build_csr();
+  refresh_csr();
   const long long N = ndof_;
   eta_ = 0.0;
   residual_1norm_ = 0.0;
@@ -746,11 +1027,13 @@ void FrontSolver::check(int) {
     check_hip(hipMemsetAsync(nrm, 0, 4 * sizeof(double), stream_), "memset norms");
     check_k(uhmk_residual((int)N, d_rowptr_.as<long long>(), d_colidx_.as<int>(), d_val_.as<double>(),
                           d_xg_.as<double>() + (long long)col * N, d_bg_.as<double>() + (long long)col * N,
-                          d_r_.as<double>(), nrm, stream_),
+                          d_r_.as<double>() + (long long)col * N, nrm, stream_),
             "residual");
     double h[4];
     check_hip(hipMemcpyAsync(h, nrm, sizeof(h), hipMemcpyDeviceToHost, stream_), "norms");
-    check_hip(hipMemcpyAsync(r.data(), d_r_.p, N * sizeof(double), hipMemcpyDeviceToHost, stream_), "residual");
+    check_hip(hipMemcpyAsync(r.data(), d_r_.as<double>() + (long long)col * N, N * sizeof(double),
+                             hipMemcpyDeviceToHost, stream_),
+              "residual");
     check_hip(hipStreamSynchronize(stream_), "check");
     for (int k = 0; k < 4; ++k) norms_[k] = std::max(norms_[k], h[k]);
     const double eta = h[0] / (h[1] * h[2] + h[3]);
@@ -760,7 +1043,7 @@ void FrontSolver::check(int) {
 }
 
 void FrontSolver::csr(std::vector<long long>& rowptr, std::vector<int>& col, std::vector<double>& val) {
-  if (!csr_built_) build_csr();
+  refresh_csr();
   rowptr.resize(ndof_ + 1);
   col.resize(nnz_);
   val.resize(nnz_);
@@ -781,9 +1064,16 @@ void FrontSolver::fetch_x(int f, int m, int nr, const int* map, double* out) {
 }
 
 void FrontSolver::fetch_front_raw(int f, double* out) {
-  const int n = fr_[f].n;
-  check_hip(hipMemcpy(out, d_arena_.as<double>() + fr_[f].off, (size_t)n * n * sizeof(double), hipMemcpyDeviceToHost),
-            "fetch front");
+  const size_t nn = (size_t)fr_[f].n * fr_[f].n;
+  if (fr_[f].prec) {
+    std::vector<float> h(nn);
+    check_hip(hipMemcpy(h.data(), d_arena32_.as<float>() + fr_[f].off, nn * sizeof(float), hipMemcpyDeviceToHost),
+              "fetch front");
+    for (size_t k = 0; k < nn; ++k) out[k] = h[k];
+  } else {
+    check_hip(hipMemcpy(out, d_arena_.as<double>() + fr_[f].off, nn * sizeof(double), hipMemcpyDeviceToHost),
+              "fetch front");
+  }
 }
 
 void FrontSolver::fetch_front(int f, int m, const int* map, double* out) {
@@ -831,7 +1121,7 @@ void FrontSolver::fetch_x_global(double* out) {
 }
 
 void FrontSolver::fetch_b_global(double* out) {
-  if (!csr_built_) build_csr();
+  refresh_csr();
   check_hip(hipMemcpy(out, d_bg_.p, ndof_ * nrhs_ * sizeof(double), hipMemcpyDeviceToHost), "fetch b global");
 }
 

@@ -48,10 +48,22 @@ struct Launch {
   int extra = 0;      // max pivot rows (panel), transb (gemm / schur)
   int level = 0;
   int tile = 64;      // gemm / schur tile edge
+  int prec = 0;       // storage of the level's fronts: 0 FP64, 1 FP32
+  int mma = 0;        // gemm / schur products: 0 FP64, 1 FP32
   double flops = 0;   // algorithmic flops of the launch
   double bytes = 0;   // algorithmic HBM bytes of the launch
 };
 constexpr int N_LAUNCH_KINDS = 7;
+
+// One launch of the level solves.
+struct SolveStep {
+  enum Kind { GATHER, TRSV, GEMV, BRANCH } kind = GATHER;
+  int off = 0, cnt = 0;            // items (gather / gemv / branch) or front list (trsv)
+  int max_fs = 0, sb = 0, sb_size = 0;  // trsv
+  int cols = 0;                    // gemv: row chunks (0) or column chunks (1)
+  int prec = 0;                    // storage of the level's fronts
+  int fwd = 0;                     // trsv: forward (1) or backward (0)
+};
 
 // Per-kind totals of a profiled factorization (device events per launch).
 struct KernelStats {
@@ -106,6 +118,24 @@ class FrontSolver {
   int n_rhs() const { return nrhs_; }
   void csr(std::vector<long long>& rowptr, std::vector<int>& col, std::vector<double>& val);
 
+  // Precision of the fronts.  mode 0: every front FP64 (FP64 MFMA).
+  // mode 1 (mixed): fronts of the top `top_levels` tree levels stay FP64
+  // (their pivot blocks factorize in FP64), all other fronts are stored in
+  // FP32, and every Schur / trailing update runs as FP32 MFMA products.
+  // Changing it re-lays out the arenas and needs a new factorization.
+  void set_precision(int mode, int top_levels);
+  int precision_mode() const { return prec_mode_; }
+  int level_precision(int L) const { return level_prec_[L]; }
+
+  // Iterative refinement on the last factorization: x <- x + M^-1 (b - A x)
+  // with the residual in FP64 against the assembled matrix, until the
+  // normwise backward error of every column is <= tol or max_iter
+  // corrections were applied.  Returns the number of corrections; the
+  // backward error after each step is in refine_history().
+  int refine(int method, double tol, int max_iter);
+  const std::vector<double>& refine_history() const { return refine_hist_; }
+  double last_refine_ms() const { return t_refine_ms_; }
+
   // tuning / timing
   void set_block(int nb) {
     if (nb != nb_) schedule_method_ = 0;
@@ -123,6 +153,12 @@ class FrontSolver {
   void build_symbolic();
   void build_schedule(int method);
   void build_csr();
+  void build_solve_schedule(int method);
+  void layout_arenas();
+  void* arena_ptr(int prec) const { return prec ? d_arena32_.p : d_arena_.p; }
+  void residual_norms(int col, double* h4);
+  void run_solve_steps(const std::vector<SolveStep>& steps, bool chol, const double* rhs);
+  void refresh_csr();
   void upload_stage();
   void upload_rhs();
   void check_hip(hipError_t e, const char* what) const;
@@ -146,7 +182,9 @@ class FrontSolver {
   std::vector<int> lmap_off_;
   std::vector<long long> goff_;    // global offset of each front's factor block
   long long ndof_ = 0;
-  long long arena_elems_ = 0, x_elems_ = 0, stage_elems_ = 0;
+  long long arena_elems_ = 0, arena32_elems_ = 0, x_elems_ = 0, stage_elems_ = 0;
+  int prec_mode_ = 0, top_fp64_ = 2;
+  std::vector<int> level_prec_;
   int piv_elems_ = 0;
 
   // host staging
@@ -161,19 +199,27 @@ class FrontSolver {
   std::vector<Launch> launches_;
   std::vector<uhmk_item> gen_items_;
   std::vector<std::vector<int>> level_lists_off_;
+  // solve schedule
+  int solve_method_ = 0;
+  std::vector<uhmk_item> sitems_;
+  std::vector<int> sflist_;
+  std::vector<SolveStep> sfwd_, sbwd_;
+  int n_fsl_ = 0;
 
   // device buffers
   DevBuf dF_, d_emap_, d_inv_, d_srcbeg_, d_srcs_, d_lmap_, d_goff_;
-  DevBuf d_items_, d_flists_, d_gen_items_, d_lvl_;
-  DevBuf d_arena_, d_piv_, d_x_, d_b_, d_stage_, d_info_;
+  DevBuf d_items_, d_flists_, d_gen_items_, d_lvl_, d_sitems_, d_sflist_, d_fsl_, d_perm_;
+  DevBuf d_arena_, d_arena32_, d_piv_, d_x_, d_b_, d_stage_, d_info_;
+  DevBuf d_rf_;  // refinement right-hand side (front layout)
   // assembled matrix (CSR) for residuals
-  bool csr_built_ = false;
+  bool csr_built_ = false, vals_dirty_ = true, bg_dirty_ = true;
   long long nnz_ = 0;
-  DevBuf d_rowptr_, d_colidx_, d_val_, d_cptr_, d_cidx_, d_bptr_, d_bidx_, d_xg_, d_bg_, d_r_, d_norms_;
+  DevBuf d_rowptr_, d_colidx_, d_val_, d_cptr_, d_cidx_, d_bptr_, d_bidx_, d_xg_, d_bg_, d_r_, d_norms_, d_bn_;
 
   double residual_1norm_ = 0.0, eta_ = 0.0;
   double norms_[4] = {0, 0, 0, 0};
-  double t_factor_ms_ = 0, t_solve_ms_ = 0;
+  double t_factor_ms_ = 0, t_solve_ms_ = 0, t_refine_ms_ = 0;
+  std::vector<double> refine_hist_;
   int last_method_ = 0;
   bool factored_ = false;
   bool profile_ = false;
