@@ -82,41 +82,61 @@ __device__ __forceinline__ void tri_solve64(double (&x)[16], const double* __res
 
 // Fill a 64 x 64 LDS block (row stride ld) with the strict lower (LOWER)
 // or strict upper triangle of the kb x kb block at A (leading dimension n),
-// zeros elsewhere, and rdiag[k] = 1 / A[k][k] (k < kb).
-template <bool LOWER, typename T>
+// zeros elsewhere, and rdiag[k] = 1 / A[k][k] (k < kb).  NT threads; every
+// thread's loads are issued before its LDS stores so they overlap.
+template <bool LOWER, int NT, typename T>
 __device__ __forceinline__ void load_tri64(double* __restrict__ s, int ld, double* __restrict__ rdiag,
-                                           const T* __restrict__ A, long long n, int kb, int tid, int nthreads) {
-  for (int idx = tid; idx < 64 * 64; idx += nthreads) {
-    const int r = idx & 63, c = idx >> 6;
-    const bool in = r < kb && c < kb && (LOWER ? r > c : r < c);
-    s[r * ld + c] = in ? (double)A[r + c * n] : 0.0;
-    if (r == c && r < kb) rdiag[r] = 1.0 / (double)A[r + c * n];
+                                           const T* __restrict__ A, long long n, int kb, int tid) {
+  constexpr int PER = 64 * 64 / NT;
+  double v[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int idx = tid + NT * u, r = idx & 63, c = idx >> 6;
+    v[u] = (r < kb && c < kb && (LOWER ? r >= c : r <= c)) ? (double)A[r + c * n] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int idx = tid + NT * u, r = idx & 63, c = idx >> 6;
+    s[r * ld + c] = (r == c) ? 0.0 : v[u];
+    if (r == c && r < kb) rdiag[r] = 1.0 / v[u];
   }
 }
 
-// Right-looking Cholesky of a 64 x 64 block held one row per lane
-// (a[j] = A[i][j], lane i).  Column k is read from the other lanes with
-// v_readlane; updates run on the whole row (entries above the diagonal are
-// scratch and never stored).  first <- k + 1 of the first non-positive pivot.
+// Right-looking Cholesky of a 64 x 64 block by four wavefronts: lane i
+// holds row i, wavefront w holds columns j = w + 4 u (u = 0..15) in a[u].
+// Step K: the owner wavefront of column K takes the pivot from lane K with
+// v_readlane, scales the column (1 / sqrt from v_rsq_f64 and two Newton
+// steps) and publishes it in LDS (two alternating buffers, so one barrier
+// per step suffices); every wavefront then updates its columns right of K.
+// Entries above the diagonal are zero on return.  first <- K + 1 of the
+// first non-positive pivot (owner wavefronts only; combine across waves).
 template <int K>
-struct CholStep {
-  __device__ __forceinline__ static void run(double (&a)[64], int i, int kb, int& first) {
+struct CholColStep {
+  __device__ __forceinline__ static void run(double (&a)[16], int i, int w, int kb, int& first,
+                                             double (*sc)[64]) {
     if (K < kb) {
-      const double d = readlane_d<K>(a[K]);
-      if (!(d > 0.0) && first == 0) first = K + 1;
-      const double r = rsqrt_d(d);
-      a[K] = (i == K) ? d * r : a[K] * r;
+      constexpr int U = K >> 2, WK = K & 3;
+      if (w == WK) {
+        const double d = readlane_d<K>(a[U]);
+        if (!(d > 0.0) && first == 0) first = K + 1;
+        const double r = rsqrt_d(d);
+        a[U] = (i == K) ? d * r : (i > K ? a[U] * r : 0.0);
+        sc[K & 1][i] = a[U];
+      }
+      __syncthreads();
+      const double lik = sc[K & 1][i];
 #pragma unroll
-      for (int j = K + 1; j < 64; ++j) a[j] = fma(-a[K], __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(a[K]), j),
-                                                                          __builtin_amdgcn_readlane(__double2loint(a[K]), j)),
-                                                  a[j]);
-      CholStep<K + 1>::run(a, i, kb, first);
+      for (int u = 0; u < 16; ++u) {
+        const int j = w + 4 * u;
+        if (j > K) a[u] = fma(-lik, sc[K & 1][j], a[u]);
+      }
+      CholColStep<K + 1>::run(a, i, w, kb, first, sc);
     }
   }
 };
 template <>
-struct CholStep<64> {
-  __device__ __forceinline__ static void run(double (&)[64], int, int, int&) {}
+struct CholColStep<64> {
+  __device__ __forceinline__ static void run(double (&)[16], int, int, int, int&, double (*)[64]) {}
 };
 
 }  // namespace uhmd

@@ -18,33 +18,128 @@ constexpr int TILE = 64;
 constexpr int LDP = TILE + 1;
 
 // ------------------------------------------------------------------------
-// Cholesky of the kb x kb diagonal panel block, one wavefront per front:
-// lane i holds row i in registers (uhmd::CholStep, fully unrolled).
+// Cholesky of the kb x kb diagonal panel block, one workgroup of four
+// wavefronts per front (uhmd::CholColStep, fully unrolled), followed by the
+// inverse W = L11^-1.  W's strict lower part is stored transposed in the
+// strict upper triangle of the diagonal block, which the Cholesky path never
+// reads otherwise (only the lower triangle of a front is used); the panel
+// TRSM and the solves read it from there (its diagonal is 1 / L(k, k)).
 // ------------------------------------------------------------------------
+constexpr int LDW = TILE + 1;
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// One wavefront: C += X Y for 16 x 16 blocks X, Y in LDS (row strides ldx,
+// ldy), K = 16 as four v_mfma_f64_16x16x4f64.  Operand lanes: X(i, k) in lane
+// i + 16 k, Y(k, j) in lane j + 16 k; result register r of lane l holds
+// C(4 r + l / 16, l % 16).
+__device__ __forceinline__ d4 mma16_ld(const double* X, int ldx, const double* Y, int ldy, d4 c, int lane) {
+#pragma unroll
+  for (int k0 = 0; k0 < 16; k0 += 4) {
+    const int k = k0 + (lane >> 4), q = lane & 15;
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(X[q * ldx + k], Y[k * ldy + q], c, 0, 0, 0);
+  }
+  return c;
+}
+__device__ __forceinline__ d4 mma16(const double* X, const double* Y, int ld, d4 c, int lane) {
+  return mma16_ld(X, ld, Y, ld, c, lane);
+}
+
 template <typename T>
-__global__ __launch_bounds__(64) void k_chol_diag(const uhmk_front* __restrict__ F,
-                                                  const int* __restrict__ flist, int c0, int nb,
-                                                  T* __restrict__ arena, int* __restrict__ info) {
+__global__ __launch_bounds__(256) void k_chol_diag(const uhmk_front* __restrict__ F,
+                                                   const int* __restrict__ flist, int c0, int nb,
+                                                   T* __restrict__ arena, int* __restrict__ info) {
+  __shared__ double sc[2][64];
+  __shared__ double sL[TILE * LDW];
+  __shared__ double sW[TILE * LDW];
+  __shared__ double sS[4][16 * 17];
+  __shared__ int s_first[4];
   const int f = flist[blockIdx.x];
   const uhmk_front fr = F[f];
   const int n = fr.n, fs = fr.fs;
   const int kb = min(nb, fs - c0);
-  T* __restrict__ A = arena + fr.off;
-  const int i = threadIdx.x;
-  double a[TILE];
+  T* __restrict__ A = arena + fr.off + c0 + (long long)c0 * n;
+  const int i = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double a[16];
 #pragma unroll
-  for (int j = 0; j < TILE; ++j)
-    a[j] = (i < kb && j < kb && j <= i) ? (double)A[(c0 + i) + (long long)(c0 + j) * n] : 0.0;
+  for (int u = 0; u < 16; ++u) {
+    const int j = w + 4 * u;
+    a[u] = (i < kb && j < kb && j <= i) ? (double)A[i + (long long)j * n] : 0.0;
+  }
   int first = 0;
-  uhmd::CholStep<0>::run(a, i, kb, first);
-  if (i == 0 && first && info[f] == 0) info[f] = c0 + first;
+  uhmd::CholColStep<0>::run(a, i, w, kb, first, sc);
+  if (i == 0) s_first[w] = first;
 #pragma unroll
-  for (int j = 0; j < TILE; ++j)
-    if (i < kb && j < kb && j <= i) A[(c0 + i) + (long long)(c0 + j) * n] = (T)a[j];
+  for (int u = 0; u < 16; ++u) {
+    const int j = w + 4 * u;
+    sL[i * LDW + j] = a[u];
+    if (i < kb && j < kb && j <= i) A[i + (long long)j * n] = (T)a[u];
+  }
+  __syncthreads();
+  // W = L^-1 by 16 x 16 blocks: the four diagonal blocks are inverted in
+  // parallel (one wavefront each, lane c < 16 forms column c), then the
+  // blocks below the diagonal follow diagonal by diagonal,
+  // W(I,J) = -W(I,I) sum_{K=J}^{I-1} L(I,K) W(K,J), as 16 x 16 x 16 FP64
+  // MFMA products through LDS.
+  {
+    const int b0 = 16 * w;
+    if (i < 16) {
+      double wv[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        double acc = (k == i) ? 1.0 : 0.0;
+#pragma unroll
+        for (int m = 0; m < k; ++m) acc = fma(-sL[(b0 + k) * LDW + b0 + m], wv[m], acc);
+        const double d = sL[(b0 + k) * LDW + b0 + k];
+        wv[k] = (b0 + k < kb) ? acc / d : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < 16; ++k) sW[(b0 + k) * LDW + b0 + i] = wv[k];
+    }
+  }
+  __syncthreads();
+  for (int d = 1; d < 4; ++d) {
+    // wavefront w forms block (I, J) = (w + d, w) when it exists
+    const int I = w + d, J = w;
+    d4 S = d4{0.0, 0.0, 0.0, 0.0};
+    if (I < 4) {
+      // S = sum_K L(I,K) W(K,J)
+      for (int K = J; K < I; ++K) S = mma16(sL + (16 * I) * LDW + 16 * K, sW + (16 * K) * LDW + 16 * J, LDW, S, i);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sS[w][(4 * r + (i >> 4)) * 17 + (i & 15)] = S[r];
+    }
+    __syncthreads();
+    if (I < 4) {
+      d4 Wb = d4{0.0, 0.0, 0.0, 0.0};
+      Wb = mma16_ld(sW + (16 * I) * LDW + 16 * I, LDW, sS[w], 17, Wb, i);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sW[(16 * I + 4 * r + (i >> 4)) * LDW + 16 * J + (i & 15)] = -Wb[r];
+    }
+    __syncthreads();
+  }
+  double wc[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) wc[u] = sW[i * LDW + w + 4 * u];
+  // strict lower part of W, transposed into the strict upper triangle
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int j = w + 4 * u;
+    if (i < kb && j < i) A[j + (long long)i * n] = (T)wc[u];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && info[f] == 0) {
+    int k1 = 0;
+    for (int q = 0; q < 4; ++q)
+      if (s_first[q] && (k1 == 0 || s_first[q] < k1)) k1 = s_first[q];
+    if (k1) info[f] = c0 + k1;
+  }
 }
 
-// Cholesky panel rows [i0, i0+64) below the diagonal block: X <- X L11^-T.
-// 16 rows per wavefront, 4 lanes per row (tri_solve64).
+// Cholesky panel rows [b, b+64) below the diagonal block: X <- X L11^-T =
+// X W^T with W = L11^-1 from the diagonal block (strict upper triangle:
+// W transposed; diagonal: 1 / L(k,k)).  One 64 x 64 x kb FP64 MFMA product
+// per workgroup; each wavefront owns a 32 x 32 quarter of the result.
+constexpr int LDT = TILE + 4;
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_chol_trsm(const uhmk_front* __restrict__ F,
                                                    const uhmk_item* __restrict__ items, int c0, int nb,
@@ -54,25 +149,56 @@ __global__ __launch_bounds__(256) void k_chol_trsm(const uhmk_front* __restrict_
   const int n = fr.n, fs = fr.fs;
   const int kb = min(nb, fs - c0);
   T* __restrict__ A = arena + fr.off;
-  __shared__ double s_T[TILE * LDP];
-  __shared__ double s_rd[TILE];
+  __shared__ double sWt[TILE * LDT];  // sWt[k][j] = W(j, k)
+  __shared__ double sXt[TILE * LDT];  // sXt[k][i] = X(i, k)
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  uhmd::load_tri64<true>(s_T, LDP, s_rd, A + c0 + (long long)c0 * n, n, kb, tid, 256);
-  __syncthreads();
-  const int row = it.b + 16 * w + (lane >> 2), q = lane & 3;
-  const bool live = row < n;
-  double x[16];
+  const T* __restrict__ D = A + c0 + (long long)c0 * n;
+  double vw[16], vx[16];
 #pragma unroll
-  for (int cr = 0; cr < 16; ++cr) {
-    const int c = 16 * q + cr;
-    x[cr] = (live && c < kb) ? (double)A[row + (long long)(c0 + c) * n] : 0.0;
+  for (int u = 0; u < 16; ++u) {
+    const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
+    vw[u] = (r < kb && c < kb && r <= c) ? (double)D[r + (long long)c * n] : 0.0;
+    const int row = it.b + r;
+    vx[u] = (row < n && c < kb) ? (double)A[row + (long long)(c0 + c) * n] : 0.0;
   }
-  uhmd::tri_solve64<false, false>(x, s_T, LDP, s_rd, kb, lane);
-  if (live)
 #pragma unroll
-    for (int cr = 0; cr < 16; ++cr) {
-      const int c = 16 * q + cr;
-      if (c < kb) A[row + (long long)(c0 + c) * n] = (T)x[cr];
+  for (int u = 0; u < 16; ++u) {
+    const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
+    // block entry (r, c), r <= c: W(c, r) (r < c) or 1 / L(c, c) (r == c)
+    sWt[r * LDT + c] = (r == c && r < kb) ? 1.0 / vw[u] : vw[u];
+    sXt[c * LDT + r] = vx[u];
+  }
+  __syncthreads();
+  const int iw = (w & 1) * 32, jw = (w >> 1) * 32, cl = lane & 15;
+  d4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k0 = 0; k0 < TILE; k0 += 4) {
+    const int kr = k0 + (lane >> 4);
+    double wo[2], xo[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) wo[b] = sWt[kr * LDT + jw + 16 * b + cl];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) xo[a] = sXt[kr * LDT + iw + 16 * a + cl];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(wo[b], xo[a], acc[a][b], 0, 0, 0);
+  }
+  // accumulator register r of lane l: D(j, i) with j = 4 r + l / 16, i = l % 16
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int row = it.b + iw + 16 * a + cl;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = jw + 16 * b + 4 * r + (lane >> 4);
+        if (row < n && j < kb) A[row + (long long)(c0 + j) * n] = (T)acc[a][b][r];
+      }
     }
 }
 
@@ -93,7 +219,6 @@ __global__ __launch_bounds__(256) void k_chol_trsm(const uhmk_front* __restrict_
 // product is formed transposed (D = B^T A^T) so that a 16-lane group
 // stores 16 consecutive rows of C.
 // ------------------------------------------------------------------------
-typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 constexpr int KC = 16;
 
@@ -241,10 +366,10 @@ int uhmk_chol_diag(const uhmk_front* fronts, const int* flist, int n_fronts, int
   if (n_fronts <= 0) return 0;
   if (nb > TILE || nb < 1) return -1000;
   if (prec == 0)
-    hipLaunchKernelGGL(k_chol_diag<double>, dim3(n_fronts), dim3(64), 0, stream, fronts, flist, c0, nb,
+    hipLaunchKernelGGL(k_chol_diag<double>, dim3(n_fronts), dim3(256), 0, stream, fronts, flist, c0, nb,
                        static_cast<double*>(arena), info);
   else if (prec == 1)
-    hipLaunchKernelGGL(k_chol_diag<float>, dim3(n_fronts), dim3(64), 0, stream, fronts, flist, c0, nb,
+    hipLaunchKernelGGL(k_chol_diag<float>, dim3(n_fronts), dim3(256), 0, stream, fronts, flist, c0, nb,
                        static_cast<float*>(arena), info);
   else
     return -1006;

@@ -410,7 +410,7 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
   __shared__ int s_cnt;
 
   if (it.a == 1) {
-    uhmd::load_tri64<false>(s_T, LDP, s_rd, A + c0 + (long long)c0 * n, n, kb, tid, 256);
+    uhmd::load_tri64<false, 256>(s_T, LDP, s_rd, A + c0 + (long long)c0 * n, n, kb, tid);
     __syncthreads();
     const int row = it.b + 16 * w + (lane >> 2);
     const bool live = row < n;
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
     const int k = kb + idx % (pc - kb), t = idx / (pc - kb);
     A[s_prow[k] + (long long)(j0 + t) * n] = (T)s_X[s_psrc[k]][t];
   }
-  uhmd::load_tri64<true>(s_T, LDP, s_rd, A + c0 + (long long)c0 * n, n, kb, tid, 256);
+  uhmd::load_tri64<true, 256>(s_T, LDP, s_rd, A + c0 + (long long)c0 * n, n, kb, tid);
   __syncthreads();
   const int t = 16 * w + (lane >> 2);
   double x[16];
