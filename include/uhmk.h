@@ -82,11 +82,15 @@ int uhmk_lu_rows(const uhmk_front* fronts, const uhmk_item* items, int n_items, 
                  int prec, const int* piv, hipStream_t stream);
 
 /* Cholesky of the diagonal panel block (FLA_Chol in linal::dense::chol,
- * LINAL/src/dense/chol.cxx:67).  info[f] <- first non-positive pivot + 1. */
+ * LINAL/src/dense/chol.cxx:67) and its inverse W = L11^-1, whose strict
+ * lower part is stored transposed in the strict upper triangle of the block
+ * (never read as matrix data on the Cholesky path).  info[f] <- first
+ * non-positive pivot + 1. */
 int uhmk_chol_diag(const uhmk_front* fronts, const int* flist, int n_fronts, int c0, int nb, void* arena, int prec,
                    int* info, hipStream_t stream);
 
-/* Panel rows below the diagonal block: X <- X L11^-T (trsm right lower
+/* Panel rows below the diagonal block: X <- X L11^-T = X W^T with the
+ * inverse from uhmk_chol_diag, as an FP64 MFMA product (trsm right lower
  * transpose, LINAL/src/dense/chol.cxx:80-84).  items: {f, 0, first row, 0}. */
 int uhmk_chol_trsm(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, void* arena,
                    int prec, hipStream_t stream);
@@ -124,6 +128,8 @@ int uhmk_solve_gather(const uhmk_front* fronts, const uhmk_item* items, int n_it
  * workgroup per front; fronts with fs <= sb * sb_size are skipped;
  * sb_size a multiple of 64, at most 1024): fwd = 1: x_t <- L^-1 x_t (unit
  * lower for LU); fwd = 0: x_t <- U^-1 x_t (LU) or L^-T x_t (Cholesky).
+ * chol: 0 LU, 1 Cholesky, 2 Cholesky whose 64 x 64 diagonal blocks hold
+ * the inverse written by uhmk_chol_diag (applied as a GEMV per block).
  * With sb_size >= fs this is the whole ATL solve; otherwise the couplings
  * between super-blocks go through uhmk_solve_gemv.  arena holds the fronts
  * in FP64 (prec 0) or FP32 (prec 1); arithmetic is FP64.  Replaces the
@@ -135,12 +141,25 @@ int uhmk_solve_trsv(const uhmk_front* fronts, const int* flist, int n_fronts, in
 /* Off-diagonal solve updates.  cols = 0: x[rows] -= F[rows, a:c) x[a:c)
  * over row chunks, items {f, a, first row, c}, rows [b, min(b + RT,
  * b < fs ? fs : n)), RT = 16 (FP64) or 32 (FP32): xb -= ABL x_t, x_t -=
- * ATR xb and super-block couplings.  cols = 1: x[col] -= sum over rows
- * [a, c) of F(row, col) x[row] for columns [b, min(b + 16, fs)), items
- * {f, a, first column, c}: x_t -= ABL^T xb and the Cholesky backward
- * couplings. */
+ * ATR xb and super-block couplings (dst must be x or null).  cols = 1:
+ * dst[col] -= sum over rows [a, c) of F(row, col) x[row] for columns
+ * [b, min(b + 16, fs)), items {f, a, first column, c}: x_t -= ABL^T xb and
+ * the Cholesky backward couplings (dst: x when null). */
 int uhmk_solve_gemv(const uhmk_front* fronts, const uhmk_item* items, int n_items, int cols, const void* arena,
-                    int prec, double* x, int nrhs, hipStream_t stream);
+                    int prec, double* x, double* dst, int nrhs, hipStream_t stream);
+
+/* One 64-row block step k0 of the Cholesky solves of a tree level, for
+ * factors whose diagonal blocks carry the inverse from uhmk_chol_diag
+ * (solve_chol_{1,2}_hier, UHM/src/matrix/uhm/fla/chol/solve.cxx:152-202).
+ * fwd = 1: y = W x[k0:k0+64) goes to z, and x[rows] -= L[rows, block] y for
+ * the 64-row chunk of the item (items {f, 0, first row, 0}: k0 for the
+ * block itself, else k0 + kb + 64 m with kb = min(64, fs - k0); rows run
+ * into ABL, so the xb update is included).  fwd = 0: y = W^T z[k0:k0+64) goes to x, and
+ * z[cols] -= L[block, cols]^T y for the 64-column chunk of the item (items
+ * {f, 0, first column <= k0, 0}).  x and z have the front layout of the
+ * right-hand sides. */
+int uhmk_solve_chol_step(const uhmk_front* fronts, const uhmk_item* items, int n_items, int fwd, int k0,
+                         const void* arena, int prec, double* x, double* z, int nrhs, hipStream_t stream);
 
 /* Backward branch: xb <- the parent's solution rows through emap
  * (Helper_::_branch_rhs, helper.hxx:302-331).  items {f, 0, first row, 0},

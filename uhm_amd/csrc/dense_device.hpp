@@ -5,6 +5,27 @@
 
 namespace uhmd {
 
+// p[idx] when ok, else zero, without a branch: the address falls back to
+// p[0] (which must be a valid address) and the value is selected after the
+// load, so that a group of such loads issues back to back (a conditional
+// load followed by a conversion would otherwise wait inside its branch).
+template <typename T>
+__device__ __forceinline__ double ldz(const T* p, long long idx, bool ok) {
+  const T v = p[ok ? idx : 0];
+  return ok ? (double)v : 0.0;
+}
+template <typename TM, typename T>
+__device__ __forceinline__ TM ldz_as(const T* p, long long idx, bool ok) {
+  if constexpr (sizeof(TM) == sizeof(T)) {
+    // no conversion: a predicated load keeps the address arithmetic off the
+    // critical path
+    return ok ? (TM)p[idx] : (TM)0;
+  } else {
+    const T v = p[ok ? idx : 0];
+    return ok ? (TM)v : (TM)0;
+  }
+}
+
 // Broadcast of a double from lane (l & ~3) + KQ to the four lanes of its
 // quad (DPP quad_perm, no LDS round trip).
 template <int KQ>
@@ -92,7 +113,7 @@ __device__ __forceinline__ void load_tri64(double* __restrict__ s, int ld, doubl
 #pragma unroll
   for (int u = 0; u < PER; ++u) {
     const int idx = tid + NT * u, r = idx & 63, c = idx >> 6;
-    v[u] = (r < kb && c < kb && (LOWER ? r >= c : r <= c)) ? (double)A[r + c * n] : 0.0;
+    v[u] = ldz(A, r + c * n, r < kb && c < kb && (LOWER ? r >= c : r <= c));
   }
 #pragma unroll
   for (int u = 0; u < PER; ++u) {

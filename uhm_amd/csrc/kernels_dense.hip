@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void k_chol_diag(const uhmk_front* __restrict_
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
     const int j = w + 4 * u;
-    a[u] = (i < kb && j < kb && j <= i) ? (double)A[i + (long long)j * n] : 0.0;
+    a[u] = uhmd::ldz(A, i + (long long)j * n, i < kb && j < kb && j <= i);
   }
   int first = 0;
   uhmd::CholColStep<0>::run(a, i, w, kb, first, sc);
@@ -157,9 +157,9 @@ __global__ __launch_bounds__(256) void k_chol_trsm(const uhmk_front* __restrict_
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
     const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
-    vw[u] = (r < kb && c < kb && r <= c) ? (double)D[r + (long long)c * n] : 0.0;
+    vw[u] = uhmd::ldz(D, r + (long long)c * n, r < kb && c < kb && r <= c);
     const int row = it.b + r;
-    vx[u] = (row < n && c < kb) ? (double)A[row + (long long)(c0 + c) * n] : 0.0;
+    vx[u] = uhmd::ldz(A, row + (long long)(c0 + c) * n, row < n && c < kb);
   }
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
@@ -242,7 +242,7 @@ struct Mfma<float> {
 };
 
 template <bool TRANSB, int T, typename TS, typename TM>
-__global__ __launch_bounds__(256) void k_gemm(const uhmk_front* __restrict__ F,
+__global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ F,
                                               const uhmk_item* __restrict__ items, int c0, int nb, int schur,
                                               TS* __restrict__ arena) {
   typedef typename Mfma<TM>::acc_t acc_t;
@@ -279,19 +279,19 @@ __global__ __launch_bounds__(256) void k_gemm(const uhmk_front* __restrict__ F,
 #pragma unroll
     for (int u = 0; u < NL; ++u) {
       const int k = k0 + lk + KR * u;
-      ra[u] = (i0 + li < i1 && k < kend) ? (TM)M[(i0 + li) + (long long)k * n] : (TM)0;
+      ra[u] = uhmd::ldz_as<TM>(M, (i0 + li) + (long long)k * n, i0 + li < i1 && k < kend);
     }
     if (TRANSB) {
 #pragma unroll
       for (int u = 0; u < NL; ++u) {
         const int k = k0 + lk + KR * u;
-        rb[u] = (j0 + li < j1 && k < kend) ? (TM)M[(j0 + li) + (long long)k * n] : (TM)0;
+        rb[u] = uhmd::ldz_as<TM>(M, (j0 + li) + (long long)k * n, j0 + li < j1 && k < kend);
       }
     } else {
 #pragma unroll
       for (int u = 0; u < NL; ++u) {
         const int j = bj + 16 * u;
-        rb[u] = (j0 + j < j1 && k0 + bk < kend) ? (TM)M[(k0 + bk) + (long long)(j0 + j) * n] : (TM)0;
+        rb[u] = uhmd::ldz_as<TM>(M, (k0 + bk) + (long long)(j0 + j) * n, j0 + j < j1 && k0 + bk < kend);
       }
     }
   };

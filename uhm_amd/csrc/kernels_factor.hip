@@ -138,7 +138,7 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__
     for (int q = 0; q < R; ++q) {
       const int r = rb + tid + PNT * q;
 #pragma unroll
-      for (int j = 0; j < PIB; ++j) v[q][j] = (r < fs && j < ibw) ? (double)A[r + (long long)(rb + j) * n] : 0.0;
+      for (int j = 0; j < PIB; ++j) v[q][j] = uhmd::ldz(A, r + (long long)(rb + j) * n, r < fs && j < ibw);
     }
 #pragma unroll
     for (int j = 0; j < PIB; ++j) {
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__
         const int col = rb + ibw + t;
         double u[PIB];
 #pragma unroll
-        for (int jj = 0; jj < PIB; ++jj) u[jj] = (jj < ibw) ? (double)A[rb + jj + (long long)col * n] : 0.0;
+        for (int jj = 0; jj < PIB; ++jj) u[jj] = uhmd::ldz(A, rb + jj + (long long)col * n, jj < ibw);
 #pragma unroll
         for (int jj = 1; jj < PIB; ++jj)
 #pragma unroll
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__
             double a[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u)
-              a[u] = (t0 + u < nr) ? (double)A[r + (long long)(rb + ibw + t0 + u) * n] : 0.0;
+              a[u] = uhmd::ldz(A, r + (long long)(rb + ibw + t0 + u) * n, t0 + u < nr);
 #pragma unroll
             for (int u = 0; u < 8; ++u)
 #pragma unroll
@@ -418,7 +418,7 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
 #pragma unroll
     for (int cr = 0; cr < 16; ++cr) {
       const int c = 16 * q + cr;
-      x[cr] = (live && c < kb) ? (double)A[row + (long long)(c0 + c) * n] : 0.0;
+      x[cr] = uhmd::ldz(A, row + (long long)(c0 + c) * n, live && c < kb);
     }
     uhmd::tri_solve64<false, true>(x, s_T, LDP, s_rd, kb, lane);
     if (live)

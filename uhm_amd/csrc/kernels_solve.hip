@@ -14,6 +14,7 @@
 
 #include <algorithm>
 
+#include "dense_device.hpp"
 #include "uhmk.h"
 
 namespace {
@@ -130,16 +131,22 @@ __device__ __forceinline__ double ld(const T* p) {
 // x <- U^-1 x (LU) or L^-T x (Cholesky).  The couplings with the rest of
 // ATL are applied by the gemv kernels between super-blocks.  One workgroup
 // of NT threads per front; the super-block's part of x lives in LDS and is
-// processed in 64-row blocks: one wavefront solves the 64 x 64 triangle
-// (TrsvStep), the workgroup updates the rest of the super-block.
+// processed in 64-row blocks.  With INV (Cholesky factors from 64-wide
+// panels) the diagonal block holds W = L(k,k)^-1 in its strict upper
+// triangle (transposed) and L's diagonal, and the block step is a GEMV with
+// W (forward) or W^T (backward); otherwise one wavefront solves the 64 x 64
+// triangle step by step (TrsvStep).  Each block's matrix loads are issued
+// before its barriers so that they overlap.
 constexpr int SB_MAX = 1024;
 
-template <typename T, bool CHOL, bool FWD, int NT>
+template <typename T, bool CHOL, bool FWD, bool INV, int NT>
 __global__ __launch_bounds__(NT) void k_sv_trsv(const uhmk_front* __restrict__ F, const int* __restrict__ flist,
                                                 const T* __restrict__ arena, double* __restrict__ x, int nrhs,
                                                 int sb, int SB) {
+  constexpr bool STAGE = INV || (!FWD && CHOL);
+  constexpr int NPRE = 64 * 64 / NT;
   __shared__ double sx[SB_MAX];
-  __shared__ double sL[(!FWD && CHOL) ? 64 * 65 : 1];
+  __shared__ double sL[STAGE ? 64 * 65 : 1];
   const uhmk_front fr = F[flist[blockIdx.x]];
   const int n = fr.n, fs = fr.fs;
   const int s0 = sb * SB;
@@ -157,16 +164,16 @@ __global__ __launch_bounds__(NT) void k_sv_trsv(const uhmk_front* __restrict__ F
       const int bk = FWD ? bq : nblk - 1 - bq;
       const int l0 = 64 * bk, kb = min(64, m - l0), l1 = l0 + kb;  // local rows
       const int k0 = s0 + l0;                                       // absolute
-      if (!FWD && CHOL) {
-        // the diagonal block goes to LDS (transposed reads below); its loads
-        // are issued first so that they overlap the dot products
-        constexpr int NPRE = 64 * 64 / NT;
-        double pre[NPRE];
+      // ---- loads that do not depend on x
+      double pre[STAGE ? NPRE : 1];
+      if (STAGE) {
 #pragma unroll
         for (int u = 0; u < NPRE; ++u) {
           const int idx = tid + NT * u, r = idx & 63, cc = idx >> 6;
-          pre[u] = (r < kb && cc < kb) ? ld(A + (k0 + r) + (long long)(k0 + cc) * n) : 0.0;
+          pre[u] = uhmd::ldz(A, (k0 + r) + (long long)(k0 + cc) * n, r < kb && cc < kb);
         }
+      }
+      if (!FWD && CHOL) {
         // x_q -= sum_{i in [l1, m)} L(i, q) x_i, NW columns at a time per wave
         double acc[64 / NW];
 #pragma unroll
@@ -188,7 +195,8 @@ __global__ __launch_bounds__(NT) void k_sv_trsv(const uhmk_front* __restrict__ F
           const int q = wid + NW * u;
           if (lane == 0 && q < kb) sx[l0 + q] -= a;
         }
-        // stage the diagonal block so that lane j can read column j
+      }
+      if (STAGE) {
 #pragma unroll
         for (int u = 0; u < NPRE; ++u) {
           const int idx = tid + NT * u, r = idx & 63, cc = idx >> 6;
@@ -197,34 +205,50 @@ __global__ __launch_bounds__(NT) void k_sv_trsv(const uhmk_front* __restrict__ F
         __syncthreads();
       }
       if (tid < 64) {
-        double e[64];
-        double rd = 1.0;
         const int j = lane;
-        if (FWD) {
-          // row j of the block, strict lower part
-#pragma unroll
-          for (int q = 0; q < 64; ++q) e[q] = (j < kb && q < j) ? ld(A + (k0 + j) + (long long)(k0 + q) * n) : 0.0;
-          if (CHOL && j < kb) rd = 1.0 / ld(A + (k0 + j) + (long long)(k0 + j) * n);
-        } else if (CHOL) {
-          // column j of the block below the diagonal: e[i] = L(i, j), i > j
-#pragma unroll
-          for (int q = 0; q < 64; ++q) e[q] = (q > j) ? sL[q * 65 + j] : 0.0;
-          if (j < kb) rd = 1.0 / sL[j * 65 + j];
+        if (INV) {
+          // block entry (k, j), k < j: W(j, k); diagonal: L(j, j)
+          double y = 0.0;
+          if (j < kb) {
+            if (FWD) {
+              y = sx[l0 + j] / sL[j * 65 + j];
+              for (int k = 0; k < j; ++k) y = fma(sL[k * 65 + j], sx[l0 + k], y);
+            } else {
+              y = sx[l0 + j] / sL[j * 65 + j];
+              for (int k = j + 1; k < kb; ++k) y = fma(sL[j * 65 + k], sx[l0 + k], y);
+            }
+          }
+          __builtin_amdgcn_wave_barrier();
+          if (j < kb) sx[l0 + j] = y;
         } else {
-          // row j of U, strict upper part
+          double e[64];
+          double rd = 1.0;
+          if (FWD) {
+            // row j of the block, strict lower part
 #pragma unroll
-          for (int q = 0; q < 64; ++q)
-            e[q] = (j < kb && q > j && q < kb) ? ld(A + (k0 + j) + (long long)(k0 + q) * n) : 0.0;
-          if (j < kb) rd = 1.0 / ld(A + (k0 + j) + (long long)(k0 + j) * n);
+            for (int q = 0; q < 64; ++q) e[q] = uhmd::ldz(A, (k0 + j) + (long long)(k0 + q) * n, j < kb && q < j);
+            if (CHOL && j < kb) rd = 1.0 / ld(A + (k0 + j) + (long long)(k0 + j) * n);
+          } else if (CHOL) {
+            // column j of the block below the diagonal: e[i] = L(i, j), i > j
+#pragma unroll
+            for (int q = 0; q < 64; ++q) e[q] = (q > j) ? sL[q * 65 + j] : 0.0;
+            if (j < kb) rd = 1.0 / sL[j * 65 + j];
+          } else {
+            // row j of U, strict upper part
+#pragma unroll
+            for (int q = 0; q < 64; ++q)
+              e[q] = uhmd::ldz(A, (k0 + j) + (long long)(k0 + q) * n, j < kb && q > j && q < kb);
+            if (j < kb) rd = 1.0 / ld(A + (k0 + j) + (long long)(k0 + j) * n);
+          }
+          double xj = (j < kb) ? sx[l0 + j] : 0.0;
+          if (FWD) {
+            if (CHOL) TrsvStep<0, true, true>::run(xj, e, rd, lane, kb);
+            else TrsvStep<0, true, false>::run(xj, e, rd, lane, kb);
+          } else {
+            TrsvStep<0, false, true>::run(xj, e, rd, lane, kb);
+          }
+          if (j < kb) sx[l0 + j] = xj;
         }
-        double xj = (j < kb) ? sx[l0 + j] : 0.0;
-        if (FWD) {
-          if (CHOL) TrsvStep<0, true, true>::run(xj, e, rd, lane, kb);
-          else TrsvStep<0, true, false>::run(xj, e, rd, lane, kb);
-        } else {
-          TrsvStep<0, false, true>::run(xj, e, rd, lane, kb);
-        }
-        if (j < kb) sx[l0 + j] = xj;
       }
       __syncthreads();
       if (FWD || !CHOL) {
@@ -234,8 +258,11 @@ __global__ __launch_bounds__(NT) void k_sv_trsv(const uhmk_front* __restrict__ F
           const T* __restrict__ Ai = A + (s0 + i) + (long long)k0 * n;
           double acc = 0.0;
           if (kb == 64) {
+            double e[64];
 #pragma unroll
-            for (int q = 0; q < 64; ++q) acc = fma(ld(Ai + (long long)q * n), sx[l0 + q], acc);
+            for (int q = 0; q < 64; ++q) e[q] = ld(Ai + (long long)q * n);
+#pragma unroll
+            for (int q = 0; q < 64; ++q) acc = fma(e[q], sx[l0 + q], acc);
           } else {
             for (int q = 0; q < kb; ++q) acc = fma(ld(Ai + (long long)q * n), sx[l0 + q], acc);
           }
@@ -288,12 +315,14 @@ __global__ __launch_bounds__(256) void k_sv_gemv_rows(const uhmk_front* __restri
 
 // Column-chunk transposed GEMV: x[col] -= sum_{i in [rbeg, rend)} A(i, col)
 // x[i] for 16 columns [c0, min(c0 + 16, fs)), four per wavefront at once.
-// item {f, rbeg, c0, rend}.  Serves x_t -= ABL^T xb and the super-block
+// item {f, rbeg, c0, rend}; the result goes to dst (the same front layout
+// as x; dst may be x).  Serves x_t -= ABL^T xb and the super-block
 // couplings of the Cholesky backward solve.
 template <typename T>
 __global__ __launch_bounds__(256) void k_sv_gemv_cols(const uhmk_front* __restrict__ F,
                                                       const uhmk_item* __restrict__ items,
-                                                      const T* __restrict__ arena, double* __restrict__ x, int nrhs) {
+                                                      const T* __restrict__ arena, double* __restrict__ x, int nrhs,
+                                                      double* __restrict__ dst) {
   const uhmk_item it = items[blockIdx.x];
   const uhmk_front fr = F[it.f];
   const int n = fr.n, fs = fr.fs;
@@ -303,6 +332,7 @@ __global__ __launch_bounds__(256) void k_sv_gemv_cols(const uhmk_front* __restri
   for (int c = 0; c < nrhs; ++c) {
     double* __restrict__ xv = x + fr.xoff + (long long)c * n;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
     for (int i = it.a + lane; i < it.c; i += 64) {
       const double xi = xv[i];
 #pragma unroll
@@ -317,8 +347,167 @@ __global__ __launch_bounds__(256) void k_sv_gemv_cols(const uhmk_front* __restri
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
       const int col = it.b + w + 4 * u;
-      if (lane == 0 && col < cend) xv[col] -= a;
+      if (lane == 0 && col < cend) dst[fr.xoff + (long long)c * n + col] -= a;
     }
+  }
+}
+
+// ------------------------------------------------------------------------
+// Cholesky solves by 64-row block steps, for factors whose 64 x 64 diagonal
+// blocks carry W = L(k,k)^-1 (strict upper triangle, transposed; diagonal
+// L(k,k)).  One launch per block step k0 of a tree level; every workgroup
+// forms y = W x_k (forward) or y = W^T z_k (backward) itself and then
+// updates one 64-row (forward) or 64-column (backward) chunk, so a big
+// front is spread over many workgroups.  Two front-layout vectors keep the
+// steps free of read/write overlap: forward reads x, writes the solved
+// block to z and updates x below it; backward reads z, writes the solved
+// block to x and updates z above it.
+// ------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void stage_diag_block(double* __restrict__ sW, const T* __restrict__ D, long long n, int kb,
+                                                 int tid) {
+  double v[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
+    v[u] = uhmd::ldz(D, r + (long long)c * n, r < kb && c < kb);
+  }
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
+    sW[r * 65 + c] = v[u];
+  }
+}
+
+// item {f, 0, first row of the chunk, 0}: r0 == k0 is the block itself,
+// the others start at k0 + kb, k0 + kb + 64, ... and run into ABL.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sv_chol_fwd_step(const uhmk_front* __restrict__ F,
+                                                          const uhmk_item* __restrict__ items,
+                                                          const T* __restrict__ arena, double* __restrict__ x,
+                                                          double* __restrict__ z, int nrhs, int k0) {
+  __shared__ double sW[64 * 65];
+  __shared__ double sy[64];
+  __shared__ double sp[4][64];
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs;
+  const int kb = min(64, fs - k0);
+  const T* __restrict__ A = arena + fr.off;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int r0 = it.b;
+  const bool own = (r0 == k0);
+  // chunk rows: this thread's row lane, quarter w of the 64 columns
+  const int row = r0 + lane;
+  double e[16];
+  if (!own) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = 16 * w + q;
+      e[q] = uhmd::ldz(A, row + (long long)(k0 + k) * n, row < n && k < kb);
+    }
+  }
+  stage_diag_block(sW, A + k0 + (long long)k0 * n, n, kb, tid);
+  __syncthreads();
+  __shared__ double sv[64];
+  for (int c = 0; c < nrhs; ++c) {
+    if (tid < 64) sv[tid] = uhmd::ldz(x + fr.xoff + (long long)c * n + k0, tid, tid < kb);
+    __syncthreads();
+    const double* __restrict__ xs = sv;
+    // y_j = x_j / L(j,j) + sum_{k<j} W(j,k) x_k; four threads per j
+    {
+      const int j = lane, part = w;
+      double acc = 0.0;
+      if (j < kb) {
+        for (int k = 16 * part; k < 16 * part + 16 && k < j; ++k) acc = fma(sW[k * 65 + j], xs[k], acc);
+        if (part == 0) acc += xs[j] / sW[j * 65 + j];
+      }
+      sp[part][j] = acc;
+    }
+    __syncthreads();
+    if (tid < 64) sy[tid] = (sp[0][tid] + sp[1][tid]) + (sp[2][tid] + sp[3][tid]);
+    __syncthreads();
+    if (own) {
+      if (tid < kb) z[fr.xoff + (long long)c * n + k0 + tid] = sy[tid];
+    } else {
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc = fma(e[q], sy[16 * w + q], acc);
+      sp[w][lane] = acc;
+      __syncthreads();
+      if (tid < 64 && row < n)
+        x[fr.xoff + (long long)c * n + row] -= (sp[0][tid] + sp[1][tid]) + (sp[2][tid] + sp[3][tid]);
+    }
+    __syncthreads();
+  }
+}
+
+// item {f, 0, first column of the chunk (<= k0), 0}; the chunk starting at
+// k0 is the block itself, the others lie left of it.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sv_chol_bwd_step(const uhmk_front* __restrict__ F,
+                                                          const uhmk_item* __restrict__ items,
+                                                          const T* __restrict__ arena, double* __restrict__ x,
+                                                          double* __restrict__ z, int nrhs, int k0) {
+  __shared__ double sW[64 * 65];
+  __shared__ double sT[64 * 65];
+  __shared__ double sy[64];
+  __shared__ double sp[4][64];
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs;
+  const int kb = min(64, fs - k0);
+  const T* __restrict__ A = arena + fr.off;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int c0 = it.b;
+  const bool own = (c0 == k0);
+  if (!own) {
+    // L(k0 + r, c0 + c): rows of the block, columns of the chunk
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int idx = tid + 256 * u, r = idx & 63, cc = idx >> 6;
+      v[u] = uhmd::ldz(A, (k0 + r) + (long long)(c0 + cc) * n, r < kb);
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int idx = tid + 256 * u, r = idx & 63, cc = idx >> 6;
+      sT[r * 65 + cc] = v[u];
+    }
+  }
+  stage_diag_block(sW, A + k0 + (long long)k0 * n, n, kb, tid);
+  __syncthreads();
+  __shared__ double sv[64];
+  for (int c = 0; c < nrhs; ++c) {
+    if (tid < 64) sv[tid] = uhmd::ldz(z + fr.xoff + (long long)c * n + k0, tid, tid < kb);
+    __syncthreads();
+    const double* __restrict__ zs = sv;
+    // y_j = z_j / L(j,j) + sum_{k>j} W(k,j) z_k; W(k, j) is block entry (j, k)
+    {
+      const int j = lane, part = w;
+      double acc = 0.0;
+      if (j < kb) {
+        for (int k = max(j + 1, 16 * part); k < 16 * part + 16 && k < kb; ++k) acc = fma(sW[j * 65 + k], zs[k], acc);
+        if (part == 0) acc += zs[j] / sW[j * 65 + j];
+      }
+      sp[part][j] = acc;
+    }
+    __syncthreads();
+    if (tid < 64) sy[tid] = (sp[0][tid] + sp[1][tid]) + (sp[2][tid] + sp[3][tid]);
+    __syncthreads();
+    if (own) {
+      if (tid < kb) x[fr.xoff + (long long)c * n + k0 + tid] = sy[tid];
+    } else {
+      // z(c0 + cc) -= sum_r L(k0 + r, c0 + cc) y_r; thread (w, lane): column
+      // lane, rows 16 w .. 16 w + 15
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) acc = fma(sT[(16 * w + q) * 65 + lane], sy[16 * w + q], acc);
+      sp[w][lane] = acc;
+      __syncthreads();
+      if (tid < 64) z[fr.xoff + (long long)c * n + c0 + tid] -= (sp[0][tid] + sp[1][tid]) + (sp[2][tid] + sp[3][tid]);
+    }
+    __syncthreads();
   }
 }
 
@@ -512,27 +701,30 @@ __global__ __launch_bounds__(256) void k_gather_rhs(long long nrow, const long l
 
 inline int ret(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
 
-template <typename T, bool CHOL, bool FWD>
+template <typename T, bool CHOL, bool FWD, bool INV>
 int launch_trsv(const uhmk_front* fronts, const int* flist, int n_fronts, int max_fs, const T* arena, double* x,
                 int nrhs, int sb, int SB, hipStream_t stream) {
   if (SB > SB_MAX || SB < 64 || SB % 64) return -1008;
   if (std::min(max_fs, SB) > 256)
-    hipLaunchKernelGGL((k_sv_trsv<T, CHOL, FWD, 512>), dim3(n_fronts), dim3(512), 0, stream, fronts, flist, arena, x,
-                       nrhs, sb, SB);
+    hipLaunchKernelGGL((k_sv_trsv<T, CHOL, FWD, INV, 512>), dim3(n_fronts), dim3(512), 0, stream, fronts, flist, arena,
+                       x, nrhs, sb, SB);
   else
-    hipLaunchKernelGGL((k_sv_trsv<T, CHOL, FWD, 256>), dim3(n_fronts), dim3(256), 0, stream, fronts, flist, arena, x,
-                       nrhs, sb, SB);
+    hipLaunchKernelGGL((k_sv_trsv<T, CHOL, FWD, INV, 256>), dim3(n_fronts), dim3(256), 0, stream, fronts, flist, arena,
+                       x, nrhs, sb, SB);
   return ret(hipGetLastError());
 }
 
 template <typename T>
-int trsv_any(int chol, int fwd, const uhmk_front* fronts, const int* flist, int n_fronts, int max_fs,
+int trsv_any(int chol, int fwd, int inv, const uhmk_front* fronts, const int* flist, int n_fronts, int max_fs,
              const T* arena, double* x, int nrhs, int sb, int SB, hipStream_t stream) {
+  if (chol && inv)
+    return fwd ? launch_trsv<T, true, true, true>(fronts, flist, n_fronts, max_fs, arena, x, nrhs, sb, SB, stream)
+               : launch_trsv<T, true, false, true>(fronts, flist, n_fronts, max_fs, arena, x, nrhs, sb, SB, stream);
   if (chol)
-    return fwd ? launch_trsv<T, true, true>(fronts, flist, n_fronts, max_fs, arena, x, nrhs, sb, SB, stream)
-               : launch_trsv<T, true, false>(fronts, flist, n_fronts, max_fs, arena, x, nrhs, sb, SB, stream);
-  return fwd ? launch_trsv<T, false, true>(fronts, flist, n_fronts, max_fs, arena, x, nrhs, sb, SB, stream)
-             : launch_trsv<T, false, false>(fronts, flist, n_fronts, max_fs, arena, x, nrhs, sb, SB, stream);
+    return fwd ? launch_trsv<T, true, true, false>(fronts, flist, n_fronts, max_fs, arena, x, nrhs, sb, SB, stream)
+               : launch_trsv<T, true, false, false>(fronts, flist, n_fronts, max_fs, arena, x, nrhs, sb, SB, stream);
+  return fwd ? launch_trsv<T, false, true, false>(fronts, flist, n_fronts, max_fs, arena, x, nrhs, sb, SB, stream)
+             : launch_trsv<T, false, false, false>(fronts, flist, n_fronts, max_fs, arena, x, nrhs, sb, SB, stream);
 }
 }  // namespace
 
@@ -562,20 +754,24 @@ int uhmk_solve_gather(const uhmk_front* fronts, const uhmk_item* items, int n_it
 int uhmk_solve_trsv(const uhmk_front* fronts, const int* flist, int n_fronts, int max_fs, int chol, int fwd,
                     int sb, int sb_size, const void* arena, int prec, double* x, int nrhs, hipStream_t stream) {
   if (n_fronts <= 0 || max_fs <= 0) return 0;
+  // chol = 2: Cholesky factors whose 64 x 64 diagonal blocks carry W = L^-1
+  const int inv = chol == 2, ch = chol != 0;
   if (prec == 0)
-    return trsv_any<double>(chol, fwd, fronts, flist, n_fronts, max_fs, static_cast<const double*>(arena), x, nrhs, sb,
-                            sb_size, stream);
+    return trsv_any<double>(ch, fwd, inv, fronts, flist, n_fronts, max_fs, static_cast<const double*>(arena), x, nrhs,
+                            sb, sb_size, stream);
   if (prec == 1)
-    return trsv_any<float>(chol, fwd, fronts, flist, n_fronts, max_fs, static_cast<const float*>(arena), x, nrhs, sb,
-                           sb_size, stream);
+    return trsv_any<float>(ch, fwd, inv, fronts, flist, n_fronts, max_fs, static_cast<const float*>(arena), x, nrhs,
+                           sb, sb_size, stream);
   return -1006;
 }
 
 int uhmk_solve_gemv(const uhmk_front* fronts, const uhmk_item* items, int n_items, int cols, const void* arena,
-                    int prec, double* x, int nrhs, hipStream_t stream) {
+                    int prec, double* x, double* dst, int nrhs, hipStream_t stream) {
+  if (!dst) dst = x;
   if (n_items <= 0) return 0;
   if (prec != 0 && prec != 1) return -1006;
   if (!cols) {
+    if (dst != x) return -1009;
     if (prec == 0)
       hipLaunchKernelGGL(k_sv_gemv_rows<double>, dim3(n_items), dim3(256), 0, stream, fronts, items,
                          static_cast<const double*>(arena), x, nrhs);
@@ -585,10 +781,33 @@ int uhmk_solve_gemv(const uhmk_front* fronts, const uhmk_item* items, int n_item
   } else {
     if (prec == 0)
       hipLaunchKernelGGL(k_sv_gemv_cols<double>, dim3(n_items), dim3(256), 0, stream, fronts, items,
-                         static_cast<const double*>(arena), x, nrhs);
+                         static_cast<const double*>(arena), x, nrhs, dst);
     else
       hipLaunchKernelGGL(k_sv_gemv_cols<float>, dim3(n_items), dim3(256), 0, stream, fronts, items,
-                         static_cast<const float*>(arena), x, nrhs);
+                         static_cast<const float*>(arena), x, nrhs, dst);
+  }
+  return ret(hipGetLastError());
+}
+
+int uhmk_solve_chol_step(const uhmk_front* fronts, const uhmk_item* items, int n_items, int fwd, int k0,
+                         const void* arena, int prec, double* x, double* z, int nrhs, hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  if (prec == 0) {
+    if (fwd)
+      hipLaunchKernelGGL(k_sv_chol_fwd_step<double>, dim3(n_items), dim3(256), 0, stream, fronts, items,
+                         static_cast<const double*>(arena), x, z, nrhs, k0);
+    else
+      hipLaunchKernelGGL(k_sv_chol_bwd_step<double>, dim3(n_items), dim3(256), 0, stream, fronts, items,
+                         static_cast<const double*>(arena), x, z, nrhs, k0);
+  } else if (prec == 1) {
+    if (fwd)
+      hipLaunchKernelGGL(k_sv_chol_fwd_step<float>, dim3(n_items), dim3(256), 0, stream, fronts, items,
+                         static_cast<const float*>(arena), x, z, nrhs, k0);
+    else
+      hipLaunchKernelGGL(k_sv_chol_bwd_step<float>, dim3(n_items), dim3(256), 0, stream, fronts, items,
+                         static_cast<const float*>(arena), x, z, nrhs, k0);
+  } else {
+    return -1006;
   }
   return ret(hipGetLastError());
 }

@@ -245,6 +245,7 @@ void FrontSolver::create_buffers() {
   d_arena_.alloc(std::max(1LL, arena_elems_) * sizeof(double));
   d_arena32_.alloc(std::max(1LL, arena32_elems_) * sizeof(float));
   d_rf_.alloc(std::max(1LL, x_elems_) * sizeof(double));
+  d_z_.alloc(std::max(1LL, x_elems_) * sizeof(double));
   check_hip(hipMemsetAsync(d_rf_.p, 0, d_rf_.bytes, stream_), "memset rf");
   d_piv_.alloc(std::max(1, piv_elems_) * sizeof(int));
   d_perm_.alloc(std::max(1, piv_elems_) * sizeof(int));
@@ -645,6 +646,7 @@ int FrontSolver::factor(int method) {
   check_hip(hipMemcpy(hinfo.data(), info, hinfo.size() * sizeof(int), hipMemcpyDeviceToHost), "info");
   factored_ = true;
   last_method_ = method;
+  factor_nb_ = nb_;
   for (size_t f = 0; f < hinfo.size(); ++f)
     if (hinfo[f]) return order_[f]->id + 1;
   return 0;
@@ -656,6 +658,7 @@ void FrontSolver::build_solve_schedule(int method) {
   // 512-row super-blocks whose couplings go through the gemv kernels, so
   // that a level's big fronts are read by many workgroups.
   const bool chol = (method == UHM_CHOL);
+  const bool inv = chol && factor_nb_ == 64;
   sitems_.clear();
   sflist_.clear();
   sfwd_.clear();
@@ -697,7 +700,60 @@ void FrontSolver::build_solve_schedule(int method) {
     return st;
   };
   auto rt_of = [&](int L) { return level_prec_[L] ? 32 : 16; };  // gemv row chunk: one 128-byte line
-  for (int L = 0; L < nl; ++L) {
+  auto cstep = [&](int L, int fwd, int k0, int off) {
+    SolveStep st = step(SolveStep::CSTEP, off);
+    st.fwd = fwd;
+    st.k0 = k0;
+    st.prec = level_prec_[L];
+    return st;
+  };
+  if (inv) {
+    // Cholesky factors with diagonal-block inverses: 64-row block steps
+    for (int L = 0; L < nl; ++L) {
+      int o = (int)sitems_.size();
+      for (int f : levels_[L])
+        for (int r = 0; r < fr_[f].n; r += 256) sitems_.push_back(uhmk_item{f, 0, r, 0});
+      sfwd_.push_back(step(SolveStep::GATHER, o));
+      for (int k0 = 0; k0 < mfs[L]; k0 += 64) {
+        o = (int)sitems_.size();
+        for (int f : levels_[L]) {
+          const int fs = fr_[f].fs, n = fr_[f].n;
+          if (fs <= k0) continue;
+          sitems_.push_back(uhmk_item{f, 0, k0, 0});
+          for (int r = k0 + std::min(64, fs - k0); r < n; r += 64) sitems_.push_back(uhmk_item{f, 0, r, 0});
+        }
+        sfwd_.push_back(cstep(L, 1, k0, o));
+      }
+    }
+    for (int L = nl - 1; L >= 0; --L) {
+      int o = (int)sitems_.size();
+      for (int f : levels_[L])
+        if (fr_[f].parent >= 0)
+          for (int r = 0; r < fr_[f].n - fr_[f].fs; r += 256) sitems_.push_back(uhmk_item{f, 0, r, 0});
+      SolveStep br = step(SolveStep::BRANCH, o);
+      if (br.cnt) sbwd_.push_back(br);
+      o = (int)sitems_.size();
+      for (int f : levels_[L]) {
+        const int fs = fr_[f].fs, n = fr_[f].n;
+        if (fs == 0 || n == fs) continue;
+        for (int c0 = 0; c0 < fs; c0 += 16) sitems_.push_back(uhmk_item{f, fs, c0, n});
+      }
+      SolveStep g = step(SolveStep::GEMV, o);
+      g.cols = 1;
+      g.prec = level_prec_[L];
+      if (g.cnt) sbwd_.push_back(g);
+      for (int k0 = ((mfs[L] - 1) / 64) * 64; k0 >= 0 && mfs[L] > 0; k0 -= 64) {
+        o = (int)sitems_.size();
+        for (int f : levels_[L]) {
+          if (fr_[f].fs <= k0) continue;
+          sitems_.push_back(uhmk_item{f, 0, k0, 0});
+          for (int c0 = 0; c0 < k0; c0 += 64) sitems_.push_back(uhmk_item{f, 0, c0, 0});
+        }
+        sbwd_.push_back(cstep(L, 0, k0, o));
+      }
+    }
+  }
+  for (int L = 0; L < nl && !inv; ++L) {
     const int rt = rt_of(L);
     int o = (int)sitems_.size();
     for (int f : levels_[L])
@@ -719,7 +775,7 @@ void FrontSolver::build_solve_schedule(int method) {
       if (g.cnt) sfwd_.push_back(g);
     }
   }
-  for (int L = nl - 1; L >= 0; --L) {
+  for (int L = nl - 1; L >= 0 && !inv; --L) {
     const int rt = rt_of(L);
     int o = (int)sitems_.size();
     for (int f : levels_[L])
@@ -763,6 +819,7 @@ void FrontSolver::build_solve_schedule(int method) {
   d_sflist_.upload(sflist_, stream_);
   check_hip(hipStreamSynchronize(stream_), "solve schedule upload");
   solve_method_ = method;
+  solve_inv_ = inv ? 1 : 0;
 }
 
 void FrontSolver::run_solve_steps(const std::vector<SolveStep>& steps, bool chol, const double* rhs) {
@@ -779,11 +836,16 @@ void FrontSolver::run_solve_steps(const std::vector<SolveStep>& steps, bool chol
                                d_inv_.as<int>(), d_perm_.as<int>(), X, rhs, nrhs_, stream_);
         break;
       case SolveStep::TRSV:
-        rc = uhmk_solve_trsv(F, SF + st.off, st.cnt, st.max_fs, chol, st.fwd, st.sb, st.sb_size, A, st.prec, X, nrhs_,
-                             stream_);
+        rc = uhmk_solve_trsv(F, SF + st.off, st.cnt, st.max_fs, chol ? (factor_nb_ == 64 ? 2 : 1) : 0, st.fwd, st.sb,
+                             st.sb_size, A, st.prec, X, nrhs_, stream_);
         break;
       case SolveStep::GEMV:
-        rc = uhmk_solve_gemv(F, SI + st.off, st.cnt, st.cols, A, st.prec, X, nrhs_, stream_);
+        rc = uhmk_solve_gemv(F, SI + st.off, st.cnt, st.cols, A, st.prec, X, solve_inv_ ? d_z_.as<double>() : X,
+                             nrhs_, stream_);
+        break;
+      case SolveStep::CSTEP:
+        rc = uhmk_solve_chol_step(F, SI + st.off, st.cnt, st.fwd, st.k0, A, st.prec, X, d_z_.as<double>(), nrhs_,
+                                  stream_);
         break;
       case SolveStep::BRANCH:
         rc = uhmk_solve_branch(F, SI + st.off, st.cnt, d_emap_.as<int>(), X, nrhs_, stream_);
@@ -797,7 +859,8 @@ void FrontSolver::solve(int method) {
   if (!factored_) throw std::runtime_error("solve: not factorized");
   if (method != last_method_) throw std::runtime_error("solve: method differs from the factorization's");
   upload_rhs();
-  if (solve_method_ != method) build_solve_schedule(method);
+  if (solve_method_ != method || solve_inv_ != (method == UHM_CHOL && factor_nb_ == 64 ? 1 : 0))
+    build_solve_schedule(method);
   const bool chol = (method == UHM_CHOL);
   hipEvent_t e0, e1;
   check_hip(hipEventCreate(&e0), "event");
@@ -958,7 +1021,8 @@ int FrontSolver::refine(int method, double tol, int max_iter) {
   if (!factored_) throw std::runtime_error("refine: not factorized");
   if (method != last_method_) throw std::runtime_error("refine: method differs from the factorization's");
   refresh_csr();
-  if (solve_method_ != method) build_solve_schedule(method);
+  if (solve_method_ != method || solve_inv_ != (method == UHM_CHOL && factor_nb_ == 64 ? 1 : 0))
+    build_solve_schedule(method);
   const bool chol = (method == UHM_CHOL);
   const long long N = ndof_;
   const uhmk_front* F = dF_.as<uhmk_front>();

@@ -57,12 +57,13 @@ constexpr int N_LAUNCH_KINDS = 7;
 
 // One launch of the level solves.
 struct SolveStep {
-  enum Kind { GATHER, TRSV, GEMV, BRANCH } kind = GATHER;
+  enum Kind { GATHER, TRSV, GEMV, BRANCH, CSTEP } kind = GATHER;
   int off = 0, cnt = 0;            // items (gather / gemv / branch) or front list (trsv)
   int max_fs = 0, sb = 0, sb_size = 0;  // trsv
   int cols = 0;                    // gemv: row chunks (0) or column chunks (1)
   int prec = 0;                    // storage of the level's fronts
-  int fwd = 0;                     // trsv: forward (1) or backward (0)
+  int fwd = 0;                     // trsv / cstep: forward (1) or backward (0)
+  int k0 = 0;                      // cstep: first row of the block
 };
 
 // Per-kind totals of a profiled factorization (device events per launch).
@@ -211,6 +212,7 @@ class FrontSolver {
   DevBuf d_items_, d_flists_, d_gen_items_, d_lvl_, d_sitems_, d_sflist_, d_fsl_, d_perm_;
   DevBuf d_arena_, d_arena32_, d_piv_, d_x_, d_b_, d_stage_, d_info_;
   DevBuf d_rf_;  // refinement right-hand side (front layout)
+  DevBuf d_z_;   // second solve vector (front layout) of the block-step Cholesky solves
   // assembled matrix (CSR) for residuals
   bool csr_built_ = false, vals_dirty_ = true, bg_dirty_ = true;
   long long nnz_ = 0;
@@ -221,6 +223,8 @@ class FrontSolver {
   double t_factor_ms_ = 0, t_solve_ms_ = 0, t_refine_ms_ = 0;
   std::vector<double> refine_hist_;
   int last_method_ = 0;
+  int factor_nb_ = 64;  // panel width of the last factorization
+  int solve_inv_ = -1;  // solve schedule built for inverse-carrying Cholesky factors (1) or not (0)
   bool factored_ = false;
   bool profile_ = false;
   KernelStats stats_;
