@@ -70,16 +70,19 @@ int uhmk_assemble(const uhmk_front* fronts, const uhmk_item* items, int n_items,
 /* Panel LU with partial pivoting restricted to ATL rows [c0, fs) for the
  * fronts in flist; writes relative pivots.  Replaces FLA_LU_piv on the
  * panel inside linal::dense::lu_piv (LINAL/src/dense/lu_piv.cxx:79).
- * info[f] <- first zero-pivot row + 1. */
+ * info[f] <- first zero-pivot row + 1.  pperm (257 ints per front, indexed
+ * by front) receives the panel's net row permutation for uhmk_lu_rows:
+ * count, then up to 128 rows, then up to 128 source list indices. */
 int uhmk_lu_panel(const uhmk_front* fronts, const int* flist, int n_fronts, int max_rows, int c0, int nb,
-                  void* arena, int prec, int* piv, int* info, hipStream_t stream);
+                  void* arena, int prec, int* piv, int* info, int* pperm, hipStream_t stream);
 
-/* Row swaps outside the panel (apply_pivots, LINAL/src/dense/pivot.cxx:46-91),
- * U12 = L11^-1 A12 (trsm left lower unit) and L21 = A21 U11^-1 on the ABL
- * rows (trsm right upper, UHM/src/matrix/uhm/fla/lu_piv/decompose.cxx:136-146).
+/* Row swaps outside the panel (apply_pivots, LINAL/src/dense/pivot.cxx:46-91)
+ * from the panel permutation in pperm, U12 = L11^-1 A12 (trsm left lower
+ * unit) and L21 = A21 U11^-1 on the ABL rows (trsm right upper,
+ * UHM/src/matrix/uhm/fla/lu_piv/decompose.cxx:136-146).
  * items: {f, kind (0 left cols, 1 ABL rows, 2 right cols), start, 0}. */
 int uhmk_lu_rows(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, void* arena,
-                 int prec, const int* piv, hipStream_t stream);
+                 int prec, const int* pperm, hipStream_t stream);
 
 /* Cholesky of the diagonal panel block (FLA_Chol in linal::dense::chol,
  * LINAL/src/dense/chol.cxx:67) and its inverse W = L11^-1, whose strict

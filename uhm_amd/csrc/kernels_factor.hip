@@ -105,24 +105,31 @@ __global__ __launch_bounds__(256) void k_assemble(const uhmk_front* __restrict__
 // Swaps are applied to the whole outer panel here; the row kernel applies
 // them to every other column.
 // ------------------------------------------------------------------------
-constexpr int PNT = 512, PIB = 8, PNW = PNT / 64;
+constexpr int PIB = 8;
 
-template <int R, typename T>
+constexpr int PP_STRIDE = 1 + 4 * TILE;  // ints per front in the panel-permutation buffer
+constexpr int PP_MAXROWS = 6144;
+
+template <int R, typename T, int PNT>
 __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__ F,
                                                   const int* __restrict__ flist, int c0, int nb,
                                                   T* __restrict__ arena, int* __restrict__ piv,
-                                                  int* __restrict__ info) {
+                                                  int* __restrict__ info, int* __restrict__ pperm) {
   const int f = flist[blockIdx.x];
   const uhmk_front fr = F[f];
   const int n = fr.n, fs = fr.fs;
   const int kb = min(nb, fs - c0);
   T* __restrict__ A = arena + fr.off;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  constexpr int PNW = PNT / 64;
 
-  __shared__ double s_redv[PNW];
-  __shared__ int s_redi[PNW];
-  __shared__ int s_pi;
-  __shared__ double s_rowA[PIB], s_rowB[PIB];
+  // per-column candidate slots, double-buffered by column parity: one
+  // barrier per column publishes every wavefront's best row (value, index
+  // and its inner-block values) and the current row's values
+  __shared__ double s_cv[2][PNW];
+  __shared__ int s_ci[2][PNW];
+  __shared__ double s_crow[2][PNW][PIB];
+  __shared__ double s_cur[2][PIB];
   __shared__ int s_piv[TILE];
   __shared__ double s_L[PIB][PIB];
   __shared__ double s_U[PIB][TILE];
@@ -130,6 +137,7 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__
   __shared__ int s_pcnt;
   __shared__ double s_tmp[2 * PIB][TILE];
 
+  int first_zero = 0;  // thread 0: first zero-pivot row + 1 of this panel
   for (int ib = 0; ib < kb; ib += PIB) {
     const int ibw = min(PIB, kb - ib);
     const int rb = c0 + ib;
@@ -143,7 +151,8 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__
 #pragma unroll
     for (int j = 0; j < PIB; ++j) {
       if (j < ibw) {
-        const int cr = rb + j;
+        const int cr = rb + j, buf = j & 1;
+        // best row of this thread, then of the wavefront (first maximal |a|)
         double bv = -1.0;
         int bi = 0x7fffffff;
 #pragma unroll
@@ -167,51 +176,56 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__
           }
         }
         if (lane == 0) {
-          s_redv[wid] = bv;
-          s_redi[wid] = bi;
+          s_cv[buf][wid] = bv;
+          s_ci[buf][wid] = bi;
         }
-        __syncthreads();
-        if (tid == 0) {
-          double b2 = s_redv[0];
-          int i2 = s_redi[0];
-          for (int w = 1; w < PNW; ++w)
-            if (s_redv[w] > b2 || (s_redv[w] == b2 && s_redi[w] < i2)) {
-              b2 = s_redv[w];
-              i2 = s_redi[w];
-            }
-          // A column of NaNs has no maximum: keep the diagonal row.
-          if (i2 < cr || i2 >= fs) i2 = cr;
-          s_pi = i2;
-        }
-        __syncthreads();
-        const int pi = s_pi;
+        // the owner of the wavefront's best row publishes its values, the
+        // owner of the current row (thread j, first row) publishes its own
 #pragma unroll
         for (int q = 0; q < R; ++q) {
           const int r = rb + tid + PNT * q;
-          if (r == pi)
+          if (r == bi)
 #pragma unroll
-            for (int jj = 0; jj < PIB; ++jj) s_rowA[jj] = v[q][jj];
-          if (r == cr)
-#pragma unroll
-            for (int jj = 0; jj < PIB; ++jj) s_rowB[jj] = v[q][jj];
+            for (int jj = 0; jj < PIB; ++jj) s_crow[buf][wid][jj] = v[q][jj];
         }
+        if (tid == j)
+#pragma unroll
+          for (int jj = 0; jj < PIB; ++jj) s_cur[buf][jj] = v[0][jj];
         __syncthreads();
+        // every thread reduces the wavefront candidates itself
+        double b2 = s_cv[buf][0];
+        int i2 = s_ci[buf][0], wb = 0;
+#pragma unroll
+        for (int w = 1; w < PNW; ++w) {
+          const double cv = s_cv[buf][w];
+          const int ci = s_ci[buf][w];
+          if (cv > b2 || (cv == b2 && ci < i2)) {
+            b2 = cv;
+            i2 = ci;
+            wb = w;
+          }
+        }
+        // a column of NaNs has no maximum: keep the current row
+        const bool keep = (i2 < cr || i2 >= fs);
+        const int pi = keep ? cr : i2;
+        const double* __restrict__ prow = keep ? s_cur[buf] : s_crow[buf][wb];
+        const double* __restrict__ crow = s_cur[buf];
         if (pi != cr) {
 #pragma unroll
           for (int q = 0; q < R; ++q) {
             const int r = rb + tid + PNT * q;
             if (r == pi)
 #pragma unroll
-              for (int jj = 0; jj < PIB; ++jj) v[q][jj] = s_rowB[jj];
+              for (int jj = 0; jj < PIB; ++jj) v[q][jj] = crow[jj];
             if (r == cr)
 #pragma unroll
-              for (int jj = 0; jj < PIB; ++jj) v[q][jj] = s_rowA[jj];
+              for (int jj = 0; jj < PIB; ++jj) v[q][jj] = prow[jj];
           }
         }
-        const double pv = s_rowA[j];
+        const double pv = prow[j];
         if (tid == 0) {
           s_piv[ib + j] = pi - cr;
-          if (!(pv != 0.0) && info[f] == 0) info[f] = cr + 1;
+          if (!(pv != 0.0) && first_zero == 0) first_zero = cr + 1;
         }
 #pragma unroll
         for (int q = 0; q < R; ++q) {
@@ -221,11 +235,12 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__
             if (pv != 0.0) l = l / pv;
             v[q][j] = l;
 #pragma unroll
-            for (int kk = j + 1; kk < PIB; ++kk) v[q][kk] -= l * s_rowA[kk];
+            for (int kk = j + 1; kk < PIB; ++kk) v[q][kk] = fma(-l, prow[kk], v[q][kk]);
           }
         }
       }
     }
+    __syncthreads();
     // store the factored inner block
 #pragma unroll
     for (int q = 0; q < R; ++q) {
@@ -308,30 +323,91 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__
         }
       }
       __syncthreads();
-      // rank-ibw update of the rows below for those columns
+      // rank-ibw update of the rows below for those columns: per step every
+      // thread loads CW columns of all its rows before computing, so the
+      // loads of a step are in flight together
+      constexpr int CW = (R <= 2) ? 8 : ((R <= 4) ? 4 : 2);
+#pragma unroll 1
+      for (int t0 = 0; t0 < nr; t0 += CW) {
+        double a[R][CW];
 #pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const int r = rb + tid + PNT * q;
-        if (r >= rb + ibw && r < fs) {
-          for (int t0 = 0; t0 < nr; t0 += 8) {
-            double a[8];
+        for (int q = 0; q < R; ++q) {
+          const int r = rb + tid + PNT * q;
+          const bool rv = r >= rb + ibw && r < fs;
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-              a[u] = uhmd::ldz(A, r + (long long)(rb + ibw + t0 + u) * n, t0 + u < nr);
+          for (int u = 0; u < CW; ++u)
+            a[q][u] = uhmd::ldz(A, r + (long long)(rb + ibw + t0 + u) * n, rv && t0 + u < nr);
+        }
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
+        for (int kk = 0; kk < PIB; ++kk) {
+          double uu[CW];
 #pragma unroll
-              for (int kk = 0; kk < PIB; ++kk) a[u] -= v[q][kk] * s_U[kk][(t0 + u) & (TILE - 1)];
+          for (int u = 0; u < CW; ++u) uu[u] = s_U[kk][(t0 + u) & (TILE - 1)];
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-              if (t0 + u < nr) A[r + (long long)(rb + ibw + t0 + u) * n] = (T)a[u];
-          }
+          for (int q = 0; q < R; ++q)
+#pragma unroll
+            for (int u = 0; u < CW; ++u) a[q][u] = fma(-v[q][kk], uu[u], a[q][u]);
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const int r = rb + tid + PNT * q;
+          if (r >= rb + ibw && r < fs)
+#pragma unroll
+            for (int u = 0; u < CW; ++u)
+              if (t0 + u < nr) A[r + (long long)(rb + ibw + t0 + u) * n] = (T)a[q][u];
         }
       }
       __syncthreads();
     }
   }
   for (int k = tid; k < kb; k += PNT) piv[fr.poff + c0 + k] = s_piv[k];
+  if (tid == 0 && first_zero && info[f] == 0) info[f] = first_zero;
+  // Net row permutation of the whole panel for the row kernel: rows whose
+  // content changed (the kb strip rows in order, then the distinct swap
+  // targets below the strip in order of first use) and, per listed row,
+  // the list index of the row whose old value lands there.
+  __shared__ int s_cont[PP_MAXROWS];
+  __shared__ int s_tgt[TILE];
+  __shared__ int s_ncnt;
+  const int m = fs - c0;
+  for (int i = tid; i < m; i += PNT) s_cont[i] = i;
+  __syncthreads();
+  if (tid == 0)
+    for (int k = 0; k < kb; ++k) {
+      const int d = s_piv[k];
+      if (d) {
+        const int t = s_cont[k];
+        s_cont[k] = s_cont[k + d];
+        s_cont[k + d] = t;
+      }
+    }
+  if (tid < 64) {
+    const int k = tid;
+    int t = -1;
+    bool first = false;
+    if (k < kb) {
+      t = k + s_piv[k];
+      first = t >= kb;
+      for (int k2 = 0; k2 < k && first; ++k2)
+        if (k2 + s_piv[k2] == t) first = false;
+    }
+    const unsigned long long mask = __ballot(first);
+    if (first) s_tgt[__popcll(mask & ((1ull << k) - 1ull))] = t;
+    if (k == 0) s_ncnt = kb + __popcll(mask);
+  }
+  __syncthreads();
+  const int cnt = s_ncnt;
+  int* __restrict__ pp = pperm + (long long)f * PP_STRIDE;
+  for (int k = tid; k < cnt; k += PNT) {
+    const int lr = (k < kb) ? k : s_tgt[k - kb];
+    const int src = s_cont[lr];
+    int si = (src < kb) ? src : -1;
+    for (int u = 0; u < cnt - kb && si < 0; ++u)
+      if (s_tgt[u] == src) si = kb + u;
+    pp[1 + k] = c0 + lr;
+    pp[1 + 2 * TILE + k] = si;
+  }
+  if (tid == 0) pp[0] = cnt;
 }
 
 // ------------------------------------------------------------------------
@@ -342,47 +418,6 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__
 // b = first column / row of the 64-wide chunk.
 // ------------------------------------------------------------------------
 constexpr int LDP = TILE + 1;
-
-__device__ inline void panel_perm(const int* s_p, int kb, int c0, int fs, int* s_prow, int* s_psrc,
-                                  int* s_cnt) {
-  // The kb strip rows first (in order), then the distinct swap targets
-  // below the strip.  content[k] = original row now at s_prow[k].
-  int cnt = kb;
-  int content[2 * TILE];
-  for (int k = 0; k < kb; ++k) {
-    s_prow[k] = c0 + k;
-    content[k] = c0 + k;
-  }
-  for (int s = 0; s < kb; ++s) {
-    const int ra = c0 + s, rt = c0 + s + s_p[s];
-    int it = -1;
-    if (rt < c0 + kb) it = rt - c0;
-    else
-      for (int k = kb; k < cnt; ++k)
-        if (s_prow[k] == rt) it = k;
-    if (it < 0) {
-      s_prow[cnt] = rt;
-      content[cnt] = rt;
-      it = cnt++;
-    }
-    const int ia = ra - c0;
-    const int t = content[ia];
-    content[ia] = content[it];
-    content[it] = t;
-  }
-  for (int k = 0; k < cnt; ++k) {
-    const int r = content[k];
-    int src;
-    if (r < c0 + kb) src = r - c0;
-    else {
-      src = k;
-      for (int k2 = kb; k2 < cnt; ++k2)
-        if (s_prow[k2] == r) src = k2;
-    }
-    s_psrc[k] = src;
-  }
-  *s_cnt = cnt;
-}
 
 // ------------------------------------------------------------------------
 // After a panel: row swaps for every column outside the panel
@@ -396,7 +431,7 @@ __device__ inline void panel_perm(const int* s_p, int kb, int c0, int fs, int* s
 template <typename T>
 __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ F,
                                                  const uhmk_item* __restrict__ items, int c0, int nb,
-                                                 T* __restrict__ arena, const int* __restrict__ piv) {
+                                                 T* __restrict__ arena, const int* __restrict__ pperm) {
   const uhmk_item it = items[blockIdx.x];
   const uhmk_front fr = F[it.f];
   const int n = fr.n, fs = fr.fs;
@@ -406,7 +441,7 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
   __shared__ double s_X[2 * TILE][TILE];  // rows touched by the swaps, per column
   __shared__ double s_T[TILE * LDP];      // triangular panel block
   __shared__ double s_rd[TILE];
-  __shared__ int s_p[TILE], s_prow[2 * TILE], s_psrc[2 * TILE];
+  __shared__ int s_prow[2 * TILE], s_psrc[2 * TILE];
   __shared__ int s_cnt;
 
   if (it.a == 1) {
@@ -432,9 +467,14 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
 
   const int j0 = it.b;
   const int width = min(TILE, (it.a == 0 ? c0 : n) - j0);
-  if (tid < kb) s_p[tid] = piv[fr.poff + c0 + tid];
-  __syncthreads();
-  if (tid == 0) panel_perm(s_p, kb, c0, fs, s_prow, s_psrc, &s_cnt);
+  {
+    const int* __restrict__ pp = pperm + (long long)it.f * PP_STRIDE;
+    if (tid == 0) s_cnt = pp[0];
+    if (tid < 2 * TILE) {
+      s_prow[tid] = pp[1 + tid];
+      s_psrc[tid] = pp[1 + 2 * TILE + tid];
+    }
+  }
   __syncthreads();
   const int pc = s_cnt;
   for (int idx = tid; idx < width * pc; idx += 256) {
@@ -477,15 +517,22 @@ inline int ret(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
 
 template <typename T>
 int launch_panel(const uhmk_front* fronts, const int* flist, int n_fronts, int max_rows, int c0, int nb, T* arena,
-                 int* piv, int* info, hipStream_t stream) {
-  const int R = (max_rows + PNT - 1) / PNT;
-  dim3 g(n_fronts), b(PNT);
-  if (R <= 1) hipLaunchKernelGGL((k_lu_panel<1, T>), g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
-  else if (R <= 2) hipLaunchKernelGGL((k_lu_panel<2, T>), g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
-  else if (R <= 4) hipLaunchKernelGGL((k_lu_panel<4, T>), g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
-  else if (R <= 8) hipLaunchKernelGGL((k_lu_panel<8, T>), g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
-  else if (R <= 12) hipLaunchKernelGGL((k_lu_panel<12, T>), g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info);
-  else return -1001;  // more than 6144 pivot-candidate rows
+                 int* piv, int* info, int* pperm, hipStream_t stream) {
+  dim3 g(n_fronts), b(512);
+  if (max_rows <= 512)
+    hipLaunchKernelGGL((k_lu_panel<1, T, 512>), g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info, pperm);
+  else if (max_rows <= 1024)
+    hipLaunchKernelGGL((k_lu_panel<2, T, 512>), g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info, pperm);
+  else if (max_rows <= 2048)
+    hipLaunchKernelGGL((k_lu_panel<4, T, 512>), g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info, pperm);
+  else if (max_rows <= 3072)
+    hipLaunchKernelGGL((k_lu_panel<6, T, 512>), g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info, pperm);
+  else if (max_rows <= 4096)
+    hipLaunchKernelGGL((k_lu_panel<8, T, 512>), g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info, pperm);
+  else if (max_rows <= 6144)
+    hipLaunchKernelGGL((k_lu_panel<12, T, 512>), g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info, pperm);
+  else
+    return -1001;  // more than 6144 pivot-candidate rows
   return ret(hipGetLastError());
 }
 
@@ -509,28 +556,28 @@ int uhmk_assemble(const uhmk_front* fronts, const uhmk_item* items, int n_items,
 }
 
 int uhmk_lu_panel(const uhmk_front* fronts, const int* flist, int n_fronts, int max_rows, int c0, int nb,
-                  void* arena, int prec, int* piv, int* info, hipStream_t stream) {
+                  void* arena, int prec, int* piv, int* info, int* pperm, hipStream_t stream) {
   if (n_fronts <= 0) return 0;
   if (nb > TILE || nb < 1) return -1000;
   if (prec == 0)
     return launch_panel<double>(fronts, flist, n_fronts, max_rows, c0, nb, static_cast<double*>(arena), piv, info,
-                                stream);
+                                pperm, stream);
   if (prec == 1)
     return launch_panel<float>(fronts, flist, n_fronts, max_rows, c0, nb, static_cast<float*>(arena), piv, info,
-                               stream);
+                               pperm, stream);
   return -1006;
 }
 
 int uhmk_lu_rows(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, void* arena,
-                 int prec, const int* piv, hipStream_t stream) {
+                 int prec, const int* pperm, hipStream_t stream) {
   if (n_items <= 0) return 0;
   if (nb > TILE || nb < 1) return -1000;
   if (prec == 0)
     hipLaunchKernelGGL(k_lu_rows<double>, dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb,
-                       static_cast<double*>(arena), piv);
+                       static_cast<double*>(arena), pperm);
   else if (prec == 1)
     hipLaunchKernelGGL(k_lu_rows<float>, dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb,
-                       static_cast<float*>(arena), piv);
+                       static_cast<float*>(arena), pperm);
   else
     return -1006;
   return ret(hipGetLastError());

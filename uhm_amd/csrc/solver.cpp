@@ -249,6 +249,7 @@ void FrontSolver::create_buffers() {
   check_hip(hipMemsetAsync(d_rf_.p, 0, d_rf_.bytes, stream_), "memset rf");
   d_piv_.alloc(std::max(1, piv_elems_) * sizeof(int));
   d_perm_.alloc(std::max(1, piv_elems_) * sizeof(int));
+  d_pperm_.alloc(std::max<size_t>(1, fr_.size()) * 257 * sizeof(int));
   {
     std::vector<int> fsl;
     for (int f = 0; f < (int)fr_.size(); ++f)
@@ -600,10 +601,11 @@ int FrontSolver::factor(int method) {
                            d_arena_.as<double>(), d_arena32_.as<float>(), l.prec, d_stage_.as<double>(), stream_);
         break;
       case Launch::LU_PANEL:
-        rc = uhmk_lu_panel(F, FL + l.off, l.count, l.extra, l.c0, l.nb, A, l.prec, P, info, stream_);
+        rc = uhmk_lu_panel(F, FL + l.off, l.count, l.extra, l.c0, l.nb, A, l.prec, P, info, d_pperm_.as<int>(),
+                           stream_);
         break;
       case Launch::LU_ROWS:
-        rc = uhmk_lu_rows(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, P, stream_);
+        rc = uhmk_lu_rows(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, d_pperm_.as<int>(), stream_);
         break;
       case Launch::CHOL_DIAG:
         rc = uhmk_chol_diag(F, FL + l.off, l.count, l.c0, l.nb, A, l.prec, info, stream_);

@@ -209,7 +209,7 @@ class FrontSolver {
 
   // device buffers
   DevBuf dF_, d_emap_, d_inv_, d_srcbeg_, d_srcs_, d_lmap_, d_goff_;
-  DevBuf d_items_, d_flists_, d_gen_items_, d_lvl_, d_sitems_, d_sflist_, d_fsl_, d_perm_;
+  DevBuf d_items_, d_flists_, d_gen_items_, d_lvl_, d_sitems_, d_sflist_, d_fsl_, d_perm_, d_pperm_;
   DevBuf d_arena_, d_arena32_, d_piv_, d_x_, d_b_, d_stage_, d_info_;
   DevBuf d_rf_;  // refinement right-hand side (front layout)
   DevBuf d_z_;   // second solve vector (front layout) of the block-step Cholesky solves
