@@ -169,6 +169,10 @@ void uhm_improve_lu_piv_(uhm_fort_p* mesh, uhm_fort_double* tol, uhm_fort_int* m
 /* Backward error before each correction of the last refinement (n in:
  * capacity of eta, out: number of entries) and its wall time in ms. */
 void uhm_get_refine_history_(uhm_fort_p* mesh, uhm_fort_double* eta, uhm_fort_int* n, uhm_fort_double* ms);
+/* Graph mode (on = 1): the factorization's and each solve's launch
+ * sequence is captured once as a HIP graph and replayed afterwards, for
+ * re-factorization of new element values on a fixed sparsity pattern. */
+void uhm_set_graph_(uhm_fort_p* mesh, uhm_fort_int* on);
 /* Last error message (empty when none); returns its length. */
 int uhm_get_last_error_(char* buf, int len);
 

@@ -170,6 +170,10 @@ int uhmk_solve_chol_step(const uhmk_front* fronts, const uhmk_item* items, int n
 int uhmk_solve_branch(const uhmk_front* fronts, const uhmk_item* items, int n_items, const int* emap, double* x,
                       int nrhs, hipStream_t stream);
 
+/* p[0..n) = value, as a kernel (usable inside a captured graph, where
+ * memset nodes are avoided). */
+int uhmk_fill_int(int* p, long long n, int value, hipStream_t stream);
+
 /* ---- inputs and residuals -------------------------------------------------- */
 
 /* Synthetic element matrices K_e = s_e (M^T M + alpha I + beta S) and

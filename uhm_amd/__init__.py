@@ -84,7 +84,7 @@ _SIGS = {
     "uhm_get_csr_size_": [_P, _LL], "uhm_fetch_csr_": [_P, _LL, _I, _D],
     "uhm_export_connectivity_": [_P, _I, _I, _I, _I, _I, _I, _I, _I, _I],
     "uhm_set_profile_": [_P, _I], "uhm_get_kernel_stats_": [_P, _D, _D, _D, _I],
-    "uhm_set_precision_": [_P, _I, _I],
+    "uhm_set_precision_": [_P, _I, _I], "uhm_set_graph_": [_P, _I],
     "uhm_improve_chol_": [_P, _D, _I, _I], "uhm_improve_lu_piv_": [_P, _D, _I, _I],
     "uhm_get_refine_history_": [_P, _D, _I, _D],
 }
@@ -317,6 +317,10 @@ class Mesh:
         n, ms = C.c_int(len(eta)), C.c_double(0)
         _call("uhm_get_refine_history_", C.byref(self.handle), _dptr(eta), C.byref(n), C.byref(ms))
         return eta[:min(n.value, len(eta))].copy(), ms.value
+
+    def set_graph(self, on=True):
+        """Capture the factorization / solves as HIP graphs and replay them."""
+        _call("uhm_set_graph_", C.byref(self.handle), _i(1 if on else 0))
 
     def set_profile(self, on=True):
         _call("uhm_set_profile_", C.byref(self.handle), _i(1 if on else 0))

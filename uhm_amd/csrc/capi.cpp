@@ -262,6 +262,9 @@ void uhm_get_refine_history_(uhm_fort_p* mesh, uhm_fort_double* eta, uhm_fort_in
     *ms = S(M(mesh))->last_refine_ms();
   });
 }
+void uhm_set_graph_(uhm_fort_p* mesh, uhm_fort_int* on) {
+  guard("uhm_set_graph", [&] { S(M(mesh))->set_graph(*on != 0); });
+}
 void uhm_set_profile_(uhm_fort_p* mesh, uhm_fort_int* on) {
   guard("uhm_set_profile", [&] { S(M(mesh))->set_profile(*on != 0); });
 }

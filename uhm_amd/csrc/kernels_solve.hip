@@ -699,6 +699,11 @@ __global__ __launch_bounds__(256) void k_gather_rhs(long long nrow, const long l
   bg[g] = acc;
 }
 
+__global__ __launch_bounds__(256) void k_fill_int(int* __restrict__ p, long long n, int v) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 inline int ret(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
 
 template <typename T, bool CHOL, bool FWD, bool INV>
@@ -729,6 +734,12 @@ int trsv_any(int chol, int fwd, int inv, const uhmk_front* fronts, const int* fl
 }  // namespace
 
 extern "C" {
+
+int uhmk_fill_int(int* p, long long n, int value, hipStream_t stream) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, p, n, value);
+  return ret(hipGetLastError());
+}
 
 int uhmk_lu_perm(const uhmk_front* fronts, const int* flist, int n_fronts, const int* piv, int* perm,
                  hipStream_t stream) {

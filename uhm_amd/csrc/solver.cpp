@@ -46,6 +46,7 @@ void DevBuf::upload(const std::vector<T>& v, hipStream_t s) {
 FrontSolver::FrontSolver(Mesh_* m, int n_rhs) : mesh_(m), nrhs_(n_rhs) { build_symbolic(); }
 
 FrontSolver::~FrontSolver() {
+  drop_graphs();
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -89,6 +90,7 @@ void FrontSolver::set_precision(int mode, int top_levels) {
   if (mode == prec_mode_ && top_levels == top_fp64_) return;
   prec_mode_ = mode;
   top_fp64_ = top_levels;
+  drop_graphs();
   layout_arenas();
   schedule_method_ = 0;
   solve_method_ = 0;
@@ -565,35 +567,25 @@ void FrontSolver::build_schedule(int method) {
   d_items_.upload(items_, stream_);
   d_flists_.upload(flists_, stream_);
   check_hip(hipStreamSynchronize(stream_), "schedule upload");
+  if (factor_graph_) (void)hipGraphExecDestroy(factor_graph_);
+  factor_graph_ = nullptr;
   schedule_method_ = method;
 }
 
-int FrontSolver::factor(int method) {
-  if (method != UHM_LU_PIV && method != UHM_CHOL)
-    throw std::runtime_error("factor: only UHM_CHOL and UHM_LU_PIV are supported");
-  if (!created_) create_buffers();
-  upload_stage();
-  if (schedule_method_ != method) build_schedule(method);
+void FrontSolver::enqueue_factor(int method, std::vector<hipEvent_t>* pev) {
+  // The level schedule as a fixed launch sequence on stream_ (also the body
+  // that a graph capture records: no memset nodes, the info flags are
+  // cleared by a fill kernel).
   const uhmk_front* F = dF_.as<uhmk_front>();
   const uhmk_item* IT = d_items_.as<uhmk_item>();
   const int* FL = d_flists_.as<int>();
   int* P = d_piv_.as<int>();
   int* info = d_info_.as<int>();
-  check_hip(hipMemsetAsync(info, 0, d_info_.bytes, stream_), "memset info");
-  hipEvent_t e0, e1;
-  check_hip(hipEventCreate(&e0), "event");
-  check_hip(hipEventCreate(&e1), "event");
-  check_hip(hipEventRecord(e0, stream_), "event");
-  std::vector<hipEvent_t> pev;
-  if (profile_) {
-    stats_ = KernelStats();
-    pev.resize(2 * launches_.size());
-    for (auto& e : pev) check_hip(hipEventCreate(&e), "event");
-  }
+  check_k(uhmk_fill_int(info, (long long)(d_info_.bytes / sizeof(int)), 0, stream_), "clear info");
   for (size_t li = 0; li < launches_.size(); ++li) {
     const Launch& l = launches_[li];
     int rc = 0;
-    if (profile_) check_hip(hipEventRecord(pev[2 * li], stream_), "event");
+    if (pev) check_hip(hipEventRecord((*pev)[2 * li], stream_), "event");
     void* A = arena_ptr(l.prec);
     switch (l.kind) {
       case Launch::ASSEMBLE:
@@ -621,10 +613,73 @@ int FrontSolver::factor(int method) {
         break;
     }
     check_k(rc, "factor");
-    if (profile_) check_hip(hipEventRecord(pev[2 * li + 1], stream_), "event");
+    if (pev) check_hip(hipEventRecord((*pev)[2 * li + 1], stream_), "event");
   }
   if (method == UHM_LU_PIV)
     check_k(uhmk_lu_perm(F, d_fsl_.as<int>(), n_fsl_, P, d_perm_.as<int>(), stream_), "lu perm");
+}
+
+void FrontSolver::drop_graphs() {
+  if (factor_graph_) (void)hipGraphExecDestroy(factor_graph_);
+  factor_graph_ = nullptr;
+  for (auto& g : solve_graphs_)
+    if (g.second) (void)hipGraphExecDestroy(g.second);
+  solve_graphs_.clear();
+}
+
+hipGraphExec_t FrontSolver::capture(const std::function<void()>& body) {
+  // Record body's launches on stream_ into a graph and instantiate it.
+  hipGraph_t g = nullptr;
+  check_hip(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture");
+  try {
+    body();
+  } catch (...) {
+    (void)hipStreamEndCapture(stream_, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  check_hip(hipStreamEndCapture(stream_, &g), "end capture");
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  check_hip(e, "graph instantiate");
+  return ex;
+}
+
+void FrontSolver::set_graph(bool on) {
+  if (!on) drop_graphs();
+  use_graph_ = on;
+}
+
+int FrontSolver::factor(int method) {
+  if (method != UHM_LU_PIV && method != UHM_CHOL)
+    throw std::runtime_error("factor: only UHM_CHOL and UHM_LU_PIV are supported");
+  if (!created_) create_buffers();
+  upload_stage();
+  if (schedule_method_ != method) build_schedule(method);
+  hipEvent_t e0, e1;
+  check_hip(hipEventCreate(&e0), "event");
+  check_hip(hipEventCreate(&e1), "event");
+  check_hip(hipEventRecord(e0, stream_), "event");
+  std::vector<hipEvent_t> pev;
+  if (profile_) {
+    stats_ = KernelStats();
+    pev.resize(2 * launches_.size());
+    for (auto& e : pev) check_hip(hipEventCreate(&e), "event");
+    enqueue_factor(method, &pev);
+  } else if (use_graph_) {
+    // re-factorization on a fixed pattern: the launch sequence is captured
+    // once and replayed
+    if (!factor_graph_ || factor_graph_method_ != method) {
+      if (factor_graph_) (void)hipGraphExecDestroy(factor_graph_);
+      factor_graph_ = nullptr;
+      factor_graph_ = capture([&] { enqueue_factor(method, nullptr); });
+      factor_graph_method_ = method;
+    }
+    check_hip(hipGraphLaunch(factor_graph_, stream_), "graph launch");
+  } else {
+    enqueue_factor(method, nullptr);
+  }
   check_hip(hipEventRecord(e1, stream_), "event");
   check_hip(hipEventSynchronize(e1), "factor");
   float ms = 0;
@@ -645,7 +700,7 @@ int FrontSolver::factor(int method) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   std::vector<int> hinfo(fr_.size());
-  check_hip(hipMemcpy(hinfo.data(), info, hinfo.size() * sizeof(int), hipMemcpyDeviceToHost), "info");
+  check_hip(hipMemcpy(hinfo.data(), d_info_.p, hinfo.size() * sizeof(int), hipMemcpyDeviceToHost), "info");
   factored_ = true;
   last_method_ = method;
   factor_nb_ = nb_;
@@ -820,6 +875,9 @@ void FrontSolver::build_solve_schedule(int method) {
   d_sitems_.upload(sitems_, stream_);
   d_sflist_.upload(sflist_, stream_);
   check_hip(hipStreamSynchronize(stream_), "solve schedule upload");
+  for (auto& g : solve_graphs_)
+    if (g.second) (void)hipGraphExecDestroy(g.second);
+  solve_graphs_.clear();
   solve_method_ = method;
   solve_inv_ = inv ? 1 : 0;
 }
@@ -857,6 +915,21 @@ void FrontSolver::run_solve_steps(const std::vector<SolveStep>& steps, bool chol
   }
 }
 
+void FrontSolver::run_solve(bool chol, const double* rhs) {
+  if (!use_graph_) {
+    run_solve_steps(sfwd_, chol, rhs);
+    run_solve_steps(sbwd_, chol, rhs);
+    return;
+  }
+  hipGraphExec_t& g = solve_graphs_[rhs];
+  if (!g)
+    g = capture([&] {
+      run_solve_steps(sfwd_, chol, rhs);
+      run_solve_steps(sbwd_, chol, rhs);
+    });
+  check_hip(hipGraphLaunch(g, stream_), "graph launch");
+}
+
 void FrontSolver::solve(int method) {
   if (!factored_) throw std::runtime_error("solve: not factorized");
   if (method != last_method_) throw std::runtime_error("solve: method differs from the factorization's");
@@ -868,8 +941,7 @@ void FrontSolver::solve(int method) {
   check_hip(hipEventCreate(&e0), "event");
   check_hip(hipEventCreate(&e1), "event");
   check_hip(hipEventRecord(e0, stream_), "event");
-  run_solve_steps(sfwd_, chol, d_b_.as<double>());
-  run_solve_steps(sbwd_, chol, d_b_.as<double>());
+  run_solve(chol, d_b_.as<double>());
   check_hip(hipEventRecord(e1, stream_), "event");
   check_hip(hipEventSynchronize(e1), "solve");
   float ms = 0;
@@ -1052,8 +1124,7 @@ int FrontSolver::refine(int method, double tol, int max_iter) {
       check_k(uhmk_front_global(F, nf, d_goff_.as<long long>(), d_rf_.as<double>(), d_r_.as<double>(), col, N, 1,
                                 stream_),
               "scatter r");
-    run_solve_steps(sfwd_, chol, d_rf_.as<double>());
-    run_solve_steps(sbwd_, chol, d_rf_.as<double>());
+    run_solve(chol, d_rf_.as<double>());
     for (int col = 0; col < nrhs_; ++col)
       check_k(uhmk_front_global(F, nf, d_goff_.as<long long>(), d_x_.as<double>(), d_xg_.as<double>(), col, N, 2,
                                 stream_),

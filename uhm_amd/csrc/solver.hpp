@@ -16,6 +16,8 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstdint>
+#include <functional>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -143,6 +145,12 @@ class FrontSolver {
     nb_ = nb;
   }
   void set_profile(bool on) { profile_ = on; }
+  // Graph mode: the factorization's launch sequence (and each solve's) is
+  // captured once per pattern / method / precision and replayed, for
+  // re-factorization of new values on a fixed sparsity pattern.  Profiled
+  // factorizations always run eagerly.
+  void set_graph(bool on);
+  bool graph_mode() const { return use_graph_; }
   const KernelStats& kernel_stats() const { return stats_; }
   int block() const { return nb_; }
   double last_factor_ms() const { return t_factor_ms_; }
@@ -159,6 +167,10 @@ class FrontSolver {
   void* arena_ptr(int prec) const { return prec ? d_arena32_.p : d_arena_.p; }
   void residual_norms(int col, double* h4);
   void run_solve_steps(const std::vector<SolveStep>& steps, bool chol, const double* rhs);
+  void run_solve(bool chol, const double* rhs);
+  void enqueue_factor(int method, std::vector<hipEvent_t>* pev);
+  hipGraphExec_t capture(const std::function<void()>& body);
+  void drop_graphs();
   void refresh_csr();
   void upload_stage();
   void upload_rhs();
@@ -227,6 +239,10 @@ class FrontSolver {
   int solve_inv_ = -1;  // solve schedule built for inverse-carrying Cholesky factors (1) or not (0)
   bool factored_ = false;
   bool profile_ = false;
+  bool use_graph_ = false;
+  hipGraphExec_t factor_graph_ = nullptr;
+  int factor_graph_method_ = 0;
+  std::map<const double*, hipGraphExec_t> solve_graphs_;
   KernelStats stats_;
 };
 
