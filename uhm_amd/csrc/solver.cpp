@@ -394,7 +394,15 @@ void FrontSolver::build_schedule(int method) {
     if (la.count) launches_.push_back(la);
     int maxfs = 0;
     for (int f : lv) maxfs = std::max(maxfs, fr_[f].fs);
+    // Cholesky with 64-wide panels: groups of G panels; each panel updates
+    // only the rest of its group (64-wide tiles that stop at the group end),
+    // and the trailing matrix receives one update with K = 64 G per group,
+    // so the trailing entries are read and written once per group instead
+    // of once per panel.
+    const int G = (chol && nb == 64) ? std::max(1, chol_group_) : 1;
     for (int c0 = 0; c0 < maxfs; c0 += nb) {
+      const int g0 = c0 - c0 % (G * nb);  // first column of this panel's group
+      const bool group_last = (c0 + nb >= g0 + G * nb) || (c0 + nb >= maxfs);
       std::vector<int> act;
       int maxrows = 0;
       for (int f : lv)
@@ -433,6 +441,69 @@ void FrontSolver::build_schedule(int method) {
       lr.count = (int)(items_.size() - lr.off);
       if (lr.count) launches_.push_back(lr);
 
+      if (G > 1) {
+        // update of the rest of the group by this panel (K = [c0, c0+nb))
+        Launch li;
+        li.c0 = c0;
+        li.nb = nb;
+        li.level = L;
+        li.kind = Launch::GEMM;
+        li.extra = 1;
+        li.tile = 64;
+        li.off = (long long)items_.size();
+        for (int f : act) {
+          const int fs = fr_[f].fs, n = fr_[f].n;
+          const int c1 = std::min(c0 + nb, fs), ge = std::min(g0 + G * nb, fs);
+          if (c1 >= ge) continue;
+          tile_starts(c1, n, fs, 64, R);
+          tile_starts(c1, ge, fs, 64, C);
+          for (int j0 : C)
+            for (int i0 : R) {
+              if (tile_end(i0, fs, n, 64) - 1 < j0) continue;
+              items_.push_back(uhmk_item{f, i0, j0, 0});
+            }
+        }
+        li.count = (int)(items_.size() - li.off);
+        if (li.count) launches_.push_back(li);
+        if (!group_last) continue;
+        // trailing update of the whole group (K = [g0, g0 + G nb))
+        Launch lt;
+        lt.c0 = g0;
+        lt.nb = G * nb;
+        lt.level = L;
+        lt.kind = Launch::GEMM;
+        lt.extra = 1;
+        {
+          double sr = 0, sc = 0;
+          int cnt = 0;
+          for (int f : lv) {
+            const int ge = std::min(g0 + G * nb, fr_[f].fs);
+            if (fr_[f].fs <= g0 || ge >= fr_[f].fs) continue;
+            sr += fr_[f].n - ge;
+            sc += fr_[f].fs - ge;
+            ++cnt;
+          }
+          lt.tile = cnt ? gemm_tile_for(sr / cnt, sc / cnt) : 64;
+        }
+        const int T = lt.tile;
+        lt.off = (long long)items_.size();
+        for (int f : lv) {
+          const int fs = fr_[f].fs, n = fr_[f].n;
+          if (fs <= g0) continue;
+          const int ge = std::min(g0 + G * nb, fs);
+          if (ge >= fs) continue;
+          tile_starts(ge, n, fs, T, R);
+          tile_starts(ge, fs, fs, T, C);
+          for (int j0 : C)
+            for (int i0 : R) {
+              if (tile_end(i0, fs, n, T) - 1 < j0) continue;
+              items_.push_back(uhmk_item{f, i0, j0, 0});
+            }
+        }
+        lt.count = (int)(items_.size() - lt.off);
+        if (lt.count) launches_.push_back(lt);
+        continue;
+      }
       Launch lg;
       lg.c0 = c0;
       lg.nb = nb;

@@ -144,6 +144,12 @@ class FrontSolver {
     if (nb != nb_) schedule_method_ = 0;
     nb_ = nb;
   }
+  // Cholesky panels per trailing update (1: update after every panel)
+  void set_chol_group(int g) {
+    if (g < 1) g = 1;
+    if (g != chol_group_) schedule_method_ = 0;
+    chol_group_ = g;
+  }
   void set_profile(bool on) { profile_ = on; }
   // Graph mode: the factorization's launch sequence (and each solve's) is
   // captured once per pattern / method / precision and replayed, for
@@ -180,6 +186,7 @@ class FrontSolver {
   Mesh_* mesh_;
   int nrhs_;
   int nb_ = 64;
+  int chol_group_ = 4;
   bool created_ = false;
   hipStream_t stream_ = nullptr;
 
