@@ -119,6 +119,11 @@ def run_path(U, m, w, mode, top, args):
     solve = m.solve_chol if chol else m.solve_lu_piv
     name = "fp64" if mode == U.PREC_FP64 else "mixed"
     m.set_precision(mode, top)
+    tuned = None
+    if args.tune:
+        # the project's autotuner picks the schedule knobs on this pattern
+        t_ms, tuned = m.autotune(w["method"])
+        log("[%s] autotune: %.2f ms with %s" % (name, t_ms, tuned))
     for i in range(args.warmup):
         info = factor()
         log("[%s] warmup %d: %.2f ms (info %d)" % (name, i, m.timing()[0], info))
@@ -160,7 +165,7 @@ def run_path(U, m, w, mode, top, args):
             log("  %-10s %5d launches %9.3f ms  %8.3f TFLOP/s  %8.1f GB/s"
                 % (k, n_k, ms_k, fl_k / ms_k / 1e9 if ms_k else 0, by_k / ms_k / 1e6 if ms_k else 0))
     return dict(ms_step=ms_step, t_sol=t_sol, eta=eta, hist=[float(e) for e in hist], n_corr=n_corr, solve_ms=s_ms,
-                refine_ms=ref_ms, stats=stats)
+                refine_ms=ref_ms, stats=stats, tuned=tuned)
 
 
 def gemm_roofline(stats, peak, kernel):
@@ -182,6 +187,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--nb", type=int, default=64, help="panel width")
     ap.add_argument("--top", type=int, default=2, help="tree levels kept in FP64 by the mixed path")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the autotuner over the schedule knobs before timing (default: built-in knobs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one process, one MI355X); "
@@ -218,7 +225,7 @@ def main():
         return {"factor_tflops": tflops(r), "factor_ms": round(r["ms_step"], 4),
                 "time_to_1e-12_ms": round(r["t_sol"], 3) if r["eta"] <= 1e-12 else None,
                 "backward_error": r["eta"], "backward_error_history": r["hist"], "corrections": r["n_corr"],
-                "solve_ms": round(r["solve_ms"], 3), "refine_ms": round(r["refine_ms"], 3),
+                "solve_ms": round(r["solve_ms"], 3), "refine_ms": round(r["refine_ms"], 3), "schedule": r["tuned"],
                 "kernels": {k: {"ms": round(v[0], 4), "gflop": round(v[1] / 1e9, 3),
                                 "gbytes": round(v[2] / 1e9, 4), "launches": v[3]}
                             for k, v in r["stats"].items() if v[3]}}

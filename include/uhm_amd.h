@@ -169,6 +169,14 @@ void uhm_improve_lu_piv_(uhm_fort_p* mesh, uhm_fort_double* tol, uhm_fort_int* m
 /* Backward error before each correction of the last refinement (n in:
  * capacity of eta, out: number of entries) and its wall time in ms. */
 void uhm_get_refine_history_(uhm_fort_p* mesh, uhm_fort_double* eta, uhm_fort_int* n, uhm_fort_double* ms);
+/* Schedule knobs: key 1 = Cholesky panels per trailing update (default 0:
+ * 8 on the FP64 path, 4 on the mixed path), key 2 = mean tile extent from
+ * which a GEMM launch uses 128 x 128 tiles (default 1024). */
+void uhm_set_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value);
+void uhm_get_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value);
+/* Autotuner: times factorizations of the current values over the knobs
+ * (coordinate search) and keeps the fastest setting; best_ms = its time. */
+void uhm_autotune_(uhm_fort_p* mesh, uhm_fort_int* method, uhm_fort_double* best_ms);
 /* Graph mode (on = 1): the factorization's and each solve's launch
  * sequence is captured once as a HIP graph and replayed afterwards, for
  * re-factorization of new element values on a fixed sparsity pattern. */

@@ -275,3 +275,35 @@ def test_zero_pivot_reports_element(lib):
         m.copy_in(E[e], spec.elems[e], U.UHM_LHS, K)
     info = m.lu_piv_with_free()
     assert info >= 1
+
+
+def test_schedule_knobs_preserve_results(lib):
+    """Every Cholesky panel grouping and tile threshold the autotuner may
+    pick factors the same matrix to rounding (different summation order
+    only), and the autotuner leaves a valid setting behind."""
+    m = U.Mesh()
+    m.build_synthetic(2, (8, 8, 8), 3, 0)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    m.create_matrix_buffer()
+    m.set_random_params(seed=4, alpha=64.0, scale_kind=1, p0=0.5)
+    m.random_matrix()
+    m.set_rhs()
+    assert m.chol_with_free() == 0
+    ref = [np.tril(m.fetch_front(f)) for f in range(m.n_fronts())]
+    for key, val in [(1, 1), (1, 2), (1, 8), (2, 64), (2, 1024)]:
+        m.set_tuning(key, val)
+        assert m.chol_with_free() == 0
+        for f in range(m.n_fronts()):
+            G = np.tril(m.fetch_front(f))
+            err = np.abs(G - ref[f]).max() / max(1e-300, np.abs(ref[f]).max())
+            assert err <= 1e-12, (key, val, f, err)
+        m.set_tuning(1, 0)
+        m.set_tuning(2, 1024)
+    best, knobs = m.autotune(U.UHM_CHOL)
+    say("  autotune", best, knobs)
+    assert best > 0 and knobs["chol_group"] in (1, 2, 4, 8)
+    assert m.chol_with_free() == 0
+    m.solve_chol()
+    m.check_chol()
+    assert m.backward_error() <= ETA_TOL

@@ -85,6 +85,7 @@ _SIGS = {
     "uhm_export_connectivity_": [_P, _I, _I, _I, _I, _I, _I, _I, _I, _I],
     "uhm_set_profile_": [_P, _I], "uhm_get_kernel_stats_": [_P, _D, _D, _D, _I],
     "uhm_set_precision_": [_P, _I, _I], "uhm_set_graph_": [_P, _I],
+    "uhm_set_tuning_": [_P, _I, _I], "uhm_get_tuning_": [_P, _I, _I], "uhm_autotune_": [_P, _I, _D],
     "uhm_improve_chol_": [_P, _D, _I, _I], "uhm_improve_lu_piv_": [_P, _D, _I, _I],
     "uhm_get_refine_history_": [_P, _D, _I, _D],
 }
@@ -317,6 +318,21 @@ class Mesh:
         n, ms = C.c_int(len(eta)), C.c_double(0)
         _call("uhm_get_refine_history_", C.byref(self.handle), _dptr(eta), C.byref(n), C.byref(ms))
         return eta[:min(n.value, len(eta))].copy(), ms.value
+
+    def set_tuning(self, key, value):
+        _call("uhm_set_tuning_", C.byref(self.handle), _i(key), _i(value))
+
+    def get_tuning(self, key):
+        v = C.c_int(0)
+        _call("uhm_get_tuning_", C.byref(self.handle), _i(key), C.byref(v))
+        return v.value
+
+    def autotune(self, method):
+        """Times factorizations over the schedule knobs, keeps the fastest;
+        returns (best ms, {knob: value})."""
+        t = C.c_double(0)
+        _call("uhm_autotune_", C.byref(self.handle), _i(method), C.byref(t))
+        return t.value, {"chol_group": self.get_tuning(1), "tile_threshold": self.get_tuning(2)}
 
     def set_graph(self, on=True):
         """Capture the factorization / solves as HIP graphs and replay them."""

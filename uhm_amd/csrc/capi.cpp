@@ -262,6 +262,15 @@ void uhm_get_refine_history_(uhm_fort_p* mesh, uhm_fort_double* eta, uhm_fort_in
     *ms = S(M(mesh))->last_refine_ms();
   });
 }
+void uhm_set_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value) {
+  guard("uhm_set_tuning", [&] { S(M(mesh))->set_tuning(*key, *value); });
+}
+void uhm_get_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value) {
+  guard("uhm_get_tuning", [&] { *value = S(M(mesh))->get_tuning(*key); });
+}
+void uhm_autotune_(uhm_fort_p* mesh, uhm_fort_int* method, uhm_fort_double* best_ms) {
+  guard("uhm_autotune", [&] { *best_ms = S(M(mesh))->autotune(*method); });
+}
 void uhm_set_graph_(uhm_fort_p* mesh, uhm_fort_int* on) {
   guard("uhm_set_graph", [&] { S(M(mesh))->set_graph(*on != 0); });
 }

@@ -362,10 +362,11 @@ static void tile_starts(int a, int b, int cut, int T, std::vector<int>& out) {
 }
 
 // GEMM tile edge of one launch from the mean extent of the updated blocks
-// (rows x columns): 128 x 128 tiles when both average at least 256, so the
-// ragged edge tiles stay a small share, 64 x 64 otherwise.
-static int gemm_tile_for(double mean_rows, double mean_cols) {
-  return (mean_rows >= 256.0 && mean_cols >= 256.0) ? 128 : 64;
+// (rows x columns): 128 x 128 tiles when both average at least the
+// threshold (256 by default, tunable), so the ragged edge tiles stay a small
+// share, 64 x 64 otherwise.
+static int gemm_tile_for(double mean_rows, double mean_cols, double threshold) {
+  return (mean_rows >= threshold && mean_cols >= threshold) ? 128 : 64;
 }
 
 void FrontSolver::build_schedule(int method) {
@@ -399,7 +400,7 @@ void FrontSolver::build_schedule(int method) {
     // and the trailing matrix receives one update with K = 64 G per group,
     // so the trailing entries are read and written once per group instead
     // of once per panel.
-    const int G = (chol && nb == 64) ? std::max(1, chol_group_) : 1;
+    const int G = (chol && nb == 64) ? std::max(1, chol_group()) : 1;
     for (int c0 = 0; c0 < maxfs; c0 += nb) {
       const int g0 = c0 - c0 % (G * nb);  // first column of this panel's group
       const bool group_last = (c0 + nb >= g0 + G * nb) || (c0 + nb >= maxfs);
@@ -483,7 +484,7 @@ void FrontSolver::build_schedule(int method) {
             sc += fr_[f].fs - ge;
             ++cnt;
           }
-          lt.tile = cnt ? gemm_tile_for(sr / cnt, sc / cnt) : 64;
+          lt.tile = cnt ? gemm_tile_for(sr / cnt, sc / cnt, tile_threshold_) : 64;
         }
         const int T = lt.tile;
         lt.off = (long long)items_.size();
@@ -517,7 +518,7 @@ void FrontSolver::build_schedule(int method) {
           sr += fr_[f].n - c1;
           sc += (chol ? fr_[f].fs : fr_[f].n) - c1;
         }
-        lg.tile = gemm_tile_for(sr / act.size(), sc / act.size());
+        lg.tile = gemm_tile_for(sr / act.size(), sc / act.size(), tile_threshold_);
       }
       const int T = lg.tile;
       lg.off = (long long)items_.size();
@@ -548,7 +549,7 @@ void FrontSolver::build_schedule(int method) {
           sss += fr_[f].n - fr_[f].fs;
           ++cnt;
         }
-      ls.tile = cnt ? gemm_tile_for(sss / cnt, sss / cnt) : 64;
+      ls.tile = cnt ? gemm_tile_for(sss / cnt, sss / cnt, tile_threshold_) : 64;
     }
     const int T = ls.tile;
     ls.off = (long long)items_.size();
@@ -720,6 +721,71 @@ hipGraphExec_t FrontSolver::capture(const std::function<void()>& body) {
 void FrontSolver::set_graph(bool on) {
   if (!on) drop_graphs();
   use_graph_ = on;
+}
+
+void FrontSolver::set_tuning(int key, int value) {
+  if (key == 1) {
+    set_chol_group(value);
+  } else if (key == 2) {
+    if (value < 64) throw std::runtime_error("set_tuning: tile threshold must be >= 64");
+    // (values above every front size select 64 x 64 tiles everywhere)
+    if (value != tile_threshold_) schedule_method_ = 0;
+    tile_threshold_ = value;
+  } else {
+    throw std::runtime_error("set_tuning: unknown key (1: Cholesky panel group, 2: 128-tile threshold)");
+  }
+}
+
+int FrontSolver::get_tuning(int key) const {
+  if (key == 1) return chol_group();
+  if (key == 2) return tile_threshold_;
+  throw std::runtime_error("get_tuning: unknown key");
+}
+
+double FrontSolver::autotune(int method) {
+  // Coordinate search over the schedule knobs on this pattern and these
+  // values: the Cholesky panel group (panels per trailing update), then the
+  // mean tile extent from which a launch uses 128 x 128 GEMM tiles.  Each
+  // candidate is timed as the better of two factorizations; the best
+  // setting stays in place and its time is returned.
+  auto time_it = [&]() {
+    double best = 1e300;
+    for (int r = 0; r < 2; ++r) {
+      factor(method);
+      best = std::min(best, t_factor_ms_);
+    }
+    return best;
+  };
+  const bool save_graph = use_graph_;
+  use_graph_ = false;
+  double best = time_it();
+  if (method == UHM_CHOL) {
+    int bg = chol_group();
+    for (int g : {1, 2, 4, 8}) {
+      if (g == bg) continue;
+      set_chol_group(g);
+      const double t = time_it();
+      if (t < best) {
+        best = t;
+        bg = g;
+      }
+    }
+    set_chol_group(bg);
+  }
+  int bt = tile_threshold_;
+  for (int th : {128, 192, 256, 384, 512, 1024}) {
+    if (th == bt) continue;
+    set_tuning(2, th);
+    const double t = time_it();
+    if (t < best) {
+      best = t;
+      bt = th;
+    }
+  }
+  set_tuning(2, bt);
+  use_graph_ = save_graph;
+  drop_graphs();
+  return best;
 }
 
 int FrontSolver::factor(int method) {

@@ -144,12 +144,20 @@ class FrontSolver {
     if (nb != nb_) schedule_method_ = 0;
     nb_ = nb;
   }
+  // Schedule knobs: key 1 = Cholesky panels per trailing update, key 2 =
+  // mean tile extent from which a GEMM launch uses 128 x 128 tiles.
+  void set_tuning(int key, int value);
+  int get_tuning(int key) const;
+  // Times factorizations of the current values over the knobs and keeps the
+  // fastest setting; returns its factorization time in ms.
+  double autotune(int method);
   // Cholesky panels per trailing update (1: update after every panel)
   void set_chol_group(int g) {
-    if (g < 1) g = 1;
+    if (g < 0) g = 0;
     if (g != chol_group_) schedule_method_ = 0;
     chol_group_ = g;
   }
+  int chol_group() const { return chol_group_ ? chol_group_ : (prec_mode_ ? 4 : 8); }
   void set_profile(bool on) { profile_ = on; }
   // Graph mode: the factorization's launch sequence (and each solve's) is
   // captured once per pattern / method / precision and replayed, for
@@ -186,7 +194,8 @@ class FrontSolver {
   Mesh_* mesh_;
   int nrhs_;
   int nb_ = 64;
-  int chol_group_ = 4;
+  int chol_group_ = 0;  // 0: by precision (FP64 8, mixed 4, the autotuner's usual picks)
+  int tile_threshold_ = 1024;
   bool created_ = false;
   hipStream_t stream_ = nullptr;
 
