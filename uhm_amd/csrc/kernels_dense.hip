@@ -28,110 +28,188 @@ constexpr int LDP = TILE + 1;
 constexpr int LDW = TILE + 1;
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-// One wavefront: C += X Y for 16 x 16 blocks X, Y in LDS (row strides ldx,
-// ldy), K = 16 as four v_mfma_f64_16x16x4f64.  Operand lanes: X(i, k) in lane
-// i + 16 k, Y(k, j) in lane j + 16 k; result register r of lane l holds
-// C(4 r + l / 16, l % 16).
-__device__ __forceinline__ d4 mma16_ld(const double* X, int ldx, const double* Y, int ldy, d4 c, int lane) {
+template <int L>
+__device__ __forceinline__ double readlane_d16(double v) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), L),
+                          __builtin_amdgcn_readlane(__double2loint(v), L));
+}
+__device__ __forceinline__ double readlane_d16_rt(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+// One wavefront: C += X op(Y) for 16 x 16 blocks X, Y in LDS (row strides
+// ldx, ldy), K = 16 as four v_mfma_f64_16x16x4f64; YT: op(Y) = Y^T.
+// Operand lanes: X(i, k) in lane i + 16 k, op(Y)(k, j) in lane j + 16 k;
+// result register r of lane l holds C(4 r + l / 16, l % 16).
+template <bool YT>
+__device__ __forceinline__ d4 mma16_op(const double* X, int ldx, const double* Y, int ldy, d4 c, int lane) {
 #pragma unroll
   for (int k0 = 0; k0 < 16; k0 += 4) {
     const int k = k0 + (lane >> 4), q = lane & 15;
-    c = __builtin_amdgcn_mfma_f64_16x16x4f64(X[q * ldx + k], Y[k * ldy + q], c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(X[q * ldx + k], YT ? Y[q * ldy + k] : Y[k * ldy + q], c, 0, 0, 0);
   }
   return c;
 }
+__device__ __forceinline__ d4 mma16_ld(const double* X, int ldx, const double* Y, int ldy, d4 c, int lane) {
+  return mma16_op<false>(X, ldx, Y, ldy, c, lane);
+}
 __device__ __forceinline__ d4 mma16(const double* X, const double* Y, int ld, d4 c, int lane) {
-  return mma16_ld(X, ld, Y, ld, c, lane);
+  return mma16_op<false>(X, ld, Y, ld, c, lane);
 }
 
+// Unblocked Cholesky of the 16 x 16 block at s (row stride ld) by lanes
+// 0..15 of one wavefront (lane i holds row i), then its inverse into w
+// (lane c forms column c).  first <- k + 1 of the first non-positive pivot.
+template <int K>
+struct Chol16Step {
+  __device__ __forceinline__ static void run(double (&a)[16], int i, int& first) {
+    const double d = readlane_d16<K>(a[K]);
+    if (!(d > 0.0) && first == 0) first = K + 1;
+    const double r = uhmd::rsqrt_d(d);
+    a[K] = (i == K) ? d * r : (i > K ? a[K] * r : 0.0);
+#pragma unroll
+    for (int j = K + 1; j < 16; ++j) a[j] = fma(-a[K], readlane_d16_rt(a[K], j), a[j]);
+    Chol16Step<K + 1>::run(a, i, first);
+  }
+};
+template <>
+struct Chol16Step<16> {
+  __device__ __forceinline__ static void run(double (&)[16], int, int&) {}
+};
+
+// Cholesky of the kb x kb diagonal panel block and its inverse, one
+// workgroup of four wavefronts per front, by 16 x 16 blocks.  Block step B:
+// lanes 0..15 of wavefront 0 factor the diagonal block (v_readlane
+// broadcast, 1 / sqrt from v_rsq_f64 and two Newton steps) and invert it;
+// the blocks below are multiplied by that inverse and the trailing blocks
+// updated, as 16 x 16 x 16 FP64 MFMA products spread over the wavefronts.
+// The blocks of W = L11^-1 below the diagonal then follow diagonal by
+// diagonal, W(I,J) = -W(I,I) sum_{K=J}^{I-1} L(I,K) W(K,J).  W's strict
+// lower part is stored transposed in the strict upper triangle of the
+// diagonal block, which the Cholesky path never reads as matrix data (only
+// the lower triangle of a front is used); the panel TRSM and the solves
+// read it from there (its diagonal is 1 / L(k,k)).  Padding beyond kb is
+// an identity block, so it never produces a non-positive pivot.
 template <typename T>
 __global__ __launch_bounds__(256) void k_chol_diag(const uhmk_front* __restrict__ F,
                                                    const int* __restrict__ flist, int c0, int nb,
                                                    T* __restrict__ arena, int* __restrict__ info) {
-  __shared__ double sc[2][64];
   __shared__ double sL[TILE * LDW];
   __shared__ double sW[TILE * LDW];
   __shared__ double sS[4][16 * 17];
-  __shared__ int s_first[4];
+  __shared__ int s_first;
   const int f = flist[blockIdx.x];
   const uhmk_front fr = F[f];
   const int n = fr.n, fs = fr.fs;
   const int kb = min(nb, fs - c0);
   T* __restrict__ A = arena + fr.off + c0 + (long long)c0 * n;
-  const int i = threadIdx.x & 63, w = threadIdx.x >> 6;
-  double a[16];
-#pragma unroll
-  for (int u = 0; u < 16; ++u) {
-    const int j = w + 4 * u;
-    a[u] = uhmd::ldz(A, i + (long long)j * n, i < kb && j < kb && j <= i);
-  }
-  int first = 0;
-  uhmd::CholColStep<0>::run(a, i, w, kb, first, sc);
-  if (i == 0) s_first[w] = first;
-#pragma unroll
-  for (int u = 0; u < 16; ++u) {
-    const int j = w + 4 * u;
-    sL[i * LDW + j] = a[u];
-    if (i < kb && j < kb && j <= i) A[i + (long long)j * n] = (T)a[u];
-  }
-  __syncthreads();
-  // W = L^-1 by 16 x 16 blocks: the four diagonal blocks are inverted in
-  // parallel (one wavefront each, lane c < 16 forms column c), then the
-  // blocks below the diagonal follow diagonal by diagonal,
-  // W(I,J) = -W(I,I) sum_{K=J}^{I-1} L(I,K) W(K,J), as 16 x 16 x 16 FP64
-  // MFMA products through LDS.
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   {
-    const int b0 = 16 * w;
-    if (i < 16) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
+      v[u] = uhmd::ldz(A, r + (long long)c * n, r < kb && c < kb && c <= r);
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
+      sL[r * LDW + c] = (r >= kb && r == c) ? 1.0 : (c <= r ? v[u] : 0.0);
+      sW[r * LDW + c] = 0.0;
+    }
+  }
+  if (tid == 0) s_first = 0;
+  __syncthreads();
+  for (int B = 0; B < 4; ++B) {
+    const int b0 = 16 * B;
+    if (w == 0 && lane < 16) {
+      double a[16];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) a[c] = (c <= lane) ? sL[(b0 + lane) * LDW + b0 + c] : 0.0;
+      int first = 0;
+      Chol16Step<0>::run(a, lane, first);
+      if (lane == 0 && first && b0 + first <= kb && s_first == 0) s_first = b0 + first;
+#pragma unroll
+      for (int c = 0; c < 16; ++c) sL[(b0 + lane) * LDW + b0 + c] = a[c];
+    }
+    __builtin_amdgcn_s_barrier();  // wavefront 0's block rows visible to its own column pass below
+    if (w == 0 && lane < 16) {
+      // inverse of the diagonal block, column lane
       double wv[16];
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
-        double acc = (k == i) ? 1.0 : 0.0;
+        double acc = (k == lane) ? 1.0 : 0.0;
 #pragma unroll
         for (int m = 0; m < k; ++m) acc = fma(-sL[(b0 + k) * LDW + b0 + m], wv[m], acc);
-        const double d = sL[(b0 + k) * LDW + b0 + k];
-        wv[k] = (b0 + k < kb) ? acc / d : 0.0;
+        wv[k] = acc / sL[(b0 + k) * LDW + b0 + k];
       }
 #pragma unroll
-      for (int k = 0; k < 16; ++k) sW[(b0 + k) * LDW + b0 + i] = wv[k];
+      for (int k = 0; k < 16; ++k) sW[(b0 + k) * LDW + b0 + lane] = wv[k];
     }
+    __syncthreads();
+    // L(I,B) = A(I,B) W(B,B)^T for the blocks below, one wavefront each
+    const int I = B + 1 + w;
+    if (I < 4) {
+      d4 c = d4{0.0, 0.0, 0.0, 0.0};
+      c = mma16_op<true>(sL + (16 * I) * LDW + b0, LDW, sW + b0 * LDW + b0, LDW, c, lane);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sS[w][(4 * r + (lane >> 4)) * 17 + (lane & 15)] = c[r];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) sL[(16 * I + 4 * r + (lane >> 4)) * LDW + b0 + (lane & 15)] = c[r];
+    }
+    __syncthreads();
+    // trailing blocks A(I,J) -= L(I,B) L(J,B)^T, B < J <= I
+    for (int t = w; t < 6; t += 4) {
+      // t -> (I, J) over the lower blocks right of column block B
+      int Ib = -1, Jb = -1, cnt = 0;
+      for (int J = B + 1; J < 4; ++J)
+        for (int I2 = J; I2 < 4; ++I2) {
+          if (cnt == t) {
+            Ib = I2;
+            Jb = J;
+          }
+          ++cnt;
+        }
+      if (Ib < 0) continue;
+      d4 c = d4{0.0, 0.0, 0.0, 0.0};
+      c = mma16_op<true>(sL + (16 * Ib) * LDW + b0, LDW, sL + (16 * Jb) * LDW + b0, LDW, c, lane);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double* e = sL + (16 * Ib + 4 * r + (lane >> 4)) * LDW + 16 * Jb + (lane & 15);
+        *e -= c[r];
+      }
+    }
+    __syncthreads();
   }
-  __syncthreads();
   for (int d = 1; d < 4; ++d) {
-    // wavefront w forms block (I, J) = (w + d, w) when it exists
+    // wavefront w forms block (I, J) = (w + d, w) of W when it exists
     const int I = w + d, J = w;
     d4 S = d4{0.0, 0.0, 0.0, 0.0};
     if (I < 4) {
-      // S = sum_K L(I,K) W(K,J)
-      for (int K = J; K < I; ++K) S = mma16(sL + (16 * I) * LDW + 16 * K, sW + (16 * K) * LDW + 16 * J, LDW, S, i);
+      for (int K = J; K < I; ++K) S = mma16(sL + (16 * I) * LDW + 16 * K, sW + (16 * K) * LDW + 16 * J, LDW, S, lane);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) sS[w][(4 * r + (i >> 4)) * 17 + (i & 15)] = S[r];
+      for (int r = 0; r < 4; ++r) sS[w][(4 * r + (lane >> 4)) * 17 + (lane & 15)] = S[r];
     }
     __syncthreads();
     if (I < 4) {
       d4 Wb = d4{0.0, 0.0, 0.0, 0.0};
-      Wb = mma16_ld(sW + (16 * I) * LDW + 16 * I, LDW, sS[w], 17, Wb, i);
+      Wb = mma16_ld(sW + (16 * I) * LDW + 16 * I, LDW, sS[w], 17, Wb, lane);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) sW[(16 * I + 4 * r + (i >> 4)) * LDW + 16 * J + (i & 15)] = -Wb[r];
+      for (int r = 0; r < 4; ++r) sW[(16 * I + 4 * r + (lane >> 4)) * LDW + 16 * J + (lane & 15)] = -Wb[r];
     }
     __syncthreads();
   }
-  double wc[16];
-#pragma unroll
-  for (int u = 0; u < 16; ++u) wc[u] = sW[i * LDW + w + 4 * u];
-  // strict lower part of W, transposed into the strict upper triangle
+  // L (lower) and the strict lower part of W, transposed, back to the front
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
-    const int j = w + 4 * u;
-    if (i < kb && j < i) A[j + (long long)i * n] = (T)wc[u];
+    const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
+    if (r < kb && c < kb) {
+      if (c <= r) A[r + (long long)c * n] = (T)sL[r * LDW + c];
+      else A[r + (long long)c * n] = (T)sW[c * LDW + r];
+    }
   }
-  __syncthreads();
-  if (threadIdx.x == 0 && info[f] == 0) {
-    int k1 = 0;
-    for (int q = 0; q < 4; ++q)
-      if (s_first[q] && (k1 == 0 || s_first[q] < k1)) k1 = s_first[q];
-    if (k1) info[f] = c0 + k1;
-  }
+  if (tid == 0 && s_first && info[f] == 0) info[f] = c0 + s_first;
 }
 
 // Cholesky panel rows [b, b+64) below the diagonal block: X <- X L11^-T =

@@ -70,28 +70,41 @@ __global__ __launch_bounds__(256) void k_assemble(const uhmk_front* __restrict__
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const int row = it.a + tx;
   if (row >= n) return;
-  int rmap[MAX_SRC];
+  // thread (tx, ty): row it.a + tx, columns it.b + ty + 4 u (u < 16); the
+  // sources are summed in order into per-column registers, each source's
+  // 16 index and 16 value loads issued together
+  double acc[TILE / 4];
 #pragma unroll
-  for (int s = 0; s < MAX_SRC; ++s)
-    if (s < ns) rmap[s] = s_kind[s] ? inv[s_inv[s] + row] : row;
-  T* A = (sizeof(T) == 8 ? (T*)arena64 : (T*)arena32) + off;
-  for (int c = ty; c < TILE; c += 4) {
-    const int col = it.b + c;
-    if (col >= n) break;
-    double acc = 0.0;
+  for (int u = 0; u < TILE / 4; ++u) acc[u] = 0.0;
+  for (int s = 0; s < ns; ++s) {
+    const int k = s_kind[s];
+    const int ri = k ? inv[s_inv[s] + row] : row;
+    if (ri < 0) continue;
+    const long long base = s_off[s] + ri;
+    const int ld = s_ld[s];
+    int cj[TILE / 4];
 #pragma unroll
-    for (int s = 0; s < MAX_SRC; ++s) {
-      if (s < ns) {
-        const int cj = s_kind[s] ? inv[s_inv[s] + col] : col;
-        const int ri = rmap[s];
-        if (ri >= 0 && cj >= 0) {
-          const long long o = s_off[s] + ri + (long long)cj * s_ld[s];
-          const int k = s_kind[s];
-          acc += k == 0 ? stage[o] : (k == 1 ? arena64[o] : (double)arena32[o]);
-        }
-      }
+    for (int u = 0; u < TILE / 4; ++u) {
+      const int col = it.b + ty + 4 * u;
+      const bool ok = col < n;
+      cj[u] = k ? (ok ? inv[s_inv[s] + col] : -1) : (ok ? col : -1);
     }
-    A[row + (long long)col * n] = (T)acc;
+    if (k == 0) {
+#pragma unroll
+      for (int u = 0; u < TILE / 4; ++u) acc[u] += uhmd::ldz(stage, base + (long long)cj[u] * ld, cj[u] >= 0);
+    } else if (k == 1) {
+#pragma unroll
+      for (int u = 0; u < TILE / 4; ++u) acc[u] += uhmd::ldz(arena64, base + (long long)cj[u] * ld, cj[u] >= 0);
+    } else {
+#pragma unroll
+      for (int u = 0; u < TILE / 4; ++u) acc[u] += uhmd::ldz(arena32, base + (long long)cj[u] * ld, cj[u] >= 0);
+    }
+  }
+  T* A = (sizeof(T) == 8 ? (T*)arena64 : (T*)arena32) + off;
+#pragma unroll
+  for (int u = 0; u < TILE / 4; ++u) {
+    const int col = it.b + ty + 4 * u;
+    if (col < n) A[row + (long long)col * n] = (T)acc[u];
   }
 }
 
