@@ -42,6 +42,16 @@ WORKLOADS = {
                  name="3D 16^3 hex Q4, SPD K_e=(M^T M+125 I)*10^U(-2,2), Cholesky, 1 RHS"),
     "lu": dict(kind=2, dims=(16, 16, 16), p=3, method=3, alpha=64.0, beta=0.1, scale_kind=1, p0=1.0, p1=0.0,
                name="3D 16^3 hex Q3, K_e=(M^T M+64 I+0.1 S)*lognormal(0,1), LU partial pivoting, 1 RHS"),
+    # batched uneven fronts: FP32 fronts, FP64 refinement, 64 RHS, graph-replayed
+    # re-factorization over value reseeds on the fixed pattern
+    "varp": dict(kind=3, dims=(256, 256), p=8, mesh_seed=12345, method=3, alpha=-1.0, beta=0.1, scale_kind=0,
+                 p0=0.0, p1=0.0, nrhs=64, top=0, reseeds=10,
+                 name="2D 256^2 quads, p_e ~ U{1..8} (min-p edges), K_e=M^T M+n I+0.1 S, LU, FP32 fronts + FP64 "
+                      "refinement, 64 RHS, graph-replayed re-factorization over 10 reseeds"),
+    # low precision: Cholesky, 256 RHS
+    "q3big": dict(kind=2, dims=(24, 24, 24), p=3, method=1, alpha=64.0, beta=0.0, scale_kind=1, p0=0.5, p1=0.0,
+                  nrhs=256, top=1,
+                  name="3D 24^3 hex Q3, SPD K_e=(M^T M+64 I)*lognormal(0,0.5), Cholesky, 256 RHS"),
 }
 
 
@@ -58,9 +68,9 @@ def device_sync():
 
 def build_mesh(U, w, seed):
     m = U.Mesh()
-    m.build_synthetic(w["kind"], w["dims"], w["p"], 0)
+    m.build_synthetic(w["kind"], w["dims"], w["p"], w.get("mesh_seed", 0))
     m.lock()
-    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    m.create_matrix_without_buffer(U.UHM_REAL, w.get("nrhs", 1))
     m.create_matrix_buffer()
     m.set_random_params(seed=seed, alpha=w["alpha"], beta=w["beta"], scale_kind=w["scale_kind"], p0=w["p0"],
                         p1=w["p1"])
@@ -168,6 +178,38 @@ def run_path(U, m, w, mode, top, args):
                 refine_ms=ref_ms, stats=stats, tuned=tuned)
 
 
+def run_reseeds(U, m, w, args):
+    """Re-factorization of new element values on the fixed pattern: the
+    factorization and solves are graph-captured once and replayed for each
+    reseed (device-generated values), each refined to backward error 1e-12."""
+    chol = w["method"] == 1
+    factor = m.chol_with_free if chol else m.lu_piv_with_free
+    solve = m.solve_chol if chol else m.solve_lu_piv
+    m.set_precision(U.PREC_MIXED, w.get("top", 0))
+    m.set_graph(True)
+    out = []
+    for seed in range(1, w["reseeds"] + 1):
+        m.set_random_params(seed=seed, alpha=w["alpha"], beta=w["beta"], scale_kind=w["scale_kind"], p0=w["p0"],
+                            p1=w["p1"])
+        m.random_matrix()
+        m.set_rhs()
+        device_sync()
+        t0 = time.perf_counter()
+        info = factor()
+        solve()
+        n = m.improve(w["method"], 1e-12, 30)
+        device_sync()
+        t = (time.perf_counter() - t0) * 1e3
+        f_ms, s_ms = m.timing()
+        hist, ref_ms = m.refine_history()
+        out.append(dict(seed=seed, info=info, ms=t, factor_ms=f_ms, solve_ms=s_ms, refine_ms=ref_ms, corrections=n,
+                        eta=float(m.backward_error())))
+        log("[reseed %d] %.2f ms (factor %.2f, solve %.2f, refine %.2f, %d corrections) eta %.2e info %d"
+            % (seed, t, f_ms, s_ms, ref_ms, n, m.backward_error(), info))
+    m.set_graph(False)
+    return out
+
+
 def gemm_roofline(stats, peak, kernel):
     g_ms = stats["gemm"][0] + stats["schur"][0]
     g_fl = stats["gemm"][1] + stats["schur"][1]
@@ -205,8 +247,10 @@ def main():
     log("mesh %s: ndof %d, fronts %d, %.1f GFLOP, setup %.1f s"
         % (args.workload, ndof, m.n_fronts(), flops / 1e9, time.perf_counter() - t0))
 
-    r64 = run_path(U, m, w, U.PREC_FP64, args.top, args)
-    rmx = run_path(U, m, w, U.PREC_MIXED, args.top, args)
+    top = w.get("top", args.top)
+    r64 = run_path(U, m, w, U.PREC_FP64, top, args)
+    rmx = run_path(U, m, w, U.PREC_MIXED, top, args)
+    reseeds = run_reseeds(U, m, w, args) if w.get("reseeds") else None
     mixed_ok = rmx["eta"] <= 1e-12
     head = rmx if mixed_ok else r64
 
@@ -250,8 +294,9 @@ def main():
             "factor_gflop": round(flops / 1e9, 3),
             "precision_path": ("mixed: FP32 fronts and FP32 MFMA Schur updates below the top %d tree levels, "
                                "FP64 top-level pivot blocks, FP64-residual refinement to backward error 1e-12"
-                               % args.top) if mixed_ok else "fp64 (mixed path did not reach 1e-12)",
+                               % top) if mixed_ok else "fp64 (mixed path did not reach 1e-12)",
             "panel_width": args.nb,
+            "n_rhs": w.get("nrhs", 1),
         },
         "time_to_1e-12_ms": {"mixed": path(rmx)["time_to_1e-12_ms"], "fp64": path(r64)["time_to_1e-12_ms"],
                              "speedup": round(r64["t_sol"] / rmx["t_sol"], 3) if mixed_ok else None},
@@ -260,6 +305,7 @@ def main():
         "roofline_fp64_path": gemm_roofline(r64["stats"], PEAK_FP64_TFLOPS, "k_gemm (FP64 MFMA)"),
         "mixed": path(rmx),
         "fp64": path(r64),
+        "reseeds": reseeds,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

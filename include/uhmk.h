@@ -164,6 +164,29 @@ int uhmk_solve_gemv(const uhmk_front* fronts, const uhmk_item* items, int n_item
 int uhmk_solve_chol_step(const uhmk_front* fronts, const uhmk_item* items, int n_items, int fwd, int k0,
                          const void* arena, int prec, double* x, double* z, int nrhs, hipStream_t stream);
 
+/* Block solves with many right-hand sides (FP64 MFMA over tiles of 64
+ * right-hand sides; items {f, first right-hand side of the tile, first
+ * row, 0}).  One 64-row block step k0 of the ATL triangular solves of a
+ * tree level (the triangular parts of solve_{lu_piv,chol}_{1,2}_hier,
+ * UHM/src/matrix/uhm/fla/lu_piv/solve.cxx:161-231,
+ * UHM/src/matrix/uhm/fla/chol/solve.cxx:152-202): item row k0 is the
+ * block itself, y = T_kk^-1 x_k goes to z (fwd = 1) or y = T_kk^-1 z_k to
+ * x (fwd = 0); any other item row r is a 64-row chunk of ATL updated with
+ * y: forward x_r -= F(r, k) y (r > k0), backward z_r -= F(r, k) y (LU) or
+ * F(k, r)^T y (Cholesky) (r < k0).  mode 0: LU (unit lower L forward, U
+ * backward), 1: Cholesky (L, L^T), 2: Cholesky whose diagonal blocks carry
+ * W = L_kk^-1 from uhmk_chol_diag. */
+int uhmk_solve_block_step(const uhmk_front* fronts, const uhmk_item* items, int n_items, int mode, int fwd, int k0,
+                          const void* arena, int prec, double* x, double* z, int nrhs, hipStream_t stream);
+
+/* Level-batched off-diagonal products of the block solves, 64 rows x 64
+ * right-hand sides per item: dir 0: x[ABL rows] -= ABL z[0:fs) (forward
+ * xb update); dir 1: z[ATL rows] -= ATR x[fs:n) (LU backward); dir 2:
+ * z[ATL rows] -= ABL^T x[fs:n) (Cholesky backward).  The gemm / gemv
+ * calls of solve_{lu_piv,chol}_{1,2}_hier on n_rhs columns. */
+int uhmk_solve_block_gemm(const uhmk_front* fronts, const uhmk_item* items, int n_items, int dir,
+                          const void* arena, int prec, double* x, double* z, int nrhs, hipStream_t stream);
+
 /* Backward branch: xb <- the parent's solution rows through emap
  * (Helper_::_branch_rhs, helper.hxx:302-331).  items {f, 0, first row, 0},
  * 256 rows each. */

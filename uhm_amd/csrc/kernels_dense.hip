@@ -242,10 +242,12 @@ __global__ __launch_bounds__(256) void k_chol_trsm(const uhmk_front* __restrict_
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
     const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
-    // block entry (r, c), r <= c: W(c, r) (r < c) or 1 / L(c, c) (r == c)
-    sWt[r * LDT + c] = (r == c && r < kb) ? 1.0 / vw[u] : vw[u];
+    // block entry (r, c), r <= c: W(c, r) (r < c) or L(c, c) (r == c, inverted below)
+    sWt[r * LDT + c] = vw[u];
     sXt[c * LDT + r] = vx[u];
   }
+  __syncthreads();
+  if (tid < kb) sWt[tid * LDT + tid] = 1.0 / sWt[tid * LDT + tid];
   __syncthreads();
   const int iw = (w & 1) * 32, jw = (w >> 1) * 32, cl = lane & 15;
   d4 acc[2][2];

@@ -1,0 +1,422 @@
+// Front solves with many right-hand sides for gfx950: the level solves of
+// kernels_solve.hip recast as FP64 MFMA products over tiles of 64
+// right-hand sides, so that every factor entry read from HBM feeds 64
+// columns of the solution.
+//
+// Reference operations restated (semantics, not code):
+//   forward   solve_lu_piv_1_hier / solve_chol_1_hier
+//             UHM/src/matrix/uhm/fla/lu_piv/solve.cxx:161-187,
+//             UHM/src/matrix/uhm/fla/chol/solve.cxx:152-168
+//   backward  solve_lu_piv_2_hier / solve_chol_2_hier
+//             lu_piv/solve.cxx:211-231, chol/solve.cxx:186-202
+// with the dense triangular solves and products of LINAL
+// (LINAL/src/dense/trsm.cxx, gemm.cxx:48-137) on n x n_rhs blocks.
+//
+// Per tree level the forward pass is
+//   gather  (kernels_solve.hip)   x_f = b_f + children's xb
+//   step k0 = 0, 64, ..           y = T_kk^-1 x_k -> z_k;  x_r -= F(r, k) y
+//                                 for the 64-row chunks r of ATL below k
+//   gemm                          xb -= ABL z_t
+// and the backward pass, root first,
+//   branch  (kernels_solve.hip)   xb <- the parent's solution rows
+//   gemm                          z_t -= ATR xb (LU) or ABL^T xb (Cholesky)
+//   step k0 = .., 64, 0           y = T_kk^-1 z_k -> x_k;  z_c -= F(c, k) y
+//                                 (LU) or F(k, c)^T y (Cholesky) for the
+//                                 64-row chunks c of ATL above k
+// The final solution is in x (front layout, n x n_rhs per front, leading
+// dimension n); z holds the intermediate ATL rows.  T_kk is the unit lower
+// L (LU forward), U (LU backward), L or L^T (Cholesky); for Cholesky factors
+// whose 64 x 64 diagonal blocks carry W = L_kk^-1 (uhmk_chol_diag) the
+// block step is the product W x_k or W^T z_k on MFMA, otherwise a
+// triangular solve of 16 vectors per wavefront (uhmd::TriStep).  Factor
+// entries are FP64 or FP32 (the level's storage); arithmetic is FP64.
+#include <hip/hip_runtime.h>
+
+#include "dense_device.hpp"
+#include "uhmk.h"
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int NC = 64;   // right-hand sides per tile
+constexpr int LD = 68;   // row stride (doubles) of the 64-wide LDS blocks
+
+// acc(c, i) += sum_{k < K} Bm[k][c] Am[k][i] for the wavefront's 32 x 32
+// quarter of a 64 x 64 tile: c blocks 2 (w & 1) + {0, 1}, i blocks
+// 2 (w >> 1) + {0, 1}.  Bm (right-hand sides) and Am (matrix, transposed)
+// live in LDS with row stride LD.  v_mfma_f64_16x16x4f64 operands: lane
+// l = m + 16 k holds Bm[k][16 cb + m] (first) and Am[k][16 ib + m]
+// (second); result register r of lane l holds acc(c = 16 cb + 4 r + l / 16,
+// i = 16 ib + l % 16).
+__device__ __forceinline__ void mma_quarter(d4 (&acc)[2][2], const double* __restrict__ Bm,
+                                            const double* __restrict__ Am, int K, int lane, int w) {
+  const int cb = 32 * (w & 1), ib = 32 * (w >> 1);
+  const int q = lane & 15, kq = lane >> 4;
+#pragma unroll 4
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const double* rb = Bm + (k0 + kq) * LD + cb + q;
+    const double* ra = Am + (k0 + kq) * LD + ib + q;
+    const double b0 = rb[0], b1 = rb[16];
+    const double a0 = ra[0], a1 = ra[16];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b0, a1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(b1, a1, acc[1][1], 0, 0, 0);
+  }
+}
+
+__device__ __forceinline__ void zero_acc(d4 (&acc)[2][2]) {
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+}
+
+// C(r + i, c0 + c) -= acc(c, i) for rows r + i < rend and columns
+// c0 + c < nrhs; C has leading dimension n.
+__device__ __forceinline__ void sub_acc(double* __restrict__ C, long long n, int r, int rend, int c0, int nrhs,
+                                        const d4 (&acc)[2][2], int lane, int w) {
+  const int cb = 32 * (w & 1), ib = 32 * (w >> 1);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int c = cb + 16 * a + 4 * g + (lane >> 4), i = ib + 16 * b + (lane & 15);
+        if (r + i < rend && c0 + c < nrhs) {
+          double* p = C + (r + i) + (long long)(c0 + c) * n;
+          *p -= acc[a][b][g];
+        }
+      }
+}
+
+// ------------------------------------------------------------------------
+// Level-batched products on the off-diagonal blocks.  item {f, c0, r0, 0}:
+// the 64 x 64 tile of rows r0.. and right-hand sides c0.. of
+//   dir 0:  x[rows of ABL] -= F[rows, 0:fs) z[0:fs)          (forward)
+//   dir 1:  z[rows of ATL] -= F[rows, fs:n) x[fs:n)          (LU backward)
+//   dir 2:  z[rows of ATL] -= F[fs:n, rows]^T x[fs:n)        (Cholesky backward)
+// K runs in 16-deep stages through double-buffered LDS; the next stage's
+// global loads are in flight while the current one is multiplied.
+// ------------------------------------------------------------------------
+template <typename T, bool TA>
+__global__ __launch_bounds__(256, 2) void k_svb_gemm(const uhmk_front* __restrict__ F,
+                                                     const uhmk_item* __restrict__ items,
+                                                     const T* __restrict__ arena, double* __restrict__ x,
+                                                     double* __restrict__ z, int nrhs, int fwd) {
+  __shared__ double sA[2][16 * LD];
+  __shared__ double sB[2][16 * LD];
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs;
+  const int c0 = it.a, r0 = it.b;
+  const int rend = fwd ? n : fs;
+  const int kbeg = fwd ? 0 : fs, kend = fwd ? fs : n;
+  const T* __restrict__ A = arena + fr.off;
+  const double* __restrict__ S = (fwd ? z : x) + fr.xoff + (long long)c0 * n;
+  double* __restrict__ D = (fwd ? x : z) + fr.xoff;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  // staging coordinates: A NN i = tid & 63, k = tid >> 6 (+ 4 u);
+  // A TN and B: k = tid & 15, i or c = tid >> 4 (+ 16 u)
+  double pa[4], pb[4];
+  auto load = [&](int k) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (!TA) {
+        const int i = tid & 63, kk = (tid >> 6) + 4 * u;
+        pa[u] = uhmd::ldz(A, (r0 + i) + (long long)(k + kk) * n, r0 + i < rend && k + kk < kend);
+      } else {
+        const int kk = tid & 15, i = (tid >> 4) + 16 * u;
+        pa[u] = uhmd::ldz(A, (k + kk) + (long long)(r0 + i) * n, r0 + i < rend && k + kk < kend);
+      }
+      const int kk = tid & 15, c = (tid >> 4) + 16 * u;
+      pb[u] = uhmd::ldz(S, (k + kk) + (long long)c * n, k + kk < kend && c0 + c < nrhs);
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (!TA) {
+        const int i = tid & 63, kk = (tid >> 6) + 4 * u;
+        sA[buf][kk * LD + i] = pa[u];
+      } else {
+        const int kk = tid & 15, i = (tid >> 4) + 16 * u;
+        sA[buf][kk * LD + i] = pa[u];
+      }
+      const int kk = tid & 15, c = (tid >> 4) + 16 * u;
+      sB[buf][kk * LD + c] = pb[u];
+    }
+  };
+  d4 acc[2][2];
+  zero_acc(acc);
+  load(kbeg);
+  store(0);
+  __syncthreads();
+  int buf = 0;
+  for (int k = kbeg; k < kend; k += 16) {
+    const bool more = k + 16 < kend;
+    if (more) load(k + 16);
+    mma_quarter(acc, sB[buf], sA[buf], 16, lane, w);
+    if (more) store(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+  sub_acc(D, n, r0, rend, c0, nrhs, acc, lane, w);
+}
+
+// ------------------------------------------------------------------------
+// One 64-row block step k0 of the triangular solves on ATL for one tree
+// level.  item {f, c0, r, 0}: r == k0 is the block itself (y goes to z
+// forward, to x backward); any other r is a 64-row chunk of ATL to update,
+// below the block forward (r >= k0 + 64, rows < fs), above it backward
+// (r + 64 <= k0).  MODE 0: LU, 1: Cholesky, 2: Cholesky whose diagonal
+// blocks carry W = L_kk^-1 in their strict upper triangle (transposed).
+// Every workgroup forms y itself, so the step needs no second launch.
+// ------------------------------------------------------------------------
+
+// Triangle of the block step into LDS for uhmd::TriStep (s[c * LD + k] =
+// T(c, k) for k < c, zero elsewhere; rdiag[k] = 1 / T(k, k)).  Backward
+// solves run in reversed index order (k' = 63 - k), so that TriStep's
+// ascending steps walk the block from its last row up; the padding rows
+// beyond kb then come first and carry rdiag 1 and zero entries.
+template <typename T, int MODE, bool FWD>
+__device__ __forceinline__ void stage_tri(double* __restrict__ s, double* __restrict__ rdiag,
+                                          const T* __restrict__ B, long long n, int kb, int tid) {
+  double v[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int idx = tid + 256 * u, a = idx & 63, b = idx >> 6;  // block entry (a, b), a contiguous
+    bool use;
+    if (FWD) use = a > b || (MODE == 1 && a == b);               // lower (L)
+    else if (MODE == 0) use = a <= b;                             // upper (U)
+    else use = a >= b;                                            // lower (L, read as L^T)
+    v[u] = uhmd::ldz(B, a + (long long)b * n, use && a < kb && b < kb);
+  }
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int idx = tid + 256 * u, a = idx & 63, b = idx >> 6;
+    // (c, k) position of entry (a, b) in the ascending solve
+    int c, k;
+    if (FWD) {
+      c = a;
+      k = b;                       // T(c, k) = L(c, k)
+    } else if (MODE == 0) {
+      c = 63 - a;
+      k = 63 - b;                  // T'(c', k') = U(63 - c', 63 - k')
+    } else {
+      c = 63 - b;
+      k = 63 - a;                  // T'(c', k') = L(63 - k', 63 - c')
+    }
+    if (c > k) s[c * LD + k] = v[u];
+    else if (c < k) s[c * LD + k] = 0.0;
+    else {
+      s[c * LD + k] = 0.0;
+      if (!(FWD && MODE == 0)) rdiag[k] = a < kb ? 1.0 / v[u] : 1.0;
+    }
+  }
+}
+
+template <typename T, int MODE, bool FWD>
+__global__ __launch_bounds__(256) void k_svb_step(const uhmk_front* __restrict__ F,
+                                                  const uhmk_item* __restrict__ items,
+                                                  const T* __restrict__ arena, double* __restrict__ x,
+                                                  double* __restrict__ z, int nrhs, int k0) {
+  __shared__ double sT[64 * LD];  // triangle, then the chunk matrix (transposed)
+  __shared__ double sX[64 * LD];  // x_k (k x c), then y
+  __shared__ double rdiag[64];
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs;
+  const int c0 = it.a, r = it.b;
+  const bool own = (r == k0);
+  const int kb = min(64, fs - k0);
+  const T* __restrict__ A = arena + fr.off;
+  const T* __restrict__ Bk = A + k0 + (long long)k0 * n;
+  const double* __restrict__ src = (FWD ? x : z) + fr.xoff + (long long)c0 * n;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  constexpr bool CHOL_T = (MODE != 0) && !FWD;  // chunk matrix read transposed: F(k0 + k, r + i)
+  // chunk matrix into registers first (independent of y): A(i, k) -> sT[k][i]
+  double pa[16];
+  if (!own) {
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int idx = tid + 256 * u;
+      if (!CHOL_T) {
+        const int i = idx & 63, k = idx >> 6;
+        pa[u] = uhmd::ldz(A, (r + i) + (long long)(k0 + k) * n, k < kb && (!FWD || r + i < fs));
+      } else {
+        const int k = idx & 63, i = idx >> 6;
+        pa[u] = uhmd::ldz(A, (k0 + k) + (long long)(r + i) * n, k < kb);
+      }
+    }
+  }
+  // x_k into LDS: sX[k][c]
+  {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int idx = tid + 256 * u, k = idx & 63, c = idx >> 6;
+      v[u] = uhmd::ldz(src, (k0 + k) + (long long)c * n, k < kb && c0 + c < nrhs);
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int idx = tid + 256 * u, k = idx & 63, c = idx >> 6;
+      sX[k * LD + c] = v[u];
+    }
+  }
+  if constexpr (MODE == 2) {
+    // sT[k][i] = Y-operator entry (i, k): forward W(i, k) (lower), backward
+    // W^T(i, k) = W(k, i) (upper); W(a, b), a > b, sits at block entry
+    // (b, a); the diagonal holds L(k, k), W's is its reciprocal.
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int idx = tid + 256 * u, a = idx & 63, b = idx >> 6;  // block entry (a, b)
+      v[u] = uhmd::ldz(Bk, a + (long long)b * n, a <= b && b < kb);
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int idx = tid + 256 * u, a = idx & 63, b = idx >> 6;
+      // entry (a, b), a < b: W(b, a); a == b: L(a, a)
+      const double wv = (a == b) ? (b < kb ? 1.0 / v[u] : 0.0) : v[u];
+      // forward: operator (i, k) = W(i, k): i = b, k = a  -> sT[a][b]
+      // backward: operator (i, k) = W(k, i): k = b, i = a -> sT[b][a]
+      // (entries with a > b are the zero triangle)
+      if (FWD) sT[a * LD + b] = (a <= b) ? wv : 0.0;
+      else sT[b * LD + a] = (a <= b) ? wv : 0.0;
+    }
+    __syncthreads();
+    d4 acc[2][2];
+    zero_acc(acc);
+    mma_quarter(acc, sX, sT, 64, lane, w);
+    __syncthreads();
+    // y(i, c) = acc(c, i) -> sX[i][c]
+    const int cb = 32 * (w & 1), ib = 32 * (w >> 1);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const int c = cb + 16 * a + 4 * g + (lane >> 4), i = ib + 16 * b + (lane & 15);
+          sX[i * LD + c] = acc[a][b][g];
+        }
+  } else {
+    stage_tri<T, MODE, FWD>(sT, rdiag, Bk, n, kb, tid);
+    __syncthreads();
+    // 16 vectors per wavefront in quad layout: lane 4 v + q holds entries
+    // [16 q, 16 q + 16) of right-hand side 16 w + v (reversed backward)
+    const int v = lane >> 2, q = lane & 3, c = 16 * w + v;
+    double xv[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int e = 16 * q + j, k = FWD ? e : 63 - e;
+      xv[j] = sX[k * LD + c];
+    }
+    if constexpr (FWD && MODE == 0) {
+      if (kb == 64) uhmd::TriStep<0, true, false, true>::run(xv, sT, LD, rdiag, kb, q);
+      else uhmd::TriStep<0, true, false, false>::run(xv, sT, LD, rdiag, kb, q);
+    } else if constexpr (FWD) {
+      if (kb == 64) uhmd::TriStep<0, false, false, true>::run(xv, sT, LD, rdiag, kb, q);
+      else uhmd::TriStep<0, false, false, false>::run(xv, sT, LD, rdiag, kb, q);
+    } else {
+      uhmd::TriStep<0, false, false, true>::run(xv, sT, LD, rdiag, 64, q);
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int e = 16 * q + j, k = FWD ? e : 63 - e;
+      sX[k * LD + c] = xv[j];
+    }
+  }
+  __syncthreads();
+  if (own) {
+    double* __restrict__ dst = (FWD ? z : x) + fr.xoff + (long long)c0 * n;
+#pragma unroll 4
+    for (int u = 0; u < 16; ++u) {
+      const int idx = tid + 256 * u, k = idx & 63, c = idx >> 6;
+      if (k < kb && c0 + c < nrhs) dst[(k0 + k) + (long long)c * n] = sX[k * LD + c];
+    }
+    return;
+  }
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int idx = tid + 256 * u;
+    if (!CHOL_T) {
+      const int i = idx & 63, k = idx >> 6;
+      sT[k * LD + i] = pa[u];
+    } else {
+      const int k = idx & 63, i = idx >> 6;
+      sT[k * LD + i] = pa[u];
+    }
+  }
+  __syncthreads();
+  d4 acc[2][2];
+  zero_acc(acc);
+  mma_quarter(acc, sX, sT, 64, lane, w);
+  sub_acc((FWD ? x : z) + fr.xoff, n, r, FWD ? fs : k0, c0, nrhs, acc, lane, w);
+}
+
+inline int ret(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
+
+template <typename T>
+int launch_step(int mode, int fwd, const uhmk_front* fronts, const uhmk_item* items, int n_items, int k0,
+                const T* arena, double* x, double* z, int nrhs, hipStream_t stream) {
+  const dim3 g(n_items), b(256);
+#define UHM_SVB_STEP(M, FW) \
+  hipLaunchKernelGGL((k_svb_step<T, M, FW>), g, b, 0, stream, fronts, items, arena, x, z, nrhs, k0)
+  if (mode == 0) {
+    if (fwd) UHM_SVB_STEP(0, true);
+    else UHM_SVB_STEP(0, false);
+  } else if (mode == 1) {
+    if (fwd) UHM_SVB_STEP(1, true);
+    else UHM_SVB_STEP(1, false);
+  } else {
+    if (fwd) UHM_SVB_STEP(2, true);
+    else UHM_SVB_STEP(2, false);
+  }
+#undef UHM_SVB_STEP
+  return ret(hipGetLastError());
+}
+
+template <typename T>
+int launch_gemm(int dir, const uhmk_front* fronts, const uhmk_item* items, int n_items, const T* arena, double* x,
+                double* z, int nrhs, hipStream_t stream) {
+  if (dir == 2)
+    hipLaunchKernelGGL((k_svb_gemm<T, true>), dim3(n_items), dim3(256), 0, stream, fronts, items, arena, x, z, nrhs,
+                       0);
+  else
+    hipLaunchKernelGGL((k_svb_gemm<T, false>), dim3(n_items), dim3(256), 0, stream, fronts, items, arena, x, z, nrhs,
+                       dir == 0 ? 1 : 0);
+  return ret(hipGetLastError());
+}
+
+}  // namespace
+
+extern "C" {
+
+int uhmk_solve_block_step(const uhmk_front* fronts, const uhmk_item* items, int n_items, int mode, int fwd, int k0,
+                          const void* arena, int prec, double* x, double* z, int nrhs, hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  if (mode < 0 || mode > 2 || k0 < 0 || k0 % 64) return -1010;
+  if (prec == 0)
+    return launch_step<double>(mode, fwd, fronts, items, n_items, k0, static_cast<const double*>(arena), x, z, nrhs,
+                               stream);
+  if (prec == 1)
+    return launch_step<float>(mode, fwd, fronts, items, n_items, k0, static_cast<const float*>(arena), x, z, nrhs,
+                              stream);
+  return -1006;
+}
+
+int uhmk_solve_block_gemm(const uhmk_front* fronts, const uhmk_item* items, int n_items, int dir,
+                          const void* arena, int prec, double* x, double* z, int nrhs, hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  if (dir < 0 || dir > 2) return -1010;
+  if (prec == 0)
+    return launch_gemm<double>(dir, fronts, items, n_items, static_cast<const double*>(arena), x, z, nrhs, stream);
+  if (prec == 1)
+    return launch_gemm<float>(dir, fronts, items, n_items, static_cast<const float*>(arena), x, z, nrhs, stream);
+  return -1006;
+}
+
+}  // extern "C"

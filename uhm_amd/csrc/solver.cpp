@@ -731,14 +731,19 @@ void FrontSolver::set_tuning(int key, int value) {
     // (values above every front size select 64 x 64 tiles everywhere)
     if (value != tile_threshold_) schedule_method_ = 0;
     tile_threshold_ = value;
+  } else if (key == 3) {
+    if (value < 1) throw std::runtime_error("set_tuning: block-solve threshold must be >= 1");
+    block_rhs_ = value;
   } else {
-    throw std::runtime_error("set_tuning: unknown key (1: Cholesky panel group, 2: 128-tile threshold)");
+    throw std::runtime_error(
+        "set_tuning: unknown key (1: Cholesky panel group, 2: 128-tile threshold, 3: block-solve threshold)");
   }
 }
 
 int FrontSolver::get_tuning(int key) const {
   if (key == 1) return chol_group();
   if (key == 2) return tile_threshold_;
+  if (key == 3) return block_rhs_;
   throw std::runtime_error("get_tuning: unknown key");
 }
 
@@ -846,12 +851,105 @@ int FrontSolver::factor(int method) {
   return 0;
 }
 
+bool FrontSolver::solve_schedule_current(int method) const {
+  const int inv = (method == UHM_CHOL && factor_nb_ == 64) ? 1 : 0;
+  const int blk = nrhs_ >= block_rhs_ ? 1 : 0;
+  return solve_method_ == method && solve_inv_ == inv && solve_block_ == blk;
+}
+
+void FrontSolver::build_block_solve_schedule(int method) {
+  // Block solves (many right-hand sides): per level, forward = gather,
+  // 64-row block steps on ATL, one product xb -= ABL z_t; backward =
+  // branch, one product z_t -= ATR xb / ABL^T xb, block steps.  Items
+  // {f, first right-hand side of the 64-column tile, first row, 0}; the
+  // tiles of one matrix chunk are adjacent so that they meet in L2.
+  const bool chol = (method == UHM_CHOL);
+  const int mode = chol ? (factor_nb_ == 64 ? 2 : 1) : 0;
+  const int ntile = (nrhs_ + 63) / 64;
+  auto step = [&](SolveStep::Kind kind, int off, int L) {
+    SolveStep st;
+    st.kind = kind;
+    st.off = off;
+    st.cnt = (int)sitems_.size() - off;
+    st.prec = level_prec_[L];
+    return st;
+  };
+  auto push = [&](int f, int r) {
+    for (int t = 0; t < ntile; ++t) sitems_.push_back(uhmk_item{f, 64 * t, r, 0});
+  };
+  const int nl = (int)levels_.size();
+  std::vector<int> mfs(nl, 0);
+  for (int L = 0; L < nl; ++L)
+    for (int f : levels_[L]) mfs[L] = std::max(mfs[L], fr_[f].fs);
+  for (int L = 0; L < nl; ++L) {
+    int o = (int)sitems_.size();
+    for (int f : levels_[L])
+      for (int r = 0; r < fr_[f].n; r += 256) sitems_.push_back(uhmk_item{f, 0, r, 0});
+    sfwd_.push_back(step(SolveStep::GATHER, o, L));
+    for (int k0 = 0; k0 < mfs[L]; k0 += 64) {
+      o = (int)sitems_.size();
+      for (int f : levels_[L]) {
+        const int fs = fr_[f].fs;
+        if (fs <= k0) continue;
+        push(f, k0);
+        for (int r = k0 + 64; r < fs; r += 64) push(f, r);
+      }
+      SolveStep st = step(SolveStep::BSTEP, o, L);
+      st.fwd = 1;
+      st.k0 = k0;
+      st.mode = mode;
+      sfwd_.push_back(st);
+    }
+    o = (int)sitems_.size();
+    for (int f : levels_[L]) {
+      const int fs = fr_[f].fs, n = fr_[f].n;
+      if (fs == 0) continue;
+      for (int r = fs; r < n; r += 64) push(f, r);
+    }
+    SolveStep g = step(SolveStep::BGEMM, o, L);
+    g.mode = 0;
+    if (g.cnt) sfwd_.push_back(g);
+  }
+  for (int L = nl - 1; L >= 0; --L) {
+    int o = (int)sitems_.size();
+    for (int f : levels_[L])
+      if (fr_[f].parent >= 0)
+        for (int r = 0; r < fr_[f].n - fr_[f].fs; r += 256) sitems_.push_back(uhmk_item{f, 0, r, 0});
+    SolveStep br = step(SolveStep::BRANCH, o, L);
+    if (br.cnt) sbwd_.push_back(br);
+    o = (int)sitems_.size();
+    for (int f : levels_[L]) {
+      const int fs = fr_[f].fs, n = fr_[f].n;
+      if (fs == 0 || n == fs) continue;
+      for (int r = 0; r < fs; r += 64) push(f, r);
+    }
+    SolveStep g = step(SolveStep::BGEMM, o, L);
+    g.mode = chol ? 2 : 1;
+    if (g.cnt) sbwd_.push_back(g);
+    for (int k0 = ((mfs[L] - 1) / 64) * 64; k0 >= 0 && mfs[L] > 0; k0 -= 64) {
+      o = (int)sitems_.size();
+      for (int f : levels_[L]) {
+        if (fr_[f].fs <= k0) continue;
+        push(f, k0);
+        for (int r = 0; r < k0; r += 64) push(f, r);
+      }
+      SolveStep st = step(SolveStep::BSTEP, o, L);
+      st.fwd = 0;
+      st.k0 = k0;
+      st.mode = mode;
+      sbwd_.push_back(st);
+    }
+  }
+}
+
 void FrontSolver::build_solve_schedule(int method) {
   // Step lists of the level solves, built once per pattern and method.
   // Fronts with fs <= 1024 solve ATL in one workgroup; larger ones in
   // 512-row super-blocks whose couplings go through the gemv kernels, so
-  // that a level's big fronts are read by many workgroups.
+  // that a level's big fronts are read by many workgroups.  From block_rhs_
+  // right-hand sides on, the block solves replace all of this.
   const bool chol = (method == UHM_CHOL);
+  const bool blk = nrhs_ >= block_rhs_;
   const bool inv = chol && factor_nb_ == 64;
   sitems_.clear();
   sflist_.clear();
@@ -901,7 +999,9 @@ void FrontSolver::build_solve_schedule(int method) {
     st.prec = level_prec_[L];
     return st;
   };
-  if (inv) {
+  if (blk) {
+    build_block_solve_schedule(method);
+  } else if (inv) {
     // Cholesky factors with diagonal-block inverses: 64-row block steps
     for (int L = 0; L < nl; ++L) {
       int o = (int)sitems_.size();
@@ -947,7 +1047,7 @@ void FrontSolver::build_solve_schedule(int method) {
       }
     }
   }
-  for (int L = 0; L < nl && !inv; ++L) {
+  for (int L = 0; L < nl && !inv && !blk; ++L) {
     const int rt = rt_of(L);
     int o = (int)sitems_.size();
     for (int f : levels_[L])
@@ -969,7 +1069,7 @@ void FrontSolver::build_solve_schedule(int method) {
       if (g.cnt) sfwd_.push_back(g);
     }
   }
-  for (int L = nl - 1; L >= 0 && !inv; --L) {
+  for (int L = nl - 1; L >= 0 && !inv && !blk; --L) {
     const int rt = rt_of(L);
     int o = (int)sitems_.size();
     for (int f : levels_[L])
@@ -1017,6 +1117,7 @@ void FrontSolver::build_solve_schedule(int method) {
   solve_graphs_.clear();
   solve_method_ = method;
   solve_inv_ = inv ? 1 : 0;
+  solve_block_ = blk ? 1 : 0;
 }
 
 void FrontSolver::run_solve_steps(const std::vector<SolveStep>& steps, bool chol, const double* rhs) {
@@ -1047,6 +1148,14 @@ void FrontSolver::run_solve_steps(const std::vector<SolveStep>& steps, bool chol
       case SolveStep::BRANCH:
         rc = uhmk_solve_branch(F, SI + st.off, st.cnt, d_emap_.as<int>(), X, nrhs_, stream_);
         break;
+      case SolveStep::BSTEP:
+        rc = uhmk_solve_block_step(F, SI + st.off, st.cnt, st.mode, st.fwd, st.k0, A, st.prec, X, d_z_.as<double>(),
+                                   nrhs_, stream_);
+        break;
+      case SolveStep::BGEMM:
+        rc = uhmk_solve_block_gemm(F, SI + st.off, st.cnt, st.mode, A, st.prec, X, d_z_.as<double>(), nrhs_,
+                                   stream_);
+        break;
     }
     check_k(rc, "solve");
   }
@@ -1071,7 +1180,7 @@ void FrontSolver::solve(int method) {
   if (!factored_) throw std::runtime_error("solve: not factorized");
   if (method != last_method_) throw std::runtime_error("solve: method differs from the factorization's");
   upload_rhs();
-  if (solve_method_ != method || solve_inv_ != (method == UHM_CHOL && factor_nb_ == 64 ? 1 : 0))
+  if (!solve_schedule_current(method))
     build_solve_schedule(method);
   const bool chol = (method == UHM_CHOL);
   hipEvent_t e0, e1;
@@ -1232,7 +1341,7 @@ int FrontSolver::refine(int method, double tol, int max_iter) {
   if (!factored_) throw std::runtime_error("refine: not factorized");
   if (method != last_method_) throw std::runtime_error("refine: method differs from the factorization's");
   refresh_csr();
-  if (solve_method_ != method || solve_inv_ != (method == UHM_CHOL && factor_nb_ == 64 ? 1 : 0))
+  if (!solve_schedule_current(method))
     build_solve_schedule(method);
   const bool chol = (method == UHM_CHOL);
   const long long N = ndof_;

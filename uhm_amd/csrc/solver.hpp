@@ -59,13 +59,14 @@ constexpr int N_LAUNCH_KINDS = 7;
 
 // One launch of the level solves.
 struct SolveStep {
-  enum Kind { GATHER, TRSV, GEMV, BRANCH, CSTEP } kind = GATHER;
+  enum Kind { GATHER, TRSV, GEMV, BRANCH, CSTEP, BSTEP, BGEMM } kind = GATHER;
   int off = 0, cnt = 0;            // items (gather / gemv / branch) or front list (trsv)
   int max_fs = 0, sb = 0, sb_size = 0;  // trsv
   int cols = 0;                    // gemv: row chunks (0) or column chunks (1)
   int prec = 0;                    // storage of the level's fronts
   int fwd = 0;                     // trsv / cstep: forward (1) or backward (0)
-  int k0 = 0;                      // cstep: first row of the block
+  int k0 = 0;                      // cstep / bstep: first row of the block
+  int mode = 0;                    // bstep: 0 LU, 1 Cholesky, 2 Cholesky with W; bgemm: direction
 };
 
 // Per-kind totals of a profiled factorization (device events per launch).
@@ -145,7 +146,9 @@ class FrontSolver {
     nb_ = nb;
   }
   // Schedule knobs: key 1 = Cholesky panels per trailing update, key 2 =
-  // mean tile extent from which a GEMM launch uses 128 x 128 tiles.
+  // mean tile extent from which a GEMM launch uses 128 x 128 tiles, key 3 =
+  // number of right-hand sides from which the solves run as block solves
+  // (MFMA products over 64-column tiles of right-hand sides).
   void set_tuning(int key, int value);
   int get_tuning(int key) const;
   // Times factorizations of the current values over the knobs and keeps the
@@ -177,6 +180,8 @@ class FrontSolver {
   void build_schedule(int method);
   void build_csr();
   void build_solve_schedule(int method);
+  void build_block_solve_schedule(int method);
+  bool solve_schedule_current(int method) const;
   void layout_arenas();
   void* arena_ptr(int prec) const { return prec ? d_arena32_.p : d_arena_.p; }
   void residual_norms(int col, double* h4);
@@ -196,6 +201,7 @@ class FrontSolver {
   int nb_ = 64;
   int chol_group_ = 0;  // 0: by precision (FP64 8, mixed 4, the autotuner's usual picks)
   int tile_threshold_ = 1024;
+  int block_rhs_ = 8;
   bool created_ = false;
   hipStream_t stream_ = nullptr;
 
@@ -253,6 +259,7 @@ class FrontSolver {
   int last_method_ = 0;
   int factor_nb_ = 64;  // panel width of the last factorization
   int solve_inv_ = -1;  // solve schedule built for inverse-carrying Cholesky factors (1) or not (0)
+  int solve_block_ = -1;  // solve schedule built as block solves (1) or per right-hand side (0)
   bool factored_ = false;
   bool profile_ = false;
   bool use_graph_ = false;
