@@ -217,18 +217,40 @@ int uhmk_csr_values(long long nnz, const long long* cptr, const long long* cidx,
 int uhmk_gather_rhs(long long nrow, const long long* bptr, const long long* bidx, const int* bn, const double* b,
                     double* bg, int col, hipStream_t stream);
 
-/* r = b - A x and out4 = {max|r|, max row-sum |A|, max|x|, max|b|} (out4
- * zeroed by the caller); replaces check_* / get_residual
- * (UHM/src/mesh/lu_piv.cxx:184-196, UHM/src/mesh/matrix.cxx:241-275). */
-int uhmk_residual(int nrow, const long long* rowptr, const int* colidx, const double* val, const double* xg,
-                  const double* bg, double* r, double* out4, hipStream_t stream);
+/* Residual r = b - A x for all right-hand sides from the leaf input
+ * matrices (A = sum_e P_e K_e P_e^T, the assembly of DSparse_::import_element,
+ * UHM/src/interf/sparse.cxx:196-267), replacing check_* / get_residual
+ * (UHM/src/mesh/lu_piv.cxx:184-196, UHM/src/mesh/matrix.cxx:241-275) for
+ * many columns at once.  Compact leaf layout: leaf f's n x n_rhs block at
+ * loff_f * n_rhs, leading dimension n.
+ *   uhmk_leaf_gather    xl_e = P_e^T xg; items {f, 0, first row, loff_f},
+ *                       lg[loff_f + i] = global row of leaf row i
+ *   uhmk_leaf_mul       yl_e = K_e xl_e (FP64 MFMA); items {f, first
+ *                       right-hand side of a 64-column tile, first row,
+ *                       loff_f}; K_e from the leaf stages
+ *   uhmk_leaf_residual  r(g) = b(g) - sum of the yl entries of row g (lpos:
+ *                       compact position of each entry listed in bptr, bn
+ *                       its leaf's n), leaf order; out[4 c + {0, 2, 3}] =
+ *                       max over rows of |r|, |x|, |b| of column c (atomic
+ *                       maxima: out zeroed by the caller)
+ *   uhmk_rowsum_max     out = max row sum |A| of the CSR matrix (atomic
+ *                       maximum: out zeroed by the caller)
+ * xg, bg, r: ndof x n_rhs, column major. */
+int uhmk_leaf_gather(const uhmk_front* fronts, const uhmk_item* items, int n_items, const long long* lg,
+                     const double* xg, long long nglob, double* xl, int nrhs, hipStream_t stream);
+int uhmk_leaf_mul(const uhmk_front* fronts, const uhmk_item* items, int n_items, const double* stage,
+                  const double* xl, double* yl, int nrhs, hipStream_t stream);
+int uhmk_leaf_residual(long long nrow, const long long* bptr, const long long* lpos, const int* bn,
+                       const double* yl, const double* xg, const double* bg, double* r, int nrhs, double* out,
+                       hipStream_t stream);
+int uhmk_rowsum_max(long long nrow, const long long* rowptr, const double* val, double* out, hipStream_t stream);
 
 /* Between per-front solution blocks and a global vector over the factor
- * rows: dir 0 g = xf (fronts -> global), dir 1 xf = g (global -> fronts),
- * dir 2 g += xf (refinement update); the analogue of copy_out for the
- * whole mesh (UHM/src/mesh/interf.cxx:183-199). */
-int uhmk_front_global(const uhmk_front* fronts, int n_fronts, const long long* goff, double* xf, double* g,
-                      int col, long long nglob, int dir, hipStream_t stream);
+ * rows, all n_rhs columns: dir 0 g = xf (fronts -> global), dir 1 xf = g
+ * (global -> fronts), dir 2 g += xf (refinement update); the analogue of
+ * copy_out for the whole mesh (UHM/src/mesh/interf.cxx:183-199). */
+int uhmk_front_global_all(const uhmk_front* fronts, int n_fronts, const long long* goff, double* xf, double* g,
+                          int nrhs, long long nglob, int dir, hipStream_t stream);
 
 #ifdef __cplusplus
 }

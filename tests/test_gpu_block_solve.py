@@ -141,3 +141,33 @@ def test_block_solve_mixed_refinement(lib, method):
     say("  corrections", n, " ".join("%.2e" % e for e in eta))
     assert m.backward_error() <= 1e-12
     assert eta[0] > 1e-12 * 1e-3
+
+
+@pytest.mark.parametrize("nrhs", [1, 70])
+def test_leaf_residual_matches_csr(lib, nrhs):
+    """The residual from the leaf products (check / refinement) against
+    b - A x on the host with the assembled CSR matrix, on a mixed-precision
+    solution whose residual is far above rounding (so the two summation
+    orders must agree to 1e-6 relative)."""
+    method = U.UHM_LU_PIV
+    m = _mesh((5, 5, 4), 3, method, nrhs, seed=12)
+    m.set_precision(U.PREC_MIXED, 0)
+    assert _factor(m, method) == 0
+    _solve(m, method)
+    _check(m, method)
+    eta_dev, r1_dev = m.backward_error(), m.get_residual()
+    rp, col, val = m.fetch_csr()
+    x, b = m.fetch_x(), m.fetch_b()
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    Ax = np.zeros_like(x)
+    for c in range(nrhs):
+        Ax[:, c] = np.bincount(rows, weights=val * x[col, c], minlength=n)
+    r = b - Ax
+    amax = np.bincount(rows, weights=np.abs(val), minlength=n).max()
+    eta = max(np.abs(r[:, c]).max() / (amax * np.abs(x[:, c]).max() + np.abs(b[:, c]).max()) for c in range(nrhs))
+    r1 = np.abs(r).sum()
+    say("  eta device %.6e host %.6e   |r|_1 device %.6e host %.6e" % (eta_dev, eta, r1_dev, r1))
+    assert eta > 1e-12
+    assert abs(eta_dev - eta) <= 1e-6 * eta
+    assert abs(r1_dev - r1) <= 1e-6 * r1

@@ -608,73 +608,6 @@ __global__ __launch_bounds__(256) void k_gen_elements(const uhmk_front* __restri
   }
 }
 
-// ------------------------------------------------------------------------
-// Assembled-matrix residual r = b - A x (CSR, FP64) and the norms of the
-// normwise backward error ||r||inf / (||A||inf ||x||inf + ||b||inf).
-// out[0..3] += {max|r_i|, max row sum |A|, max|x_i|, max|b_i|} by atomics on
-// non-negative doubles ordered as unsigned 64-bit integers.
-// ------------------------------------------------------------------------
-__device__ inline void atomic_max_nonneg(double* p, double v) {
-  atomicMax(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
-}
-
-__global__ __launch_bounds__(256) void k_residual(int nrow, const long long* __restrict__ rowptr,
-                                                  const int* __restrict__ colidx,
-                                                  const double* __restrict__ val,
-                                                  const double* __restrict__ xg,
-                                                  const double* __restrict__ bg, double* __restrict__ r,
-                                                  double* __restrict__ out) {
-  // one wavefront per row, grid-stride over rows; maxima kept per wave and
-  // published with one atomic each at the end
-  const int lane = threadIdx.x & 63;
-  const int nwaves = gridDim.x * 4;
-  double mr = 0.0, ma = 0.0, mx = 0.0, mb = 0.0;
-  for (int row = blockIdx.x * 4 + (threadIdx.x >> 6); row < nrow; row += nwaves) {
-    double ax = 0.0, an = 0.0;
-    for (long long p = rowptr[row] + lane; p < rowptr[row + 1]; p += 64) {
-      const double a = val[p];
-      ax += a * xg[colidx[p]];
-      an += fabs(a);
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      ax += __shfl_xor(ax, o);
-      an += __shfl_xor(an, o);
-    }
-    const double ri = bg[row] - ax;
-    if (lane == 0) r[row] = ri;
-    mr = fmax(mr, fabs(ri));
-    ma = fmax(ma, an);
-    mx = fmax(mx, fabs(xg[row]));
-    mb = fmax(mb, fabs(bg[row]));
-  }
-  if (lane == 0) {
-    atomic_max_nonneg(out + 0, mr);
-    atomic_max_nonneg(out + 1, ma);
-    atomic_max_nonneg(out + 2, mx);
-    atomic_max_nonneg(out + 3, mb);
-  }
-}
-
-// Global <-> front vectors over the factor rows of every front: dir 0
-// g = xf, dir 1 xf = g, dir 2 g += xf (refinement update); one workgroup
-// per front.
-__global__ __launch_bounds__(256) void k_front_global(const uhmk_front* __restrict__ F, int nfront,
-                                                      const long long* __restrict__ goff,
-                                                      double* __restrict__ xf, double* __restrict__ g,
-                                                      int col, long long nglob, int dir) {
-  const int f = blockIdx.x;
-  if (f >= nfront) return;
-  const uhmk_front fr = F[f];
-  const long long xo = fr.xoff + (long long)col * fr.n;
-  const long long go = goff[f] + (long long)col * nglob;
-  for (int i = threadIdx.x; i < fr.fs; i += 256) {
-    if (dir == 0) g[go + i] = xf[xo + i];
-    else if (dir == 1) xf[xo + i] = g[go + i];
-    else g[go + i] += xf[xo + i];
-  }
-}
-
 // Assembled values: val[p] = sum of the leaf-stage entries listed for slot
 // p, in leaf order (deterministic).
 __global__ __launch_bounds__(256) void k_csr_values(long long nnz, const long long* __restrict__ cptr,
@@ -843,14 +776,6 @@ int uhmk_generate_elements(const uhmk_front* fronts, const uhmk_item* items, int
   return ret(hipGetLastError());
 }
 
-int uhmk_residual(int nrow, const long long* rowptr, const int* colidx, const double* val, const double* xg,
-                  const double* bg, double* r, double* out4, hipStream_t stream) {
-  if (nrow <= 0) return 0;
-  const int blocks = (nrow + 3) / 4 < 2048 ? (nrow + 3) / 4 : 2048;
-  hipLaunchKernelGGL(k_residual, dim3(blocks), dim3(256), 0, stream, nrow, rowptr, colidx, val, xg, bg, r, out4);
-  return ret(hipGetLastError());
-}
-
 int uhmk_csr_values(long long nnz, const long long* cptr, const long long* cidx, const double* stage, double* val,
                     hipStream_t stream) {
   if (nnz <= 0) return 0;
@@ -864,14 +789,6 @@ int uhmk_gather_rhs(long long nrow, const long long* bptr, const long long* bidx
   if (nrow <= 0) return 0;
   hipLaunchKernelGGL(k_gather_rhs, dim3((unsigned)((nrow + 255) / 256)), dim3(256), 0, stream, nrow, bptr, bidx, bn,
                      b, bg, col);
-  return ret(hipGetLastError());
-}
-
-int uhmk_front_global(const uhmk_front* fronts, int n_fronts, const long long* goff, double* xf, double* g,
-                      int col, long long nglob, int dir, hipStream_t stream) {
-  if (n_fronts <= 0) return 0;
-  hipLaunchKernelGGL(k_front_global, dim3(n_fronts), dim3(256), 0, stream, fronts, n_fronts, goff, xf, g, col,
-                     nglob, dir);
   return ret(hipGetLastError());
 }
 

@@ -72,8 +72,9 @@ __device__ __forceinline__ void zero_acc(d4 (&acc)[2][2]) {
     for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
 }
 
-// C(r + i, c0 + c) -= acc(c, i) for rows r + i < rend and columns
-// c0 + c < nrhs; C has leading dimension n.
+// C(r + i, c0 + c) -= acc(c, i) (STORE: = acc(c, i)) for rows r + i <
+// rend and columns c0 + c < nrhs; C has leading dimension n.
+template <bool STORE = false>
 __device__ __forceinline__ void sub_acc(double* __restrict__ C, long long n, int r, int rend, int c0, int nrhs,
                                         const d4 (&acc)[2][2], int lane, int w) {
   const int cb = 32 * (w & 1), ib = 32 * (w >> 1);
@@ -86,52 +87,39 @@ __device__ __forceinline__ void sub_acc(double* __restrict__ C, long long n, int
         const int c = cb + 16 * a + 4 * g + (lane >> 4), i = ib + 16 * b + (lane & 15);
         if (r + i < rend && c0 + c < nrhs) {
           double* p = C + (r + i) + (long long)(c0 + c) * n;
-          *p -= acc[a][b][g];
+          if (STORE) *p = acc[a][b][g];
+          else *p -= acc[a][b][g];
         }
       }
 }
 
-// ------------------------------------------------------------------------
-// Level-batched products on the off-diagonal blocks.  item {f, c0, r0, 0}:
-// the 64 x 64 tile of rows r0.. and right-hand sides c0.. of
-//   dir 0:  x[rows of ABL] -= F[rows, 0:fs) z[0:fs)          (forward)
-//   dir 1:  z[rows of ATL] -= F[rows, fs:n) x[fs:n)          (LU backward)
-//   dir 2:  z[rows of ATL] -= F[fs:n, rows]^T x[fs:n)        (Cholesky backward)
-// K runs in 16-deep stages through double-buffered LDS; the next stage's
-// global loads are in flight while the current one is multiplied.
-// ------------------------------------------------------------------------
+// One 64 x 64 tile of P = op(A)[rows r0.., K] S[K, c0..] accumulated into
+// acc, K = [kbeg, kend) in 16-deep stages through double-buffered LDS (the
+// next stage's global loads are in flight while the current one is
+// multiplied).  op(A)(i, k) = A[i + k lda] (NN) or A[k + i lda] (TA); rows
+// at or beyond rend and columns at or beyond nrhs read as zero.  S points
+// at column c0 (leading dimension lds).
 template <typename T, bool TA>
-__global__ __launch_bounds__(256, 2) void k_svb_gemm(const uhmk_front* __restrict__ F,
-                                                     const uhmk_item* __restrict__ items,
-                                                     const T* __restrict__ arena, double* __restrict__ x,
-                                                     double* __restrict__ z, int nrhs, int fwd) {
-  __shared__ double sA[2][16 * LD];
-  __shared__ double sB[2][16 * LD];
-  const uhmk_item it = items[blockIdx.x];
-  const uhmk_front fr = F[it.f];
-  const int n = fr.n, fs = fr.fs;
-  const int c0 = it.a, r0 = it.b;
-  const int rend = fwd ? n : fs;
-  const int kbeg = fwd ? 0 : fs, kend = fwd ? fs : n;
-  const T* __restrict__ A = arena + fr.off;
-  const double* __restrict__ S = (fwd ? z : x) + fr.xoff + (long long)c0 * n;
-  double* __restrict__ D = (fwd ? x : z) + fr.xoff;
+__device__ __forceinline__ void tile_product(d4 (&acc)[2][2], const T* __restrict__ A, long long lda, int r0,
+                                             int rend, int kbeg, int kend, const double* __restrict__ S,
+                                             long long lds, int c0, int nrhs, double (*sA)[16 * LD],
+                                             double (*sB)[16 * LD]) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   // staging coordinates: A NN i = tid & 63, k = tid >> 6 (+ 4 u);
-  // A TN and B: k = tid & 15, i or c = tid >> 4 (+ 16 u)
+  // A TN and S: k = tid & 15, i or c = tid >> 4 (+ 16 u)
   double pa[4], pb[4];
   auto load = [&](int k) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       if (!TA) {
         const int i = tid & 63, kk = (tid >> 6) + 4 * u;
-        pa[u] = uhmd::ldz(A, (r0 + i) + (long long)(k + kk) * n, r0 + i < rend && k + kk < kend);
+        pa[u] = uhmd::ldz(A, (r0 + i) + (long long)(k + kk) * lda, r0 + i < rend && k + kk < kend);
       } else {
         const int kk = tid & 15, i = (tid >> 4) + 16 * u;
-        pa[u] = uhmd::ldz(A, (k + kk) + (long long)(r0 + i) * n, r0 + i < rend && k + kk < kend);
+        pa[u] = uhmd::ldz(A, (k + kk) + (long long)(r0 + i) * lda, r0 + i < rend && k + kk < kend);
       }
       const int kk = tid & 15, c = (tid >> 4) + 16 * u;
-      pb[u] = uhmd::ldz(S, (k + kk) + (long long)c * n, k + kk < kend && c0 + c < nrhs);
+      pb[u] = uhmd::ldz(S, (k + kk) + (long long)c * lds, k + kk < kend && c0 + c < nrhs);
     }
   };
   auto store = [&](int buf) {
@@ -148,8 +136,6 @@ __global__ __launch_bounds__(256, 2) void k_svb_gemm(const uhmk_front* __restric
       sB[buf][kk * LD + c] = pb[u];
     }
   };
-  d4 acc[2][2];
-  zero_acc(acc);
   load(kbeg);
   store(0);
   __syncthreads();
@@ -162,7 +148,162 @@ __global__ __launch_bounds__(256, 2) void k_svb_gemm(const uhmk_front* __restric
     __syncthreads();
     buf ^= 1;
   }
-  sub_acc(D, n, r0, rend, c0, nrhs, acc, lane, w);
+}
+
+// ------------------------------------------------------------------------
+// Level-batched products on the off-diagonal blocks.  item {f, c0, r0, 0}:
+// the 64 x 64 tile of rows r0.. and right-hand sides c0.. of
+//   dir 0:  x[rows of ABL] -= F[rows, 0:fs) z[0:fs)          (forward)
+//   dir 1:  z[rows of ATL] -= F[rows, fs:n) x[fs:n)          (LU backward)
+//   dir 2:  z[rows of ATL] -= F[fs:n, rows]^T x[fs:n)        (Cholesky backward)
+// ------------------------------------------------------------------------
+template <typename T, bool TA>
+__global__ __launch_bounds__(256, 2) void k_svb_gemm(const uhmk_front* __restrict__ F,
+                                                     const uhmk_item* __restrict__ items,
+                                                     const T* __restrict__ arena, double* __restrict__ x,
+                                                     double* __restrict__ z, int nrhs, int fwd) {
+  __shared__ double sA[2][16 * LD];
+  __shared__ double sB[2][16 * LD];
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs;
+  const int c0 = it.a, r0 = it.b;
+  const int rend = fwd ? n : fs;
+  const double* __restrict__ S = (fwd ? z : x) + fr.xoff + (long long)c0 * n;
+  d4 acc[2][2];
+  zero_acc(acc);
+  tile_product<T, TA>(acc, arena + fr.off, n, r0, rend, fwd ? 0 : fs, fwd ? fs : n, S, n, c0, nrhs, sA, sB);
+  sub_acc((fwd ? x : z) + fr.xoff, n, r0, rend, c0, nrhs, acc, threadIdx.x & 63, threadIdx.x >> 6);
+}
+
+// ------------------------------------------------------------------------
+// Residual r = b - A x for all right-hand sides at once, with A the sum of
+// the leaf input matrices (A = sum_e P_e K_e P_e^T, the assembly that
+// DSparse_::import_element performs, UHM/src/interf/sparse.cxx:196-267):
+//   leaf gather   xl_e = P_e^T x                (compact leaf layout)
+//   leaf product  yl_e = K_e xl_e               (FP64 MFMA, 64 x 64 tiles)
+//   row gather    r(g) = b(g) - sum of the yl entries of row g, in leaf
+//                 order (deterministic), with per-column maxima of |r|,
+//                 |x| and |b| for the normwise backward error.
+// Compact leaf layout: leaf f's n x n_rhs block starts at loff_f * n_rhs
+// (leading dimension n); item.c carries loff_f.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_leaf_gather(const uhmk_front* __restrict__ F,
+                                                     const uhmk_item* __restrict__ items,
+                                                     const long long* __restrict__ lg, const double* __restrict__ xg,
+                                                     long long nglob, double* __restrict__ xl, int nrhs) {
+  const uhmk_item it = items[blockIdx.x];
+  const int n = F[it.f].n;
+  const int i = it.b + threadIdx.x;
+  if (i >= n) return;
+  const long long g = lg[it.c + i];
+  double* __restrict__ o = xl + (long long)it.c * nrhs + i;
+  for (int c = 0; c < nrhs; ++c) o[(long long)c * n] = xg[g + (long long)c * nglob];
+}
+
+__global__ __launch_bounds__(256, 2) void k_leaf_mul(const uhmk_front* __restrict__ F,
+                                                     const uhmk_item* __restrict__ items,
+                                                     const double* __restrict__ stage,
+                                                     const double* __restrict__ xl, double* __restrict__ yl,
+                                                     int nrhs) {
+  __shared__ double sA[2][16 * LD];
+  __shared__ double sB[2][16 * LD];
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, c0 = it.a, r0 = it.b;
+  const long long base = (long long)it.c * nrhs;
+  d4 acc[2][2];
+  zero_acc(acc);
+  tile_product<double, false>(acc, stage + fr.soff, n, r0, n, 0, n, xl + base + (long long)c0 * n, n, c0, nrhs, sA,
+                              sB);
+  sub_acc<true>(yl + base, n, r0, n, c0, nrhs, acc, threadIdx.x & 63, threadIdx.x >> 6);
+}
+
+__device__ inline void atomic_max_nonneg(double* p, double v) {
+  atomicMax(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
+}
+
+// grid (rows / 256, columns / RC): thread = one global row, RC columns.
+constexpr int RC = 8;
+__global__ __launch_bounds__(256) void k_leaf_residual(long long nrow, const long long* __restrict__ bptr,
+                                                       const long long* __restrict__ lpos,
+                                                       const int* __restrict__ bn, const double* __restrict__ yl,
+                                                       const double* __restrict__ xg, const double* __restrict__ bg,
+                                                       double* __restrict__ r, int nrhs, double* __restrict__ out) {
+  __shared__ double red[3][4][RC];
+  const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int c0 = blockIdx.y * RC;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double mr[RC], mx[RC], mb[RC];
+#pragma unroll
+  for (int u = 0; u < RC; ++u) mr[u] = mx[u] = mb[u] = 0.0;
+  if (g < nrow) {
+    double acc[RC];
+#pragma unroll
+    for (int u = 0; u < RC; ++u) acc[u] = 0.0;
+    for (long long q = bptr[g]; q < bptr[g + 1]; ++q) {
+      const long long p = lpos[q];
+      const long long ld = bn[q];
+#pragma unroll
+      for (int u = 0; u < RC; ++u)
+        if (c0 + u < nrhs) acc[u] += yl[p + (long long)(c0 + u) * ld];
+    }
+#pragma unroll
+    for (int u = 0; u < RC; ++u)
+      if (c0 + u < nrhs) {
+        const long long o = g + (long long)(c0 + u) * nrow;
+        const double bv = bg[o], ri = bv - acc[u];
+        r[o] = ri;
+        mr[u] = fabs(ri);
+        mx[u] = fabs(xg[o]);
+        mb[u] = fabs(bv);
+      }
+  }
+#pragma unroll
+  for (int u = 0; u < RC; ++u) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      mr[u] = fmax(mr[u], __shfl_xor(mr[u], o));
+      mx[u] = fmax(mx[u], __shfl_xor(mx[u], o));
+      mb[u] = fmax(mb[u], __shfl_xor(mb[u], o));
+    }
+  }
+  if (lane == 0)
+#pragma unroll
+    for (int u = 0; u < RC; ++u) {
+      red[0][w][u] = mr[u];
+      red[1][w][u] = mx[u];
+      red[2][w][u] = mb[u];
+    }
+  __syncthreads();
+  if (threadIdx.x < 3 * RC) {
+    const int k = threadIdx.x / RC, u = threadIdx.x % RC;
+    if (c0 + u < nrhs) {
+      const double m = fmax(fmax(red[k][0][u], red[k][1][u]), fmax(red[k][2][u], red[k][3][u]));
+      // out per column: {max|r|, max row sum |A| (set by the caller), max|x|, max|b|}
+      atomic_max_nonneg(out + 4 * (c0 + u) + (k == 0 ? 0 : k + 1), m);
+    }
+  }
+}
+
+// Global <-> front vectors over the factor rows of every front, all
+// columns: dir 0 g = xf, dir 1 xf = g, dir 2 g += xf.  One workgroup per
+// front.
+__global__ __launch_bounds__(256) void k_front_global_all(const uhmk_front* __restrict__ F,
+                                                          const long long* __restrict__ goff,
+                                                          double* __restrict__ xf, double* __restrict__ g,
+                                                          int nrhs, long long nglob, int dir) {
+  const uhmk_front fr = F[blockIdx.x];
+  const int fs = fr.fs;
+  const long long tot = (long long)fs * nrhs;
+  for (long long idx = threadIdx.x; idx < tot; idx += 256) {
+    const int i = (int)(idx % fs), c = (int)(idx / fs);
+    const long long xo = fr.xoff + i + (long long)c * fr.n;
+    const long long go = goff[blockIdx.x] + i + (long long)c * nglob;
+    if (dir == 0) g[go] = xf[xo];
+    else if (dir == 1) xf[xo] = g[go];
+    else g[go] += xf[xo];
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -357,6 +498,22 @@ __global__ __launch_bounds__(256) void k_svb_step(const uhmk_front* __restrict__
   sub_acc((FWD ? x : z) + fr.xoff, n, r, FWD ? fs : k0, c0, nrhs, acc, lane, w);
 }
 
+// out = max over rows of sum |A(row, :)| (CSR), one wavefront per row.
+__global__ __launch_bounds__(256) void k_rowsum_max(long long nrow, const long long* __restrict__ rowptr,
+                                                    const double* __restrict__ val, double* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const long long nw = (long long)gridDim.x * 4;
+  double m = 0.0;
+  for (long long row = (long long)blockIdx.x * 4 + (threadIdx.x >> 6); row < nrow; row += nw) {
+    double a = 0.0;
+    for (long long p = rowptr[row] + lane; p < rowptr[row + 1]; p += 64) a += fabs(val[p]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+    m = fmax(m, a);
+  }
+  if (lane == 0) atomic_max_nonneg(out, m);
+}
+
 inline int ret(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
 
 template <typename T>
@@ -417,6 +574,46 @@ int uhmk_solve_block_gemm(const uhmk_front* fronts, const uhmk_item* items, int 
   if (prec == 1)
     return launch_gemm<float>(dir, fronts, items, n_items, static_cast<const float*>(arena), x, z, nrhs, stream);
   return -1006;
+}
+
+int uhmk_leaf_gather(const uhmk_front* fronts, const uhmk_item* items, int n_items, const long long* lg,
+                     const double* xg, long long nglob, double* xl, int nrhs, hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  hipLaunchKernelGGL(k_leaf_gather, dim3(n_items), dim3(256), 0, stream, fronts, items, lg, xg, nglob, xl, nrhs);
+  return ret(hipGetLastError());
+}
+
+int uhmk_leaf_mul(const uhmk_front* fronts, const uhmk_item* items, int n_items, const double* stage,
+                  const double* xl, double* yl, int nrhs, hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  hipLaunchKernelGGL(k_leaf_mul, dim3(n_items), dim3(256), 0, stream, fronts, items, stage, xl, yl, nrhs);
+  return ret(hipGetLastError());
+}
+
+int uhmk_leaf_residual(long long nrow, const long long* bptr, const long long* lpos, const int* bn,
+                       const double* yl, const double* xg, const double* bg, double* r, int nrhs, double* out,
+                       hipStream_t stream) {
+  if (nrow <= 0 || nrhs <= 0) return 0;
+  if ((nrhs + RC - 1) / RC > 65535) return -1011;
+  const dim3 g((unsigned)((nrow + 255) / 256), (unsigned)((nrhs + RC - 1) / RC));
+  hipLaunchKernelGGL(k_leaf_residual, g, dim3(256), 0, stream, nrow, bptr, lpos, bn, yl, xg, bg, r, nrhs, out);
+  return ret(hipGetLastError());
+}
+
+int uhmk_rowsum_max(long long nrow, const long long* rowptr, const double* val, double* out, hipStream_t stream) {
+  if (nrow <= 0) return 0;
+  const long long b = (nrow + 3) / 4 < 2048 ? (nrow + 3) / 4 : 2048;
+  hipLaunchKernelGGL(k_rowsum_max, dim3((unsigned)b), dim3(256), 0, stream, nrow, rowptr, val, out);
+  return ret(hipGetLastError());
+}
+
+int uhmk_front_global_all(const uhmk_front* fronts, int n_fronts, const long long* goff, double* xf, double* g,
+                          int nrhs, long long nglob, int dir, hipStream_t stream) {
+  if (n_fronts <= 0) return 0;
+  if (dir < 0 || dir > 2) return -1010;
+  hipLaunchKernelGGL(k_front_global_all, dim3(n_fronts), dim3(256), 0, stream, fronts, goff, xf, g, nrhs, nglob,
+                     dir);
+  return ret(hipGetLastError());
 }
 
 }  // extern "C"

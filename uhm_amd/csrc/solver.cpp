@@ -1230,8 +1230,17 @@ void FrontSolver::build_csr() {
     for (int f : leaves)
       for (int i = 0; i < fr_[f].n; ++i) rl[pos[gidx[f][i]]++] = {f, i};
   }
+  // compact leaf layout of the leaf-product residual
+  std::vector<long long> loff(nf, -1);
+  long long NL = 0;
+  for (int f : leaves) {
+    loff[f] = NL;
+    NL += fr_[f].n;
+  }
+  if (NL >= (1LL << 31)) throw std::runtime_error("residual: leaf rows exceed the 32-bit item range");
   std::vector<long long> rowptr(N + 1, 0);
   std::vector<int> colidx;
+  std::vector<long long> lpos;
   std::vector<long long> cptr(1, 0), cidx;
   std::vector<long long> bptr(N + 1, 0), bidx, bn;
   std::vector<int> mark(N, -1);
@@ -1265,6 +1274,7 @@ void FrontSolver::build_csr() {
       for (int j = 0; j < n; ++j) tmp.push_back({slot_of(gidx[f][j]), fr_[f].soff + i + (long long)j * n});
       bidx.push_back(fr_[f].xoff + i);
       bn.push_back(n);
+      lpos.push_back(loff[f] * nrhs_ + i);
     }
     bptr[g + 1] = (long long)bidx.size();
     std::stable_sort(tmp.begin(), tmp.end(),
@@ -1286,6 +1296,26 @@ void FrontSolver::build_csr() {
   d_bidx_.upload(bidx, stream_);
   std::vector<int> bn32(bn.begin(), bn.end());
   d_bn_.upload(bn32, stream_);
+  {
+    std::vector<long long> lg(std::max(1LL, NL));
+    std::vector<uhmk_item> gi, mi;
+    for (int f : leaves) {
+      const int n = fr_[f].n, lo = (int)loff[f];
+      for (int i = 0; i < n; ++i) lg[loff[f] + i] = gidx[f][i];
+      for (int r = 0; r < n; r += 256) gi.push_back(uhmk_item{f, 0, r, lo});
+      for (int r = 0; r < n; r += 64)
+        for (int c0 = 0; c0 < nrhs_; c0 += 64) mi.push_back(uhmk_item{f, c0, r, lo});
+    }
+    d_lg_.upload(lg, stream_);
+    d_lpos_.upload(lpos, stream_);
+    d_lgi_.upload(gi, stream_);
+    d_lmi_.upload(mi, stream_);
+    n_lgi_ = (int)gi.size();
+    n_lmi_ = (int)mi.size();
+    d_xl_.alloc(std::max(1LL, NL * nrhs_) * sizeof(double));
+    d_yl_.alloc(std::max(1LL, NL * nrhs_) * sizeof(double));
+    d_amax_.alloc(sizeof(double));
+  }
   d_val_.alloc(std::max(1LL, nnz_) * sizeof(double));
   d_xg_.alloc(std::max(1LL, N * nrhs_) * sizeof(double));
   d_bg_.alloc(std::max(1LL, N * nrhs_) * sizeof(double));
@@ -1307,10 +1337,16 @@ void FrontSolver::refresh_csr() {
   upload_stage();
   upload_rhs();
   const long long N = ndof_;
-  if (vals_dirty_)
+  if (vals_dirty_) {
     check_k(uhmk_csr_values(nnz_, d_cptr_.as<long long>(), d_cidx_.as<long long>(), d_stage_.as<double>(),
                             d_val_.as<double>(), stream_),
             "csr values");
+    check_hip(hipMemsetAsync(d_amax_.p, 0, sizeof(double), stream_), "memset amax");
+    check_k(uhmk_rowsum_max(N, d_rowptr_.as<long long>(), d_val_.as<double>(), d_amax_.as<double>(), stream_),
+            "row sums");
+    check_hip(hipMemcpyAsync(&amax_, d_amax_.p, sizeof(double), hipMemcpyDeviceToHost, stream_), "amax");
+    check_hip(hipStreamSynchronize(stream_), "amax");
+  }
   if (bg_dirty_)
     for (int col = 0; col < nrhs_; ++col)
       check_k(uhmk_gather_rhs(N, d_bptr_.as<long long>(), d_bidx_.as<long long>(), d_bn_.as<int>(), d_b_.as<double>(),
@@ -1319,18 +1355,28 @@ void FrontSolver::refresh_csr() {
   vals_dirty_ = bg_dirty_ = false;
 }
 
-void FrontSolver::residual_norms(int col, double* h) {
-  // r(:, col) = b - A x(:, col) from the global x; h = {max|r|, max row sum
-  // |A|, max|x|, max|b|}
+void FrontSolver::residual_all(std::vector<double>& h) {
+  // r = b - A x for every column from the global x (leaf gather, leaf
+  // products, row gather); h[4 c ..] = {max|r|, max row sum |A|, max|x|,
+  // max|b|} of column c.
   const long long N = ndof_;
-  double* nrm = d_norms_.as<double>() + 4 * col;
-  check_hip(hipMemsetAsync(nrm, 0, 4 * sizeof(double), stream_), "memset norms");
-  check_k(uhmk_residual((int)N, d_rowptr_.as<long long>(), d_colidx_.as<int>(), d_val_.as<double>(),
-                        d_xg_.as<double>() + (long long)col * N, d_bg_.as<double>() + (long long)col * N,
-                        d_r_.as<double>() + (long long)col * N, nrm, stream_),
+  const uhmk_front* F = dF_.as<uhmk_front>();
+  double* nrm = d_norms_.as<double>();
+  h.assign(4 * (size_t)nrhs_, 0.0);
+  check_hip(hipMemsetAsync(nrm, 0, 4 * sizeof(double) * nrhs_, stream_), "memset norms");
+  check_k(uhmk_leaf_gather(F, d_lgi_.as<uhmk_item>(), n_lgi_, d_lg_.as<long long>(), d_xg_.as<double>(), N,
+                           d_xl_.as<double>(), nrhs_, stream_),
+          "leaf gather");
+  check_k(uhmk_leaf_mul(F, d_lmi_.as<uhmk_item>(), n_lmi_, d_stage_.as<double>(), d_xl_.as<double>(),
+                        d_yl_.as<double>(), nrhs_, stream_),
+          "leaf products");
+  check_k(uhmk_leaf_residual(N, d_bptr_.as<long long>(), d_lpos_.as<long long>(), d_bn_.as<int>(),
+                             d_yl_.as<double>(), d_xg_.as<double>(), d_bg_.as<double>(), d_r_.as<double>(), nrhs_,
+                             nrm, stream_),
           "residual");
-  check_hip(hipMemcpyAsync(h, nrm, 4 * sizeof(double), hipMemcpyDeviceToHost, stream_), "norms");
+  check_hip(hipMemcpyAsync(h.data(), nrm, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_), "norms");
   check_hip(hipStreamSynchronize(stream_), "residual");
+  for (int c = 0; c < nrhs_; ++c) h[4 * c + 1] = amax_;
 }
 
 int FrontSolver::refine(int method, double tol, int max_iter) {
@@ -1349,39 +1395,33 @@ int FrontSolver::refine(int method, double tol, int max_iter) {
   const int nf = (int)fr_.size();
   const auto t0 = std::chrono::steady_clock::now();
   refine_hist_.clear();
-  for (int col = 0; col < nrhs_; ++col)
-    check_k(uhmk_front_global(F, nf, d_goff_.as<long long>(), d_x_.as<double>(), d_xg_.as<double>(), col, N, 0,
-                              stream_),
-            "gather x");
+  const long long* GO = d_goff_.as<long long>();
+  check_k(uhmk_front_global_all(F, nf, GO, d_x_.as<double>(), d_xg_.as<double>(), nrhs_, N, 0, stream_), "gather x");
   int it = 0;
-  double h[4];
+  std::vector<double> h;
   for (;; ++it) {
+    residual_all(h);
     double worst = 0.0;
     for (int col = 0; col < nrhs_; ++col) {
-      residual_norms(col, h);
-      worst = std::max(worst, h[0] / (h[1] * h[2] + h[3]));
+      const double* hc = &h[4 * col];
+      const double e = hc[0] / (hc[1] * hc[2] + hc[3]);
+      worst = (e == e) ? std::max(worst, e) : e;
+      if (!(worst == worst)) break;
     }
     refine_hist_.push_back(worst);
     eta_ = worst;
     if (worst <= tol || it >= max_iter || !(worst == worst)) break;
     // correction: d = M^-1 r, right-hand side r on the factor rows of the
     // owning fronts (the schur rows of the refinement buffer stay zero)
-    for (int col = 0; col < nrhs_; ++col)
-      check_k(uhmk_front_global(F, nf, d_goff_.as<long long>(), d_rf_.as<double>(), d_r_.as<double>(), col, N, 1,
-                                stream_),
-              "scatter r");
+    check_k(uhmk_front_global_all(F, nf, GO, d_rf_.as<double>(), d_r_.as<double>(), nrhs_, N, 1, stream_),
+            "scatter r");
     run_solve(chol, d_rf_.as<double>());
-    for (int col = 0; col < nrhs_; ++col)
-      check_k(uhmk_front_global(F, nf, d_goff_.as<long long>(), d_x_.as<double>(), d_xg_.as<double>(), col, N, 2,
-                                stream_),
-              "x += d");
+    check_k(uhmk_front_global_all(F, nf, GO, d_x_.as<double>(), d_xg_.as<double>(), nrhs_, N, 2, stream_), "x += d");
   }
   // the refined solution back into the fronts (factor rows, then the
   // branch pass for the schur rows) so that copy_out sees it
-  for (int col = 0; col < nrhs_; ++col)
-    check_k(uhmk_front_global(F, nf, d_goff_.as<long long>(), d_x_.as<double>(), d_xg_.as<double>(), col, N, 1,
-                              stream_),
-            "scatter x");
+  check_k(uhmk_front_global_all(F, nf, GO, d_x_.as<double>(), d_xg_.as<double>(), nrhs_, N, 1, stream_),
+          "scatter x");
   std::vector<SolveStep> br;
   for (const SolveStep& st : sbwd_)
     if (st.kind == SolveStep::BRANCH) br.push_back(st);
@@ -1399,30 +1439,23 @@ void FrontSolver::check(int) {
   if (!factored_) throw std::runtime_error("check: not factorized");
   refresh_csr();
   const long long N = ndof_;
+  check_k(uhmk_front_global_all(dF_.as<uhmk_front>(), (int)fr_.size(), d_goff_.as<long long>(), d_x_.as<double>(),
+                                d_xg_.as<double>(), nrhs_, N, 0, stream_),
+          "gather x");
+  std::vector<double> h;
+  residual_all(h);
+  std::vector<double> r((size_t)N * nrhs_);
+  check_hip(hipMemcpy(r.data(), d_r_.p, r.size() * sizeof(double), hipMemcpyDeviceToHost), "residual");
   eta_ = 0.0;
   residual_1norm_ = 0.0;
-  std::vector<double> r(N);
+  for (double& v : norms_) v = 0.0;
   for (int col = 0; col < nrhs_; ++col) {
-    check_k(uhmk_front_global(dF_.as<uhmk_front>(), (int)fr_.size(), d_goff_.as<long long>(), d_x_.as<double>(),
-                              d_xg_.as<double>(), col, N, 0, stream_),
-            "gather x");
-    double* nrm = d_norms_.as<double>() + 4 * col;
-    check_hip(hipMemsetAsync(nrm, 0, 4 * sizeof(double), stream_), "memset norms");
-    check_k(uhmk_residual((int)N, d_rowptr_.as<long long>(), d_colidx_.as<int>(), d_val_.as<double>(),
-                          d_xg_.as<double>() + (long long)col * N, d_bg_.as<double>() + (long long)col * N,
-                          d_r_.as<double>() + (long long)col * N, nrm, stream_),
-            "residual");
-    double h[4];
-    check_hip(hipMemcpyAsync(h, nrm, sizeof(h), hipMemcpyDeviceToHost, stream_), "norms");
-    check_hip(hipMemcpyAsync(r.data(), d_r_.as<double>() + (long long)col * N, N * sizeof(double),
-                             hipMemcpyDeviceToHost, stream_),
-              "residual");
-    check_hip(hipStreamSynchronize(stream_), "check");
-    for (int k = 0; k < 4; ++k) norms_[k] = std::max(norms_[k], h[k]);
-    const double eta = h[0] / (h[1] * h[2] + h[3]);
-    eta_ = std::max(eta_, eta);
-    for (long long i = 0; i < N; ++i) residual_1norm_ += std::fabs(r[i]);
+    const double* hc = &h[4 * col];
+    for (int k = 0; k < 4; ++k) norms_[k] = std::max(norms_[k], hc[k]);
+    const double eta = hc[0] / (hc[1] * hc[2] + hc[3]);
+    eta_ = (eta == eta) ? std::max(eta_, eta) : eta;
   }
+  for (double v : r) residual_1norm_ += std::fabs(v);
 }
 
 void FrontSolver::csr(std::vector<long long>& rowptr, std::vector<int>& col, std::vector<double>& val) {
@@ -1493,12 +1526,10 @@ void FrontSolver::fetch_rhs_front(int f, double* out) {
 }
 
 void FrontSolver::fetch_x_global(double* out) {
-  for (int col = 0; col < nrhs_; ++col) {
-    if (!d_xg_.p) d_xg_.alloc(std::max(1LL, ndof_ * nrhs_) * sizeof(double));
-    check_k(uhmk_front_global(dF_.as<uhmk_front>(), (int)fr_.size(), d_goff_.as<long long>(), d_x_.as<double>(),
-                              d_xg_.as<double>(), col, ndof_, 0, stream_),
-            "gather x");
-  }
+  if (!d_xg_.p) d_xg_.alloc(std::max(1LL, ndof_ * nrhs_) * sizeof(double));
+  check_k(uhmk_front_global_all(dF_.as<uhmk_front>(), (int)fr_.size(), d_goff_.as<long long>(), d_x_.as<double>(),
+                                d_xg_.as<double>(), nrhs_, ndof_, 0, stream_),
+          "gather x");
   check_hip(hipStreamSynchronize(stream_), "gather x");
   check_hip(hipMemcpy(out, d_xg_.p, ndof_ * nrhs_ * sizeof(double), hipMemcpyDeviceToHost), "fetch x global");
 }

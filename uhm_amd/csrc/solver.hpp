@@ -184,7 +184,7 @@ class FrontSolver {
   bool solve_schedule_current(int method) const;
   void layout_arenas();
   void* arena_ptr(int prec) const { return prec ? d_arena32_.p : d_arena_.p; }
-  void residual_norms(int col, double* h4);
+  void residual_all(std::vector<double>& h);  // 4 per column: max|r|, max row sum |A|, max|x|, max|b|
   void run_solve_steps(const std::vector<SolveStep>& steps, bool chol, const double* rhs);
   void run_solve(bool chol, const double* rhs);
   void enqueue_factor(int method, std::vector<hipEvent_t>* pev);
@@ -251,6 +251,10 @@ class FrontSolver {
   bool csr_built_ = false, vals_dirty_ = true, bg_dirty_ = true;
   long long nnz_ = 0;
   DevBuf d_rowptr_, d_colidx_, d_val_, d_cptr_, d_cidx_, d_bptr_, d_bidx_, d_xg_, d_bg_, d_r_, d_norms_, d_bn_;
+  // leaf-product residual: compact leaf layout, its items and buffers
+  DevBuf d_lg_, d_lpos_, d_lgi_, d_lmi_, d_xl_, d_yl_, d_amax_;
+  int n_lgi_ = 0, n_lmi_ = 0;
+  double amax_ = 0.0;  // max row sum |A| of the current values
 
   double residual_1norm_ = 0.0, eta_ = 0.0;
   double norms_[4] = {0, 0, 0, 0};
