@@ -45,14 +45,18 @@ WORKLOADS = {
     # batched uneven fronts: FP32 fronts, FP64 refinement, 64 RHS, graph-replayed
     # re-factorization over value reseeds on the fixed pattern
     "varp": dict(kind=3, dims=(256, 256), p=8, mesh_seed=12345, method=3, alpha=-1.0, beta=0.1, scale_kind=0,
-                 p0=0.0, p1=0.0, nrhs=64, top=0, reseeds=10,
+                 p0=0.0, p1=0.0, nrhs=64, top=0, reseeds=10, refine="auto",
                  name="2D 256^2 quads, p_e ~ U{1..8} (min-p edges), K_e=M^T M+n I+0.1 S, LU, FP32 fronts + FP64 "
                       "refinement, 64 RHS, graph-replayed re-factorization over 10 reseeds"),
     # low precision: Cholesky, 256 RHS
     "q3big": dict(kind=2, dims=(24, 24, 24), p=3, method=1, alpha=64.0, beta=0.0, scale_kind=1, p0=0.5, p1=0.0,
-                  nrhs=256, top=1,
+                  nrhs=256, top=1, refine="auto",
                   name="3D 24^3 hex Q3, SPD K_e=(M^T M+64 I)*lognormal(0,0.5), Cholesky, 256 RHS"),
 }
+
+
+REFINE_KINDS = {"classical": lambda U: U.REFINE_CLASSICAL, "gmres": lambda U: U.REFINE_GMRES,
+                "auto": lambda U: U.REFINE_AUTO}
 
 
 def log(*a):
@@ -129,6 +133,7 @@ def run_path(U, m, w, mode, top, args):
     solve = m.solve_chol if chol else m.solve_lu_piv
     name = "fp64" if mode == U.PREC_FP64 else "mixed"
     m.set_precision(mode, top)
+    m.set_refinement(REFINE_KINDS[args.refine or w.get("refine", "classical")](U), 20, 1e-6)
     tuned = None
     if args.tune:
         # the project's autotuner picks the schedule knobs on this pattern
@@ -162,10 +167,12 @@ def run_path(U, m, w, mode, top, args):
     device_sync()
     t_sol = (time.perf_counter() - t0) * 1e3
     hist, ref_ms = m.refine_history()
+    n_solves = m.refine_solves()
     eta = m.backward_error()
     f_ms, s_ms = m.timing()
-    log("[%s] factor+solve+refine %.2f ms (factor %.2f, solve %.2f, refine %.2f ms, %d corrections), "
-        "backward error %s" % (name, t_sol, f_ms, s_ms, ref_ms, n_corr, " -> ".join("%.2e" % e for e in hist)))
+    log("[%s] factor+solve+refine %.2f ms (factor %.2f, solve %.2f, refine %.2f ms, %d corrections, %d solves), "
+        "backward error %s" % (name, t_sol, f_ms, s_ms, ref_ms, n_corr, n_solves,
+                               " -> ".join("%.2e" % e for e in hist)))
     m.set_profile(True)
     factor()
     stats = m.kernel_stats()
@@ -175,7 +182,7 @@ def run_path(U, m, w, mode, top, args):
             log("  %-10s %5d launches %9.3f ms  %8.3f TFLOP/s  %8.1f GB/s"
                 % (k, n_k, ms_k, fl_k / ms_k / 1e9 if ms_k else 0, by_k / ms_k / 1e6 if ms_k else 0))
     return dict(ms_step=ms_step, t_sol=t_sol, eta=eta, hist=[float(e) for e in hist], n_corr=n_corr, solve_ms=s_ms,
-                refine_ms=ref_ms, stats=stats, tuned=tuned)
+                refine_ms=ref_ms, n_solves=n_solves, stats=stats, tuned=tuned)
 
 
 def run_reseeds(U, m, w, args):
@@ -186,6 +193,7 @@ def run_reseeds(U, m, w, args):
     factor = m.chol_with_free if chol else m.lu_piv_with_free
     solve = m.solve_chol if chol else m.solve_lu_piv
     m.set_precision(U.PREC_MIXED, w.get("top", 0))
+    m.set_refinement(REFINE_KINDS[args.refine or w.get("refine", "classical")](U), 20, 1e-6)
     m.set_graph(True)
     out = []
     for seed in range(1, w["reseeds"] + 1):
@@ -203,9 +211,9 @@ def run_reseeds(U, m, w, args):
         f_ms, s_ms = m.timing()
         hist, ref_ms = m.refine_history()
         out.append(dict(seed=seed, info=info, ms=t, factor_ms=f_ms, solve_ms=s_ms, refine_ms=ref_ms, corrections=n,
-                        eta=float(m.backward_error())))
-        log("[reseed %d] %.2f ms (factor %.2f, solve %.2f, refine %.2f, %d corrections) eta %.2e info %d"
-            % (seed, t, f_ms, s_ms, ref_ms, n, m.backward_error(), info))
+                        solves=m.refine_solves(), eta=float(m.backward_error())))
+        log("[reseed %d] %.2f ms (factor %.2f, solve %.2f, refine %.2f, %d corrections, %d solves) eta %.2e info %d"
+            % (seed, t, f_ms, s_ms, ref_ms, n, m.refine_solves(), m.backward_error(), info))
     m.set_graph(False)
     return out
 
@@ -229,6 +237,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--nb", type=int, default=64, help="panel width")
     ap.add_argument("--top", type=int, default=2, help="tree levels kept in FP64 by the mixed path")
+    ap.add_argument("--refine", choices=sorted(REFINE_KINDS), default=None,
+                    help="refinement kind (default: the workload's; classical for chol / lu)")
     ap.add_argument("--tune", action="store_true",
                     help="run the autotuner over the schedule knobs before timing (default: built-in knobs)")
     args = ap.parse_args()
@@ -269,6 +279,7 @@ def main():
         return {"factor_tflops": tflops(r), "factor_ms": round(r["ms_step"], 4),
                 "time_to_1e-12_ms": round(r["t_sol"], 3) if r["eta"] <= 1e-12 else None,
                 "backward_error": r["eta"], "backward_error_history": r["hist"], "corrections": r["n_corr"],
+                "refine_solves": r["n_solves"],
                 "solve_ms": round(r["solve_ms"], 3), "refine_ms": round(r["refine_ms"], 3), "schedule": r["tuned"],
                 "kernels": {k: {"ms": round(v[0], 4), "gflop": round(v[1] / 1e9, 3),
                                 "gbytes": round(v[2] / 1e9, 4), "launches": v[3]}
@@ -295,6 +306,7 @@ def main():
             "precision_path": ("mixed: FP32 fronts and FP32 MFMA Schur updates below the top %d tree levels, "
                                "FP64 top-level pivot blocks, FP64-residual refinement to backward error 1e-12"
                                % top) if mixed_ok else "fp64 (mixed path did not reach 1e-12)",
+            "refinement": args.refine or w.get("refine", "classical"),
             "panel_width": args.nb,
             "n_rhs": w.get("nrhs", 1),
         },

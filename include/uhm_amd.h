@@ -169,9 +169,21 @@ void uhm_improve_lu_piv_(uhm_fort_p* mesh, uhm_fort_double* tol, uhm_fort_int* m
 /* Backward error before each correction of the last refinement (n in:
  * capacity of eta, out: number of entries) and its wall time in ms. */
 void uhm_get_refine_history_(uhm_fort_p* mesh, uhm_fort_double* eta, uhm_fort_int* n, uhm_fort_double* ms);
+/* Correction kind of uhm_improve_*_: kind 0 classical (d = M^-1 r,
+ * default), 1 GMRES-IR (d from restarted GMRES(restart) on the
+ * left-preconditioned system M^-1 A d = M^-1 r, stopped when the
+ * preconditioned residual falls to inner_tol times its start; restart
+ * 1..31), 2 classical while every correction cuts the backward error at
+ * least tenfold, GMRES-IR from the first one that does not.  Extension
+ * (no reference counterpart: the reference's improve_* are stubs). */
+void uhm_set_refinement_(uhm_fort_p* mesh, uhm_fort_int* kind, uhm_fort_int* restart, uhm_fort_double* inner_tol);
+/* Applications of M^-1 (level solves) in the last refinement. */
+void uhm_get_refine_solves_(uhm_fort_p* mesh, uhm_fort_int* n);
 /* Schedule knobs: key 1 = Cholesky panels per trailing update (default 0:
  * 8 on the FP64 path, 4 on the mixed path), key 2 = mean tile extent from
- * which a GEMM launch uses 128 x 128 tiles (default 1024). */
+ * which a GEMM launch uses 128 x 128 tiles (default 1024), key 3 = number
+ * of right-hand sides from which the solves run as block solves, FP64 MFMA
+ * products over 64-column tiles (default 8). */
 void uhm_set_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value);
 void uhm_get_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value);
 /* Autotuner: times factorizations of the current values over the knobs

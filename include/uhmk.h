@@ -232,7 +232,8 @@ int uhmk_gather_rhs(long long nrow, const long long* bptr, const long long* bidx
  *                       compact position of each entry listed in bptr, bn
  *                       its leaf's n), leaf order; out[4 c + {0, 2, 3}] =
  *                       max over rows of |r|, |x|, |b| of column c (atomic
- *                       maxima: out zeroed by the caller)
+ *                       maxima: out zeroed by the caller).  With bg and out
+ *                       null: the product r = A x (GMRES-IR operator)
  *   uhmk_rowsum_max     out = max row sum |A| of the CSR matrix (atomic
  *                       maximum: out zeroed by the caller)
  * xg, bg, r: ndof x n_rhs, column major. */
@@ -251,6 +252,22 @@ int uhmk_rowsum_max(long long nrow, const long long* rowptr, const double* val, 
  * copy_out for the whole mesh (UHM/src/mesh/interf.cxx:183-199). */
 int uhmk_front_global_all(const uhmk_front* fronts, int n_fronts, const long long* goff, double* xf, double* g,
                           int nrhs, long long nglob, int dir, hipStream_t stream);
+
+/* ---- GMRES-IR vector kernels ------------------------------------------------
+ * Global vectors of ndof x n_rhs (column major); a basis V of k such blocks
+ * at stride vstride.  Fixed-order reductions (bitwise reproducible).
+ * Refinement slot: Mesh_::improve_chol / improve_lu_piv
+ * (UHM/src/mesh/chol.cxx:136-149, UHM/src/mesh/lu_piv.cxx:232-245).
+ *   uhmk_block_dots     out[i * n_rhs + c] = V_i(:, c) . w(:, c), i < k <= 32;
+ *                       partial: uhmk_block_dots_scratch doubles
+ *   uhmk_basis_combine  w = beta w + alpha sum_i coef[i * n_rhs + c] V_i(:, c)
+ *   uhmk_scale_columns  w(:, c) *= s[c] */
+long long uhmk_block_dots_scratch(long long nrow, int nrhs);
+int uhmk_block_dots(const double* V, long long vstride, int k, const double* w, long long nrow, int nrhs,
+                    double* partial, double* out, hipStream_t stream);
+int uhmk_basis_combine(const double* V, long long vstride, int k, const double* coef, double alpha, double beta,
+                       double* w, long long nrow, int nrhs, hipStream_t stream);
+int uhmk_scale_columns(double* w, long long nrow, int nrhs, const double* s, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -85,6 +85,7 @@ _SIGS = {
     "uhm_export_connectivity_": [_P, _I, _I, _I, _I, _I, _I, _I, _I, _I],
     "uhm_set_profile_": [_P, _I], "uhm_get_kernel_stats_": [_P, _D, _D, _D, _I],
     "uhm_set_precision_": [_P, _I, _I], "uhm_set_graph_": [_P, _I],
+    "uhm_set_refinement_": [_P, _I, _I, _D], "uhm_get_refine_solves_": [_P, _I],
     "uhm_set_tuning_": [_P, _I, _I], "uhm_get_tuning_": [_P, _I, _I], "uhm_autotune_": [_P, _I, _D],
     "uhm_improve_chol_": [_P, _D, _I, _I], "uhm_improve_lu_piv_": [_P, _D, _I, _I],
     "uhm_get_refine_history_": [_P, _D, _I, _D],
@@ -95,6 +96,7 @@ _RET = {"uhm_get_last_error_": C.c_int}
 
 
 PREC_FP64, PREC_MIXED = 0, 1
+REFINE_CLASSICAL, REFINE_GMRES, REFINE_AUTO = 0, 1, 2
 
 
 class UHMError(RuntimeError):
@@ -318,6 +320,17 @@ class Mesh:
         n, ms = C.c_int(len(eta)), C.c_double(0)
         _call("uhm_get_refine_history_", C.byref(self.handle), _dptr(eta), C.byref(n), C.byref(ms))
         return eta[:min(n.value, len(eta))].copy(), ms.value
+
+    def set_refinement(self, kind, restart=20, inner_tol=1e-6):
+        """Correction kind of improve(): REFINE_CLASSICAL, REFINE_GMRES
+        (GMRES-IR, restart length, inner tolerance) or REFINE_AUTO."""
+        _call("uhm_set_refinement_", C.byref(self.handle), _i(kind), _i(restart), C.byref(C.c_double(inner_tol)))
+
+    def refine_solves(self):
+        """Level-solve applications of M^-1 in the last improve()."""
+        n = C.c_int(0)
+        _call("uhm_get_refine_solves_", C.byref(self.handle), C.byref(n))
+        return n.value
 
     def set_tuning(self, key, value):
         _call("uhm_set_tuning_", C.byref(self.handle), _i(key), _i(value))

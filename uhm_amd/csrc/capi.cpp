@@ -262,6 +262,12 @@ void uhm_get_refine_history_(uhm_fort_p* mesh, uhm_fort_double* eta, uhm_fort_in
     *ms = S(M(mesh))->last_refine_ms();
   });
 }
+void uhm_set_refinement_(uhm_fort_p* mesh, uhm_fort_int* kind, uhm_fort_int* restart, uhm_fort_double* inner_tol) {
+  guard("uhm_set_refinement", [&] { S(M(mesh))->set_refinement(*kind, *restart, *inner_tol); });
+}
+void uhm_get_refine_solves_(uhm_fort_p* mesh, uhm_fort_int* n) {
+  guard("uhm_get_refine_solves", [&] { *n = S(M(mesh))->refine_solves(); });
+}
 void uhm_set_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value) {
   guard("uhm_set_tuning", [&] { S(M(mesh))->set_tuning(*key, *value); });
 }

@@ -252,6 +252,10 @@ __global__ __launch_bounds__(256) void k_leaf_residual(long long nrow, const lon
     for (int u = 0; u < RC; ++u)
       if (c0 + u < nrhs) {
         const long long o = g + (long long)(c0 + u) * nrow;
+        if (!bg) {
+          r[o] = acc[u];  // product mode: r = A x
+          continue;
+        }
         const double bv = bg[o], ri = bv - acc[u];
         r[o] = ri;
         mr[u] = fabs(ri);
@@ -259,6 +263,7 @@ __global__ __launch_bounds__(256) void k_leaf_residual(long long nrow, const lon
         mb[u] = fabs(bv);
       }
   }
+  if (!out) return;
 #pragma unroll
   for (int u = 0; u < RC; ++u) {
 #pragma unroll

@@ -1379,6 +1379,155 @@ void FrontSolver::residual_all(std::vector<double>& h) {
   for (int c = 0; c < nrhs_; ++c) h[4 * c + 1] = amax_;
 }
 
+void FrontSolver::set_refinement(int kind, int restart, double inner_tol) {
+  if (kind < 0 || kind > 2) throw std::runtime_error("set_refinement: kind 0 (classical), 1 (GMRES-IR) or 2 (auto)");
+  if (restart < 1 || restart > 31) throw std::runtime_error("set_refinement: GMRES restart length must be 1..31");
+  if (!(inner_tol > 0.0 && inner_tol < 1.0)) throw std::runtime_error("set_refinement: inner tolerance in (0, 1)");
+  refine_kind_ = kind;
+  gmres_m_ = restart;
+  gmres_tol_ = inner_tol;
+}
+
+void FrontSolver::apply_matrix(const double* v, double* out) {
+  const uhmk_front* F = dF_.as<uhmk_front>();
+  check_k(uhmk_leaf_gather(F, d_lgi_.as<uhmk_item>(), n_lgi_, d_lg_.as<long long>(), v, ndof_, d_xl_.as<double>(),
+                           nrhs_, stream_),
+          "leaf gather");
+  check_k(uhmk_leaf_mul(F, d_lmi_.as<uhmk_item>(), n_lmi_, d_stage_.as<double>(), d_xl_.as<double>(),
+                        d_yl_.as<double>(), nrhs_, stream_),
+          "leaf products");
+  check_k(uhmk_leaf_residual(ndof_, d_bptr_.as<long long>(), d_lpos_.as<long long>(), d_bn_.as<int>(),
+                             d_yl_.as<double>(), nullptr, nullptr, out, nrhs_, nullptr, stream_),
+          "product");
+}
+
+void FrontSolver::apply_factors(bool chol, const double* r, double* out) {
+  // r on the factor rows of the owning fronts (the schur rows of the
+  // refinement buffer stay zero), the level solves, the factor rows back
+  const uhmk_front* F = dF_.as<uhmk_front>();
+  const int nf = (int)fr_.size();
+  check_k(uhmk_front_global_all(F, nf, d_goff_.as<long long>(), d_rf_.as<double>(), const_cast<double*>(r), nrhs_,
+                                ndof_, 1, stream_),
+          "scatter r");
+  run_solve(chol, d_rf_.as<double>());
+  if (out)
+    check_k(uhmk_front_global_all(F, nf, d_goff_.as<long long>(), d_x_.as<double>(), out, nrhs_, ndof_, 0, stream_),
+            "gather d");
+  ++refine_solves_;
+}
+
+void FrontSolver::gmres_correction(bool chol) {
+  // One GMRES-IR correction (Carson and Higham, SIAM J. Sci. Comput. 40
+  // (2018) A817): GMRES(m) on the left-preconditioned system
+  // M^-1 A d = M^-1 r from d = 0, every right-hand side in lockstep,
+  // classical Gram-Schmidt with one reorthogonalisation, Givens rotations on
+  // the host; each column stops at ||M^-1 (r - A d)|| <= inner_tol
+  // ||M^-1 r|| (or at a happy breakdown); then x += V y.  M^-1 and A are
+  // applied in FP64 arithmetic (factors stored in their level precision);
+  // r is the current residual in d_r_.
+  const long long N = ndof_;
+  const int nr = nrhs_, m = gmres_m_;
+  const long long vs = N * nr;
+  d_V_.alloc((size_t)(m + 1) * vs * sizeof(double));
+  d_dpart_.alloc(std::max(1LL, uhmk_block_dots_scratch(N, nr)) * sizeof(double));
+  d_dout_.alloc((size_t)(m + 1) * nr * sizeof(double));
+  d_coef_.alloc((size_t)(m + 1) * nr * sizeof(double));
+  double* V = d_V_.as<double>();
+  double* dout = d_dout_.as<double>();
+  double* coef = d_coef_.as<double>();
+  std::vector<double> hv((size_t)(m + 1) * nr), sc(nr), beta(nr);
+  std::vector<double> H((size_t)nr * (m + 1) * m, 0.0), cs((size_t)nr * m), sn((size_t)nr * m);
+  std::vector<double> g((size_t)nr * (m + 1), 0.0);
+  std::vector<int> kc(nr, -1);  // steps at which a column stopped (-1: running)
+  auto Hc = [&](int c, int i, int j) -> double& { return H[((size_t)c * (m + 1) + i) * m + j]; };
+  auto fetch = [&](int k) {
+    check_hip(hipMemcpyAsync(hv.data(), dout, (size_t)k * nr * sizeof(double), hipMemcpyDeviceToHost, stream_),
+              "gmres dots");
+    check_hip(hipStreamSynchronize(stream_), "gmres dots");
+  };
+  auto scale = [&](double* w) {
+    check_hip(hipMemcpyAsync(coef, sc.data(), nr * sizeof(double), hipMemcpyHostToDevice, stream_), "gmres scale");
+    check_k(uhmk_scale_columns(w, N, nr, coef, stream_), "gmres scale");
+  };
+  auto norms = [&](double* w) {
+    check_k(uhmk_block_dots(w, vs, 1, w, N, nr, d_dpart_.as<double>(), dout, stream_), "gmres norm");
+    fetch(1);
+  };
+  // v0 = M^-1 r / beta
+  apply_factors(chol, d_r_.as<double>(), V);
+  norms(V);
+  for (int c = 0; c < nr; ++c) {
+    beta[c] = std::sqrt(std::max(0.0, hv[c]));
+    g[(size_t)c * (m + 1)] = beta[c];
+    if (!(beta[c] > 0.0)) kc[c] = 0;
+    sc[c] = beta[c] > 0.0 ? 1.0 / beta[c] : 0.0;
+  }
+  scale(V);
+  int k = 0;
+  std::vector<double> hj((size_t)(m + 1) * nr);
+  for (int j = 0; j < m; ++j) {
+    bool any = false;
+    for (int c = 0; c < nr; ++c) any |= kc[c] < 0;
+    if (!any) break;
+    double* W = V + (long long)(j + 1) * vs;
+    apply_matrix(V + (long long)j * vs, d_r_.as<double>());
+    apply_factors(chol, d_r_.as<double>(), W);
+    std::fill(hj.begin(), hj.end(), 0.0);
+    for (int pass = 0; pass < 2; ++pass) {
+      check_k(uhmk_block_dots(V, vs, j + 1, W, N, nr, d_dpart_.as<double>(), dout, stream_), "gmres dots");
+      check_k(uhmk_basis_combine(V, vs, j + 1, dout, -1.0, 1.0, W, N, nr, stream_), "gmres orthogonalise");
+      fetch(j + 1);
+      for (size_t q = 0; q < (size_t)(j + 1) * nr; ++q) hj[q] += hv[q];
+    }
+    norms(W);
+    for (int c = 0; c < nr; ++c) {
+      sc[c] = 0.0;
+      if (kc[c] >= 0) continue;
+      for (int i = 0; i <= j; ++i) Hc(c, i, j) = hj[(size_t)i * nr + c];
+      const double hn = std::sqrt(std::max(0.0, hv[c]));
+      Hc(c, j + 1, j) = hn;
+      for (int i = 0; i < j; ++i) {
+        const double a = Hc(c, i, j), b = Hc(c, i + 1, j);
+        const double ci = cs[(size_t)c * m + i], si = sn[(size_t)c * m + i];
+        Hc(c, i, j) = ci * a + si * b;
+        Hc(c, i + 1, j) = -si * a + ci * b;
+      }
+      const double a = Hc(c, j, j), b = Hc(c, j + 1, j);
+      const double rr = std::hypot(a, b);
+      const double cj = rr > 0.0 ? a / rr : 1.0, sj = rr > 0.0 ? b / rr : 0.0;
+      cs[(size_t)c * m + j] = cj;
+      sn[(size_t)c * m + j] = sj;
+      Hc(c, j, j) = rr;
+      Hc(c, j + 1, j) = 0.0;
+      double* gc = &g[(size_t)c * (m + 1)];
+      gc[j + 1] = -sj * gc[j];
+      gc[j] = cj * gc[j];
+      if (std::fabs(gc[j + 1]) <= gmres_tol_ * beta[c] || !(hn > 0.0)) kc[c] = j + 1;
+      else sc[c] = 1.0 / hn;
+    }
+    scale(W);
+    k = j + 1;
+  }
+  // y = R^-1 g per column; x += V y
+  std::vector<double> y((size_t)(m + 1) * nr, 0.0);
+  for (int c = 0; c < nr; ++c) {
+    const int kk = kc[c] < 0 ? k : kc[c];
+    const double* gc = &g[(size_t)c * (m + 1)];
+    for (int i = kk - 1; i >= 0; --i) {
+      double t = gc[i];
+      for (int l = i + 1; l < kk; ++l) t -= Hc(c, i, l) * y[(size_t)l * nr + c];
+      const double d = Hc(c, i, i);
+      y[(size_t)i * nr + c] = d != 0.0 ? t / d : 0.0;
+    }
+  }
+  if (k > 0) {
+    check_hip(hipMemcpyAsync(coef, y.data(), (size_t)k * nr * sizeof(double), hipMemcpyHostToDevice, stream_),
+              "gmres y");
+    check_k(uhmk_basis_combine(V, vs, k, coef, 1.0, 1.0, d_xg_.as<double>(), N, nr, stream_), "x += V y");
+  }
+  check_hip(hipStreamSynchronize(stream_), "gmres");
+}
+
 int FrontSolver::refine(int method, double tol, int max_iter) {
   // Iterative refinement with FP64 residuals on the assembled matrix and
   // corrections through the (possibly low-precision) factors: the slot of
@@ -1395,29 +1544,41 @@ int FrontSolver::refine(int method, double tol, int max_iter) {
   const int nf = (int)fr_.size();
   const auto t0 = std::chrono::steady_clock::now();
   refine_hist_.clear();
+  refine_solves_ = 0;
   const long long* GO = d_goff_.as<long long>();
   check_k(uhmk_front_global_all(F, nf, GO, d_x_.as<double>(), d_xg_.as<double>(), nrhs_, N, 0, stream_), "gather x");
-  int it = 0;
   std::vector<double> h;
-  for (;; ++it) {
+  auto worst_eta = [&]() {
     residual_all(h);
     double worst = 0.0;
     for (int col = 0; col < nrhs_; ++col) {
       const double* hc = &h[4 * col];
       const double e = hc[0] / (hc[1] * hc[2] + hc[3]);
-      worst = (e == e) ? std::max(worst, e) : e;
-      if (!(worst == worst)) break;
+      if (!(e == e)) return e;
+      worst = std::max(worst, e);
     }
-    refine_hist_.push_back(worst);
-    eta_ = worst;
-    if (worst <= tol || it >= max_iter || !(worst == worst)) break;
-    // correction: d = M^-1 r, right-hand side r on the factor rows of the
-    // owning fronts (the schur rows of the refinement buffer stay zero)
-    check_k(uhmk_front_global_all(F, nf, GO, d_rf_.as<double>(), d_r_.as<double>(), nrhs_, N, 1, stream_),
-            "scatter r");
-    run_solve(chol, d_rf_.as<double>());
-    check_k(uhmk_front_global_all(F, nf, GO, d_x_.as<double>(), d_xg_.as<double>(), nrhs_, N, 2, stream_), "x += d");
+    return worst;
+  };
+  double eta = worst_eta();
+  refine_hist_.push_back(eta);
+  bool krylov = refine_kind_ == 1;
+  int it = 0;
+  while (!(eta <= tol) && it < max_iter && eta == eta) {
+    if (krylov) {
+      gmres_correction(chol);
+    } else {
+      // d = M^-1 r (front layout, in d_x_), x += d
+      apply_factors(chol, d_r_.as<double>(), nullptr);
+      check_k(uhmk_front_global_all(F, nf, GO, d_x_.as<double>(), d_xg_.as<double>(), nrhs_, N, 2, stream_),
+              "x += d");
+    }
+    ++it;
+    const double e2 = worst_eta();
+    refine_hist_.push_back(e2);
+    if (refine_kind_ == 2 && !krylov && !(e2 <= 0.1 * eta)) krylov = true;
+    eta = e2;
   }
+  eta_ = eta;
   // the refined solution back into the fronts (factor rows, then the
   // branch pass for the schur rows) so that copy_out sees it
   check_k(uhmk_front_global_all(F, nf, GO, d_x_.as<double>(), d_xg_.as<double>(), nrhs_, N, 1, stream_),

@@ -138,6 +138,13 @@ class FrontSolver {
   // backward error after each step is in refine_history().
   int refine(int method, double tol, int max_iter);
   const std::vector<double>& refine_history() const { return refine_hist_; }
+  // Correction kind: 0 classical (d = M^-1 r), 1 GMRES-IR (d from GMRES(m)
+  // on M^-1 A d = M^-1 r, stopped at inner_tol relative to ||M^-1 r||),
+  // 2 classical while each step cuts the backward error at least tenfold,
+  // GMRES-IR from the first step that does not.
+  void set_refinement(int kind, int restart, double inner_tol);
+  int refine_kind() const { return refine_kind_; }
+  int refine_solves() const { return refine_solves_; }  // M^-1 applications of the last refinement
   double last_refine_ms() const { return t_refine_ms_; }
 
   // tuning / timing
@@ -185,6 +192,9 @@ class FrontSolver {
   void layout_arenas();
   void* arena_ptr(int prec) const { return prec ? d_arena32_.p : d_arena_.p; }
   void residual_all(std::vector<double>& h);  // 4 per column: max|r|, max row sum |A|, max|x|, max|b|
+  void apply_matrix(const double* v, double* out);             // out = A v (global vectors)
+  void apply_factors(bool chol, const double* r, double* out);  // M^-1 r into d_x_ (fronts) and out (global, if any)
+  void gmres_correction(bool chol);                             // x += GMRES(m) correction from r
   void run_solve_steps(const std::vector<SolveStep>& steps, bool chol, const double* rhs);
   void run_solve(bool chol, const double* rhs);
   void enqueue_factor(int method, std::vector<hipEvent_t>* pev);
@@ -254,6 +264,10 @@ class FrontSolver {
   // leaf-product residual: compact leaf layout, its items and buffers
   DevBuf d_lg_, d_lpos_, d_lgi_, d_lmi_, d_xl_, d_yl_, d_amax_;
   int n_lgi_ = 0, n_lmi_ = 0;
+  // GMRES-IR
+  int refine_kind_ = 0, gmres_m_ = 20, refine_solves_ = 0;
+  double gmres_tol_ = 1e-6;
+  DevBuf d_V_, d_dpart_, d_dout_, d_coef_;
   double amax_ = 0.0;  // max row sum |A| of the current values
 
   double residual_1norm_ = 0.0, eta_ = 0.0;
