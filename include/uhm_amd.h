@@ -144,8 +144,9 @@ void uhm_get_timing_(uhm_fort_p* mesh, uhm_fort_double* factor_ms, uhm_fort_doub
 /* Per-launch device timing of the next factorizations (on = 1) and the
  * totals of the last profiled one, per kernel kind in the order assemble,
  * lu_panel, lu_rows, chol_diag, chol_trsm, gemm (panel update), schur
- * (final Schur complement): milliseconds, algorithmic flops, algorithmic
- * bytes and launch counts (arrays of 7). */
+ * (final Schur complement), lu_front (whole-front LU of fronts with
+ * n <= 64): milliseconds, algorithmic flops, algorithmic bytes and launch
+ * counts (arrays of 8). */
 void uhm_set_profile_(uhm_fort_p* mesh, uhm_fort_int* on);
 void uhm_get_kernel_stats_(uhm_fort_p* mesh, uhm_fort_double* ms, uhm_fort_double* flops, uhm_fort_double* bytes,
                            uhm_fort_int* launches);
@@ -183,7 +184,9 @@ void uhm_get_refine_solves_(uhm_fort_p* mesh, uhm_fort_int* n);
  * 8 on the FP64 path, 4 on the mixed path), key 2 = mean tile extent from
  * which a GEMM launch uses 128 x 128 tiles (default 1024), key 3 = number
  * of right-hand sides from which the solves run as block solves, FP64 MFMA
- * products over 64-column tiles (default 8). */
+ * products over 64-column tiles (default 8), key 4 = largest LU front
+ * (n <= 64) factored by the one-launch whole-front kernel (default 64; 0:
+ * every front through the panel path). */
 void uhm_set_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value);
 void uhm_get_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value);
 /* Autotuner: times factorizations of the current values over the knobs

@@ -84,6 +84,15 @@ int uhmk_lu_panel(const uhmk_front* fronts, const int* flist, int n_fronts, int 
 int uhmk_lu_rows(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, void* arena,
                  int prec, const int* pperm, hipStream_t stream);
 
+/* Whole-front LU of fronts with n <= 64, one wavefront per front, rows in
+ * registers: pivots searched in ATL (relative format as uhmk_lu_panel),
+ * L\U in ATL, U12 in ATR, L21 in ABL and the Schur complement in ABR in
+ * one launch, replacing the panel / row / GEMM launches of lu_piv_hier
+ * (UHM/src/matrix/uhm/fla/lu_piv/decompose.cxx:124-153) for such fronts.
+ * info[f] = first zero-pivot row + 1. */
+int uhmk_lu_front64(const uhmk_front* fronts, const int* flist, int n_fronts, void* arena, int prec, int* piv,
+                    int* info, hipStream_t stream);
+
 /* Cholesky of the diagonal panel block (FLA_Chol in linal::dense::chol,
  * LINAL/src/dense/chol.cxx:67) and its inverse W = L11^-1, whose strict
  * lower part is stored transposed in the strict upper triangle of the block

@@ -307,3 +307,42 @@ def test_schedule_knobs_preserve_results(lib):
     m.solve_chol()
     m.check_chol()
     assert m.backward_error() <= ETA_TOL
+
+
+@pytest.mark.parametrize("dims,p,prec", [((4, 4, 4), 3, U.PREC_FP64), ((5, 4, 3), 2, U.PREC_FP64),
+                                         ((4, 4, 4), 3, U.PREC_MIXED)])
+def test_lu_whole_front_kernel_matches_panel_path(lib, dims, p, prec):
+    """Fronts with n <= 64 factored by the whole-front kernel give the
+    panel path's pivots exactly and its factors to rounding (the Schur
+    update's summation order differs: rank-1 FMAs against MFMA tiles)."""
+    m = U.Mesh()
+    m.build_synthetic(2, dims, p, 0)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    m.create_matrix_buffer()
+    m.set_random_params(seed=13, alpha=64.0, beta=0.1, scale_kind=1, p0=1.0)
+    m.random_matrix()
+    m.set_rhs()
+    m.set_precision(prec, 1)
+    m.set_tuning(4, 0)
+    assert m.lu_piv_with_free() == 0
+    F0 = [m.fetch_front(f) for f in range(m.n_fronts())]
+    P0 = [m.fetch_pivots(f) for f in range(m.n_fronts())]
+    m.set_tuning(4, 64)
+    assert m.lu_piv_with_free() == 0
+    small = 0
+    for f in range(m.n_fronts()):
+        info = m.front_info(f)
+        if info["fs"] and info["fs"] + info["ss"] <= 64:
+            small += 1
+        assert np.array_equal(P0[f], m.fetch_pivots(f)), f
+        G = m.fetch_front(f)
+        if G.size:
+            err = np.abs(G - F0[f]).max() / max(1e-300, np.abs(F0[f]).max())
+            assert err <= (1e-12 if prec == U.PREC_FP64 else 1e-5), (f, err)
+    say("  fronts through the whole-front kernel:", small, "of", m.n_fronts())
+    assert small > 0
+    m.solve_lu_piv()
+    m.check_lu_piv()
+    if prec == U.PREC_FP64:
+        assert m.backward_error() <= ETA_TOL

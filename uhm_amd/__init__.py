@@ -91,7 +91,7 @@ _SIGS = {
     "uhm_get_refine_history_": [_P, _D, _I, _D],
 }
 
-KERNEL_KINDS = ("assemble", "lu_panel", "lu_rows", "chol_diag", "chol_trsm", "gemm", "schur")
+KERNEL_KINDS = ("assemble", "lu_panel", "lu_rows", "chol_diag", "chol_trsm", "gemm", "schur", "lu_front")
 _RET = {"uhm_get_last_error_": C.c_int}
 
 
@@ -357,8 +357,8 @@ class Mesh:
     def kernel_stats(self):
         """{kind: (ms, algorithmic flops, algorithmic bytes, launches)} of the
         last profiled factorization."""
-        ms, fl, by = (np.zeros(7) for _ in range(3))
-        ln = np.zeros(7, np.int32)
+        ms, fl, by = (np.zeros(len(KERNEL_KINDS)) for _ in range(3))
+        ln = np.zeros(len(KERNEL_KINDS), np.int32)
         _call("uhm_get_kernel_stats_", C.byref(self.handle), _dptr(ms), _dptr(fl), _dptr(by), _iptr(ln))
         return {k: (float(ms[i]), float(fl[i]), float(by[i]), int(ln[i])) for i, k in enumerate(KERNEL_KINDS)}
 

@@ -123,6 +123,56 @@ constexpr int PIB = 8;
 constexpr int PP_STRIDE = 1 + 4 * TILE;  // ints per front in the panel-permutation buffer
 constexpr int PP_MAXROWS = 6144;
 
+// Net row permutation of a finished panel for the row kernel: rows whose
+// content changed (the kb strip rows in order, then the distinct swap
+// targets below the strip in order of first use) and, per listed row, the
+// list index of the row whose old value lands there.  s_piv: the panel's
+// relative pivots; m = panel rows (fs - c0).
+template <int PNT>
+__device__ __forceinline__ void panel_permutation(const int* s_piv, int kb, int m, int c0, int* __restrict__ pp,
+                                                  int tid) {
+  __shared__ int s_cont[PP_MAXROWS];
+  __shared__ int s_tgt[TILE];
+  __shared__ int s_ncnt;
+  for (int i = tid; i < m; i += PNT) s_cont[i] = i;
+  __syncthreads();
+  if (tid == 0)
+    for (int k = 0; k < kb; ++k) {
+      const int d = s_piv[k];
+      if (d) {
+        const int t = s_cont[k];
+        s_cont[k] = s_cont[k + d];
+        s_cont[k + d] = t;
+      }
+    }
+  if (tid < 64) {
+    const int k = tid;
+    int t = -1;
+    bool first = false;
+    if (k < kb) {
+      t = k + s_piv[k];
+      first = t >= kb;
+      for (int k2 = 0; k2 < k && first; ++k2)
+        if (k2 + s_piv[k2] == t) first = false;
+    }
+    const unsigned long long mask = __ballot(first);
+    if (first) s_tgt[__popcll(mask & ((1ull << k) - 1ull))] = t;
+    if (k == 0) s_ncnt = kb + __popcll(mask);
+  }
+  __syncthreads();
+  const int cnt = s_ncnt;
+  for (int k = tid; k < cnt; k += PNT) {
+    const int lr = (k < kb) ? k : s_tgt[k - kb];
+    const int src = s_cont[lr];
+    int si = (src < kb) ? src : -1;
+    for (int u = 0; u < cnt - kb && si < 0; ++u)
+      if (s_tgt[u] == src) si = kb + u;
+    pp[1 + k] = c0 + lr;
+    pp[1 + 2 * TILE + k] = si;
+  }
+  if (tid == 0) pp[0] = cnt;
+}
+
 template <int R, typename T, int PNT>
 __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__ F,
                                                   const int* __restrict__ flist, int c0, int nb,
@@ -375,52 +425,7 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__
   }
   for (int k = tid; k < kb; k += PNT) piv[fr.poff + c0 + k] = s_piv[k];
   if (tid == 0 && first_zero && info[f] == 0) info[f] = first_zero;
-  // Net row permutation of the whole panel for the row kernel: rows whose
-  // content changed (the kb strip rows in order, then the distinct swap
-  // targets below the strip in order of first use) and, per listed row,
-  // the list index of the row whose old value lands there.
-  __shared__ int s_cont[PP_MAXROWS];
-  __shared__ int s_tgt[TILE];
-  __shared__ int s_ncnt;
-  const int m = fs - c0;
-  for (int i = tid; i < m; i += PNT) s_cont[i] = i;
-  __syncthreads();
-  if (tid == 0)
-    for (int k = 0; k < kb; ++k) {
-      const int d = s_piv[k];
-      if (d) {
-        const int t = s_cont[k];
-        s_cont[k] = s_cont[k + d];
-        s_cont[k + d] = t;
-      }
-    }
-  if (tid < 64) {
-    const int k = tid;
-    int t = -1;
-    bool first = false;
-    if (k < kb) {
-      t = k + s_piv[k];
-      first = t >= kb;
-      for (int k2 = 0; k2 < k && first; ++k2)
-        if (k2 + s_piv[k2] == t) first = false;
-    }
-    const unsigned long long mask = __ballot(first);
-    if (first) s_tgt[__popcll(mask & ((1ull << k) - 1ull))] = t;
-    if (k == 0) s_ncnt = kb + __popcll(mask);
-  }
-  __syncthreads();
-  const int cnt = s_ncnt;
-  int* __restrict__ pp = pperm + (long long)f * PP_STRIDE;
-  for (int k = tid; k < cnt; k += PNT) {
-    const int lr = (k < kb) ? k : s_tgt[k - kb];
-    const int src = s_cont[lr];
-    int si = (src < kb) ? src : -1;
-    for (int u = 0; u < cnt - kb && si < 0; ++u)
-      if (s_tgt[u] == src) si = kb + u;
-    pp[1 + k] = c0 + lr;
-    pp[1 + 2 * TILE + k] = si;
-  }
-  if (tid == 0) pp[0] = cnt;
+  panel_permutation<PNT>(s_piv, kb, fs - c0, c0, pperm + (long long)f * PP_STRIDE, tid);
 }
 
 // ------------------------------------------------------------------------
@@ -528,12 +533,147 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
 
 inline int ret(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
 
+// ------------------------------------------------------------------------
+// The same panel with the whole panel in registers, for panels of at most
+// PNT <= 512 pivot-candidate rows: thread t holds panel row c0 + t (all kb
+// columns), so no panel data travels through memory between the column
+// steps.  Rows are not moved: each thread tracks its row's current position
+// and the rows are stored at their final positions at the end.  Per column
+// two barriers: one publishes the wavefronts' pivot candidates, one the
+// pivot row (written to LDS by its owner, double-buffered by column
+// parity).  The columns run in groups of 8 (group index at run time,
+// column inside the group and every register index at compile time).
+// Pivot rule and arithmetic are those of k_lu_panel (first maximal |a| in
+// current order, IEEE division, rank-1 FMA updates in pivot order), so
+// both kernels give the same bits.
+// ------------------------------------------------------------------------
+template <typename T, int PNT>
+__global__ __launch_bounds__(PNT) void k_lu_panel_reg(const uhmk_front* __restrict__ F,
+                                                      const int* __restrict__ flist, int c0, int nb,
+                                                      T* __restrict__ arena, int* __restrict__ piv,
+                                                      int* __restrict__ info, int* __restrict__ pperm) {
+  constexpr int NW = PNT / 64;
+  __shared__ double s_cv[2][NW];
+  __shared__ int s_ci[2][NW];
+  __shared__ double s_row[2][TILE];
+  __shared__ int s_piv[TILE];
+  const int f = flist[blockIdx.x];
+  const uhmk_front fr = F[f];
+  const int n = fr.n, fs = fr.fs;
+  const int kb = min(nb, fs - c0), m = fs - c0;
+  T* __restrict__ A = arena + fr.off + c0 + (long long)c0 * n;  // panel origin (c0, c0)
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const bool live = tid < m;
+  double a[TILE];
+#pragma unroll
+  for (int j = 0; j < TILE; ++j) a[j] = uhmd::ldz(A, tid + (long long)j * n, live && j < kb);
+  int pos = tid, zero = 0;
+  for (int g = 0; 8 * g < kb; ++g) {
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+      const int K = 8 * g + kk;
+      if (K >= kb) break;
+      const int buf = kk & 1;
+      double ak = 0.0;
+#pragma unroll
+      for (int h = 0; h < 8; ++h)
+        if (h == g) ak = a[8 * h + kk];
+      // first maximal |a(., K)| over the rows at positions [K, m)
+      double cv = -1.0;
+      int ck = 0x7fffffff;
+      if (live && pos >= K) {
+        const double v = fabs(ak);
+        if (v > cv) {
+          cv = v;
+          ck = pos;
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(cv, o);
+        const int ok = __shfl_xor(ck, o);
+        if (ov > cv || (ov == cv && ok < ck)) {
+          cv = ov;
+          ck = ok;
+        }
+      }
+      if (lane == 0) {
+        s_cv[buf][w] = cv;
+        s_ci[buf][w] = ck;
+      }
+      __syncthreads();
+      double bv = s_cv[buf][0];
+      int bk = s_ci[buf][0];
+#pragma unroll
+      for (int q = 1; q < NW; ++q) {
+        const double ov = s_cv[buf][q];
+        const int ok = s_ci[buf][q];
+        if (ov > bv || (ov == bv && ok < bk)) {
+          bv = ov;
+          bk = ok;
+        }
+      }
+      // a column of NaNs has no maximum: keep the current row
+      const int pp = (bk < K || bk >= m) ? K : bk;
+      if (live && pos == pp) {
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+          if (h < g) continue;
+#pragma unroll
+          for (int jj = 0; jj < 8; ++jj)
+            if (8 * h + jj < kb) s_row[buf][8 * h + jj] = a[8 * h + jj];
+        }
+      }
+      if (live) {
+        if (pos == pp) pos = K;
+        else if (pos == K) pos = pp;
+      }
+      if (tid == 0) s_piv[K] = pp - K;
+      __syncthreads();
+      const double pv = s_row[buf][K];
+      if (tid == 0 && !(pv != 0.0) && zero == 0) zero = K + 1;
+      if (live && pos > K) {
+        double l = ak;
+        if (pv != 0.0) l = l / pv;
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+          if (h < g) continue;
+          if (h == g) a[8 * h + kk] = l;
+#pragma unroll
+          for (int jj = 0; jj < 8; ++jj) {
+            const int j = 8 * h + jj;
+            if ((h > g || jj > kk) && j < kb) a[j] = fma(-l, s_row[buf][j], a[j]);
+          }
+        }
+      }
+    }
+  }
+  if (live)
+#pragma unroll
+    for (int j = 0; j < TILE; ++j)
+      if (j < kb) A[pos + (long long)j * n] = (T)a[j];
+  __syncthreads();
+  for (int k = tid; k < kb; k += PNT) piv[fr.poff + c0 + k] = s_piv[k];
+  if (tid == 0 && zero && info[f] == 0) info[f] = c0 + zero;
+  panel_permutation<PNT>(s_piv, kb, m, c0, pperm + (long long)f * PP_STRIDE, tid);
+}
+
 template <typename T>
 int launch_panel(const uhmk_front* fronts, const int* flist, int n_fronts, int max_rows, int c0, int nb, T* arena,
                  int* piv, int* info, int* pperm, hipStream_t stream) {
   dim3 g(n_fronts), b(512);
-  if (max_rows <= 512)
-    hipLaunchKernelGGL((k_lu_panel<1, T, 512>), g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info, pperm);
+  if (max_rows <= 64)
+    hipLaunchKernelGGL((k_lu_panel_reg<T, 64>), g, dim3(64), 0, stream, fronts, flist, c0, nb, arena, piv, info,
+                       pperm);
+  else if (max_rows <= 128)
+    hipLaunchKernelGGL((k_lu_panel_reg<T, 128>), g, dim3(128), 0, stream, fronts, flist, c0, nb, arena, piv, info,
+                       pperm);
+  else if (max_rows <= 256)
+    hipLaunchKernelGGL((k_lu_panel_reg<T, 256>), g, dim3(256), 0, stream, fronts, flist, c0, nb, arena, piv, info,
+                       pperm);
+  else if (max_rows <= 512)
+    hipLaunchKernelGGL((k_lu_panel_reg<T, 512>), g, dim3(512), 0, stream, fronts, flist, c0, nb, arena, piv, info,
+                       pperm);
   else if (max_rows <= 1024)
     hipLaunchKernelGGL((k_lu_panel<2, T, 512>), g, b, 0, stream, fronts, flist, c0, nb, arena, piv, info, pperm);
   else if (max_rows <= 2048)
@@ -547,6 +687,117 @@ int launch_panel(const uhmk_front* fronts, const int* flist, int n_fronts, int m
   else
     return -1001;  // more than 6144 pivot-candidate rows
   return ret(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------
+// Whole-front LU for fronts with n <= 64 (the leaves and other small
+// fronts): one wavefront per front, lane i holds row i of the assembled
+// front (all n columns) in registers, and the fs pivot steps run
+// right-looking over the whole front, so that one launch produces L\U in
+// ATL, U12 in ATR, L21 in ABL and the Schur complement in ABR
+// (lu_piv_hier, UHM/src/matrix/uhm/fla/lu_piv/decompose.cxx:124-153, for a
+// front that fits one panel).  Rows are never moved: every lane tracks its
+// row's current position (the swaps k <-> p of the pivot sequence act on
+// positions) and stores the row at its final position at the end.  The
+// pivot rule and arithmetic are those of k_lu_panel / k_lu_rows: pivots
+// searched in the ATL rows only, first maximal |a| in current order, ATL
+// multipliers by IEEE division, ABL multipliers by the reciprocal of the
+// pivot, every update a rank-1 FMA in pivot order.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double lane_d(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_lu_front64(const uhmk_front* __restrict__ F,
+                                                    const int* __restrict__ flist, int n_fronts,
+                                                    T* __restrict__ arena, int* __restrict__ piv,
+                                                    int* __restrict__ info) {
+  const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (q >= n_fronts) return;
+  const int f = flist[q];
+  const uhmk_front fr = F[f];
+  const int n = fr.n, fs = fr.fs;
+  const int lane = threadIdx.x & 63;
+  T* __restrict__ A = arena + fr.off;
+  double a[TILE];
+#pragma unroll
+  for (int j = 0; j < TILE; ++j) a[j] = uhmd::ldz(A, lane + (long long)j * n, lane < n && j < n);
+  int pos = lane;
+  int zero = 0;
+  const bool abl = lane >= fs;
+  int* __restrict__ p = piv + fr.poff;
+  // columns in groups of 8: group index at run time, column inside the
+  // group and every register index at compile time
+  for (int g = 0; 8 * g < fs; ++g) {
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+      const int K = 8 * g + kk;
+      if (K >= fs) break;
+      double ak = 0.0;
+#pragma unroll
+      for (int h = 0; h < 8; ++h)
+        if (h == g) ak = a[8 * h + kk];
+      // first maximal |a(., K)| over the rows at positions [K, fs)
+      double cv = -1.0;
+      int ck = 0x7fffffff;
+      if (pos >= K && pos < fs) {
+        const double v = fabs(ak);
+        if (v > cv) {
+          cv = v;
+          ck = pos;
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(cv, o);
+        const int ok = __shfl_xor(ck, o);
+        if (ov > cv || (ov == cv && ok < ck)) {
+          cv = ov;
+          ck = ok;
+        }
+      }
+      // a column of NaNs has no maximum: keep the current row
+      const int pp = (ck < K || ck >= fs) ? K : ck;
+      const int P = __ffsll((long long)__ballot(pos == pp)) - 1;
+      const int C = __ffsll((long long)__ballot(pos == K)) - 1;
+      const double pv = lane_d(ak, P);
+      if (lane == 0) {
+        p[K] = pp - K;
+        if (!(pv != 0.0) && zero == 0) zero = K + 1;
+      }
+      if (P != C) {
+        if (lane == P) pos = K;
+        else if (lane == C) pos = pp;
+      }
+      const bool upd = pos > K;
+      double l = 0.0;
+      if (upd) {
+        l = ak;
+        if (abl) l = l * (1.0 / pv);
+        else if (pv != 0.0) l = l / pv;
+      }
+#pragma unroll
+      for (int h = 0; h < 8; ++h) {
+        if (h < g) continue;
+        if (h == g && upd) a[8 * h + kk] = l;
+#pragma unroll
+        for (int jj = 0; jj < 8; ++jj) {
+          const int j = 8 * h + jj;
+          if ((h > g || jj > kk) && j < n) {
+            const double u = lane_d(a[j], P);
+            if (upd) a[j] = fma(-l, u, a[j]);
+          }
+        }
+      }
+    }
+  }
+  if (lane < n)
+#pragma unroll
+    for (int j = 0; j < TILE; ++j)
+      if (j < n) A[pos + (long long)j * n] = (T)a[j];
+  if (lane == 0 && zero && info[f] == 0) info[f] = zero;
 }
 
 }  // namespace
@@ -579,6 +830,21 @@ int uhmk_lu_panel(const uhmk_front* fronts, const int* flist, int n_fronts, int 
     return launch_panel<float>(fronts, flist, n_fronts, max_rows, c0, nb, static_cast<float*>(arena), piv, info,
                                pperm, stream);
   return -1006;
+}
+
+int uhmk_lu_front64(const uhmk_front* fronts, const int* flist, int n_fronts, void* arena, int prec, int* piv,
+                    int* info, hipStream_t stream) {
+  if (n_fronts <= 0) return 0;
+  const dim3 g((n_fronts + 3) / 4), b(256);
+  if (prec == 0)
+    hipLaunchKernelGGL(k_lu_front64<double>, g, b, 0, stream, fronts, flist, n_fronts, static_cast<double*>(arena),
+                       piv, info);
+  else if (prec == 1)
+    hipLaunchKernelGGL(k_lu_front64<float>, g, b, 0, stream, fronts, flist, n_fronts, static_cast<float*>(arena),
+                       piv, info);
+  else
+    return -1006;
+  return ret(hipGetLastError());
 }
 
 int uhmk_lu_rows(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, void* arena,

@@ -393,8 +393,27 @@ void FrontSolver::build_schedule(int method) {
     }
     la.count = (int)(items_.size() - la.off);
     if (la.count) launches_.push_back(la);
+    // LU fronts that fit one wavefront factor in one whole-front launch;
+    // the panel / row / GEMM launches below cover the others
+    std::vector<int> big;
+    {
+      std::vector<int> small;
+      for (int f : lv) {
+        if (!chol && fr_[f].fs > 0 && fr_[f].n <= std::min(64, front_small_)) small.push_back(f);
+        else big.push_back(f);
+      }
+      if (!small.empty()) {
+        Launch lf;
+        lf.kind = Launch::LU_FRONT;
+        lf.level = L;
+        lf.off = (long long)flists_.size();
+        lf.count = (int)small.size();
+        flists_.insert(flists_.end(), small.begin(), small.end());
+        launches_.push_back(lf);
+      }
+    }
     int maxfs = 0;
-    for (int f : lv) maxfs = std::max(maxfs, fr_[f].fs);
+    for (int f : big) maxfs = std::max(maxfs, fr_[f].fs);
     // Cholesky with 64-wide panels: groups of G panels; each panel updates
     // only the rest of its group (64-wide tiles that stop at the group end),
     // and the trailing matrix receives one update with K = 64 G per group,
@@ -406,7 +425,7 @@ void FrontSolver::build_schedule(int method) {
       const bool group_last = (c0 + nb >= g0 + G * nb) || (c0 + nb >= maxfs);
       std::vector<int> act;
       int maxrows = 0;
-      for (int f : lv)
+      for (int f : big)
         if (fr_[f].fs > c0) {
           act.push_back(f);
           maxrows = std::max(maxrows, fr_[f].fs - c0);
@@ -477,7 +496,7 @@ void FrontSolver::build_schedule(int method) {
         {
           double sr = 0, sc = 0;
           int cnt = 0;
-          for (int f : lv) {
+          for (int f : big) {
             const int ge = std::min(g0 + G * nb, fr_[f].fs);
             if (fr_[f].fs <= g0 || ge >= fr_[f].fs) continue;
             sr += fr_[f].n - ge;
@@ -488,7 +507,7 @@ void FrontSolver::build_schedule(int method) {
         }
         const int T = lt.tile;
         lt.off = (long long)items_.size();
-        for (int f : lv) {
+        for (int f : big) {
           const int fs = fr_[f].fs, n = fr_[f].n;
           if (fs <= g0) continue;
           const int ge = std::min(g0 + G * nb, fs);
@@ -544,7 +563,7 @@ void FrontSolver::build_schedule(int method) {
     {
       double sss = 0;
       int cnt = 0;
-      for (int f : lv)
+      for (int f : big)
         if (fr_[f].fs && fr_[f].n > fr_[f].fs) {
           sss += fr_[f].n - fr_[f].fs;
           ++cnt;
@@ -553,7 +572,7 @@ void FrontSolver::build_schedule(int method) {
     }
     const int T = ls.tile;
     ls.off = (long long)items_.size();
-    for (int f : lv) {
+    for (int f : big) {
       const int fs = fr_[f].fs, n = fr_[f].n;
       if (!fs || n == fs) continue;
       for (int j0 = fs; j0 < n; j0 += T)
@@ -616,6 +635,14 @@ void FrontSolver::build_schedule(int method) {
           }
         }
         break;
+      case Launch::LU_FRONT:
+        for (int q = 0; q < l.count; ++q) {
+          const int f = flists_[l.off + q];
+          const double fs = fr_[f].fs, ss = fr_[f].n - fr_[f].fs, n = fr_[f].n;
+          fl += flop_lu(fs, fs) + 2.0 * fs * fs * ss + 2.0 * fs * ss * ss;
+          by += 16.0 * n * n;
+        }
+        break;
       case Launch::GEMM:
       case Launch::SCHUR:
         for (int q = 0; q < l.count; ++q) {
@@ -670,6 +697,9 @@ void FrontSolver::enqueue_factor(int method, std::vector<hipEvent_t>* pev) {
         break;
       case Launch::LU_ROWS:
         rc = uhmk_lu_rows(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, d_pperm_.as<int>(), stream_);
+        break;
+      case Launch::LU_FRONT:
+        rc = uhmk_lu_front64(F, FL + l.off, l.count, A, l.prec, P, info, stream_);
         break;
       case Launch::CHOL_DIAG:
         rc = uhmk_chol_diag(F, FL + l.off, l.count, l.c0, l.nb, A, l.prec, info, stream_);
@@ -734,9 +764,13 @@ void FrontSolver::set_tuning(int key, int value) {
   } else if (key == 3) {
     if (value < 1) throw std::runtime_error("set_tuning: block-solve threshold must be >= 1");
     block_rhs_ = value;
+  } else if (key == 4) {
+    if (value < 0 || value > 64) throw std::runtime_error("set_tuning: whole-front LU limit must be 0..64");
+    if (value != front_small_) schedule_method_ = 0;
+    front_small_ = value;
   } else {
-    throw std::runtime_error(
-        "set_tuning: unknown key (1: Cholesky panel group, 2: 128-tile threshold, 3: block-solve threshold)");
+    throw std::runtime_error("set_tuning: unknown key (1: Cholesky panel group, 2: 128-tile threshold, "
+                             "3: block-solve threshold, 4: whole-front LU limit)");
   }
 }
 
@@ -744,6 +778,7 @@ int FrontSolver::get_tuning(int key) const {
   if (key == 1) return chol_group();
   if (key == 2) return tile_threshold_;
   if (key == 3) return block_rhs_;
+  if (key == 4) return front_small_;
   throw std::runtime_error("get_tuning: unknown key");
 }
 

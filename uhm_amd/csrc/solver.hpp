@@ -43,7 +43,7 @@ struct DevBuf {
 
 // One launch of the level schedule.
 struct Launch {
-  enum Kind { ASSEMBLE, LU_PANEL, LU_ROWS, CHOL_DIAG, CHOL_TRSM, GEMM, SCHUR } kind;
+  enum Kind { ASSEMBLE, LU_PANEL, LU_ROWS, CHOL_DIAG, CHOL_TRSM, GEMM, SCHUR, LU_FRONT } kind;
   int c0 = 0, nb = 0;
   long long off = 0;  // offset into the item (or front-list) array
   int count = 0;
@@ -55,7 +55,7 @@ struct Launch {
   double flops = 0;   // algorithmic flops of the launch
   double bytes = 0;   // algorithmic HBM bytes of the launch
 };
-constexpr int N_LAUNCH_KINDS = 7;
+constexpr int N_LAUNCH_KINDS = 8;
 
 // One launch of the level solves.
 struct SolveStep {
@@ -155,7 +155,8 @@ class FrontSolver {
   // Schedule knobs: key 1 = Cholesky panels per trailing update, key 2 =
   // mean tile extent from which a GEMM launch uses 128 x 128 tiles, key 3 =
   // number of right-hand sides from which the solves run as block solves
-  // (MFMA products over 64-column tiles of right-hand sides).
+  // (MFMA products over 64-column tiles of right-hand sides), key 4 =
+  // largest LU front (n <= 64) factored by the whole-front kernel (0: none).
   void set_tuning(int key, int value);
   int get_tuning(int key) const;
   // Times factorizations of the current values over the knobs and keeps the
@@ -212,6 +213,7 @@ class FrontSolver {
   int chol_group_ = 0;  // 0: by precision (FP64 8, mixed 4, the autotuner's usual picks)
   int tile_threshold_ = 1024;
   int block_rhs_ = 8;
+  int front_small_ = 64;  // LU fronts with n <= this (at most 64) factor in one whole-front launch
   bool created_ = false;
   hipStream_t stream_ = nullptr;
 
