@@ -218,13 +218,20 @@ def run_reseeds(U, m, w, args):
     return out
 
 
-def gemm_roofline(stats, peak, kernel):
+# HBM bytes per k_gemm launch from rocprofv3 FETCH_SIZE / WRITE_SIZE passes,
+# calibrated against a copy kernel of known size
+# (profiles/k_gemm_chol_q4_16cube_hbm_traffic.md): mixed path of the chol
+# workload only; other paths and workloads report null.
+GEMM_TRAFFIC = {("chol", "mixed"): 180.6e6}
+
+
+def gemm_roofline(stats, peak, kernel, traffic=None):
     g_ms = stats["gemm"][0] + stats["schur"][0]
     g_fl = stats["gemm"][1] + stats["schur"][1]
     g_n = stats["gemm"][3] + stats["schur"][3]
     achieved = g_fl / (g_ms * 1e-3) / 1e12 if g_ms else 0.0
     return {"kernel": kernel, "bound": "mfma", "achieved": round(achieved, 4), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 4), "traffic": None, "launches": g_n,
+            "frac": round(achieved / peak, 4), "traffic": traffic, "launches": g_n,
             "avg_launch_us": round(g_ms / g_n * 1e3, 3) if g_n else None}
 
 
@@ -313,7 +320,8 @@ def main():
         "time_to_1e-12_ms": {"mixed": path(rmx)["time_to_1e-12_ms"], "fp64": path(r64)["time_to_1e-12_ms"],
                              "speedup": round(r64["t_sol"] / rmx["t_sol"], 3) if mixed_ok else None},
         "roofline": gemm_roofline(head["stats"], PEAK_FP32_TFLOPS if mixed_ok else PEAK_FP64_TFLOPS,
-                                  "k_gemm (Schur / trailing update, %s MFMA)" % ("FP32" if mixed_ok else "FP64")),
+                                  "k_gemm (Schur / trailing update, %s MFMA)" % ("FP32" if mixed_ok else "FP64"),
+                                  GEMM_TRAFFIC.get((args.workload, "mixed" if mixed_ok else "fp64"))),
         "roofline_fp64_path": gemm_roofline(r64["stats"], PEAK_FP64_TFLOPS, "k_gemm (FP64 MFMA)"),
         "mixed": path(rmx),
         "fp64": path(r64),

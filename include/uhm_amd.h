@@ -186,7 +186,11 @@ void uhm_get_refine_solves_(uhm_fort_p* mesh, uhm_fort_int* n);
  * of right-hand sides from which the solves run as block solves, FP64 MFMA
  * products over 64-column tiles (default 8), key 4 = largest LU front
  * (n <= 64) factored by the one-launch whole-front kernel (default 64; 0:
- * every front through the panel path). */
+ * every front through the panel path), key 5 = Cholesky panel lookahead:
+ * the next panel's diagonal block and TRSM run on a second stream beside
+ * the current panel's trailing update (default 0: off; the stream
+ * dependencies cost more than the overlap wins on the measured meshes, the
+ * autotuner tries it). */
 void uhm_set_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value);
 void uhm_get_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value);
 /* Autotuner: times factorizations of the current values over the knobs

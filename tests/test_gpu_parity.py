@@ -346,3 +346,38 @@ def test_lu_whole_front_kernel_matches_panel_path(lib, dims, p, prec):
     m.check_lu_piv()
     if prec == U.PREC_FP64:
         assert m.backward_error() <= ETA_TOL
+
+
+@pytest.mark.parametrize("prec", [U.PREC_FP64, U.PREC_MIXED])
+@pytest.mark.parametrize("group", [1, 4])
+def test_cholesky_lookahead_matches_in_order(lib, prec, group):
+    """The lookahead schedule (next panel's diagonal block and TRSM on a
+    second stream) factors to the in-order schedule's values (rounding only:
+    the next panel's column block uses 64-wide tiles)."""
+    m = U.Mesh()
+    m.build_synthetic(2, (8, 8, 7), 3, 0)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    m.create_matrix_buffer()
+    m.set_random_params(seed=17, alpha=64.0, scale_kind=1, p0=0.5)
+    m.random_matrix()
+    m.set_rhs()
+    m.set_precision(prec, 1)
+    m.set_tuning(1, group)
+    m.set_tuning(5, 0)
+    assert m.chol_with_free() == 0
+    ref = [np.tril(m.fetch_front(f)) for f in range(m.n_fronts())]
+    m.set_tuning(5, 1)
+    for rep in range(2):
+        assert m.chol_with_free() == 0
+        worst = 0.0
+        for f in range(m.n_fronts()):
+            G = np.tril(m.fetch_front(f))
+            if G.size:
+                worst = max(worst, np.abs(G - ref[f]).max() / max(1e-300, np.abs(ref[f]).max()))
+        say("  lookahead vs in order, worst front rel diff %.3e" % worst)
+        assert worst <= (1e-12 if prec == U.PREC_FP64 else 1e-5)
+    m.solve_chol()
+    m.check_chol()
+    if prec == U.PREC_FP64:
+        assert m.backward_error() <= ETA_TOL

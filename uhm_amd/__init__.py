@@ -345,7 +345,8 @@ class Mesh:
         returns (best ms, {knob: value})."""
         t = C.c_double(0)
         _call("uhm_autotune_", C.byref(self.handle), _i(method), C.byref(t))
-        return t.value, {"chol_group": self.get_tuning(1), "tile_threshold": self.get_tuning(2)}
+        return t.value, {"chol_group": self.get_tuning(1), "tile_threshold": self.get_tuning(2),
+                         "lookahead": self.get_tuning(5)}
 
     def set_graph(self, on=True):
         """Capture the factorization / solves as HIP graphs and replay them."""

@@ -135,14 +135,17 @@ __global__ __launch_bounds__(256) void k_chol_diag(const uhmk_front* __restrict_
     }
     __builtin_amdgcn_s_barrier();  // wavefront 0's block rows visible to its own column pass below
     if (w == 0 && lane < 16) {
-      // inverse of the diagonal block, column lane
+      // inverse of the diagonal block, column lane; the 16 reciprocals of
+      // the diagonal come first (one division per lane, in parallel), so the
+      // substitution chain holds multiplications only
+      const double rdl = 1.0 / sL[(b0 + lane) * LDW + b0 + lane];
       double wv[16];
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
         double acc = (k == lane) ? 1.0 : 0.0;
 #pragma unroll
         for (int m = 0; m < k; ++m) acc = fma(-sL[(b0 + k) * LDW + b0 + m], wv[m], acc);
-        wv[k] = acc / sL[(b0 + k) * LDW + b0 + k];
+        wv[k] = acc * readlane_d16_rt(rdl, k);
       }
 #pragma unroll
       for (int k = 0; k < 16; ++k) sW[(b0 + k) * LDW + b0 + lane] = wv[k];

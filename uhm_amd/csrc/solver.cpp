@@ -47,6 +47,8 @@ FrontSolver::FrontSolver(Mesh_* m, int n_rhs) : mesh_(m), nrhs_(n_rhs) { build_s
 
 FrontSolver::~FrontSolver() {
   drop_graphs();
+  for (hipEvent_t e : la_ev_) (void)hipEventDestroy(e);
+  if (side_stream_) (void)hipStreamDestroy(side_stream_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -232,6 +234,7 @@ void FrontSolver::create_buffers() {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     throw std::runtime_error("no HIP device: the multifrontal path runs on the GPU only");
   check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  check_hip(hipStreamCreateWithFlags(&side_stream_, hipStreamNonBlocking), "hipStreamCreate");
   dF_.upload(fr_, stream_);
   d_emap_.upload(emap_, stream_);
   d_inv_.upload(inv_, stream_);
@@ -377,6 +380,32 @@ void FrontSolver::build_schedule(int method) {
   const int nb = nb_;
   std::vector<int> R, C;
   auto tile_end = [](int s, int fs, int n, int T) { return std::min(s + T, s < fs ? fs : n); };
+  // Cholesky lookahead: the update of the next panel's column block is
+  // split off each panel's GEMM launch; the next panel's diagonal block and
+  // TRSM then run on the lookahead stream while the main stream does the
+  // rest of the update.  Events order the two streams; every main-stream
+  // launch after a lookahead pair waits for it (pending).
+  int n_events = 0, pending = -1, ev_next = -1;
+  bool split_prev = false;
+  auto push_main = [&](Launch l) {
+    l.stream = 0;
+    if (pending >= 0) {
+      l.wait_ev = pending;
+      pending = -1;
+    }
+    launches_.push_back(l);
+  };
+  // lower-triangle tiles of front f: rows [r0, n), columns [cb, ce), edge T
+  auto lower_tiles = [&](int f, int r0, int cb, int ce, int T) {
+    const int fs = fr_[f].fs, n = fr_[f].n;
+    tile_starts(r0, n, fs, T, R);
+    tile_starts(cb, ce, fs, T, C);
+    for (int j0 : C)
+      for (int i0 : R) {
+        if (tile_end(i0, fs, n, T) - 1 < j0) continue;
+        items_.push_back(uhmk_item{f, i0, j0, 0});
+      }
+  };
   for (int L = 0; L < (int)levels_.size(); ++L) {
     const auto& lv = levels_[L];
     Launch la;
@@ -392,7 +421,7 @@ void FrontSolver::build_schedule(int method) {
         }
     }
     la.count = (int)(items_.size() - la.off);
-    if (la.count) launches_.push_back(la);
+    if (la.count) push_main(la);
     // LU fronts that fit one wavefront factor in one whole-front launch;
     // the panel / row / GEMM launches below cover the others
     std::vector<int> big;
@@ -409,7 +438,7 @@ void FrontSolver::build_schedule(int method) {
         lf.off = (long long)flists_.size();
         lf.count = (int)small.size();
         flists_.insert(flists_.end(), small.begin(), small.end());
-        launches_.push_back(lf);
+        push_main(lf);
       }
     }
     int maxfs = 0;
@@ -439,7 +468,6 @@ void FrontSolver::build_schedule(int method) {
       lp.count = (int)act.size();
       lp.extra = maxrows;
       flists_.insert(flists_.end(), act.begin(), act.end());
-      launches_.push_back(lp);
 
       Launch lr;
       lr.c0 = c0;
@@ -459,7 +487,24 @@ void FrontSolver::build_schedule(int method) {
         }
       }
       lr.count = (int)(items_.size() - lr.off);
-      if (lr.count) launches_.push_back(lr);
+      if (split_prev) {
+        // this panel's diagonal block and TRSM on the lookahead stream, after
+        // the update of its columns (ev_next); the main stream waits for them
+        // before its next launch
+        lp.stream = 1;
+        lp.wait_ev = ev_next;
+        launches_.push_back(lp);
+        if (lr.count) {
+          lr.stream = 1;
+          launches_.push_back(lr);
+        }
+        launches_.back().record_ev = pending = n_events++;
+      } else {
+        push_main(lp);
+        if (lr.count) push_main(lr);
+      }
+      split_prev = false;
+      const bool la = chol && lookahead_ && c0 + nb < maxfs;
 
       if (G > 1) {
         // update of the rest of the group by this panel (K = [c0, c0+nb))
@@ -484,7 +529,27 @@ void FrontSolver::build_schedule(int method) {
             }
         }
         li.count = (int)(items_.size() - li.off);
-        if (li.count) launches_.push_back(li);
+        if (la && !group_last && li.count) {
+          // the next panel's column block (first 64-wide column tile) first
+          std::stable_partition(items_.begin() + li.off, items_.end(), [&](const uhmk_item& it) {
+            return it.b == std::min(c0 + nb, fr_[it.f].fs);
+          });
+          int nn = 0;
+          for (long long q = li.off; q < (long long)items_.size(); ++q)
+            if (items_[q].b == std::min(c0 + nb, fr_[items_[q].f].fs)) ++nn;
+          Launch rest = li;
+          li.count = nn;
+          rest.off = li.off + nn;
+          rest.count -= nn;
+          if (li.count) {
+            li.record_ev = ev_next = n_events++;
+            push_main(li);
+            split_prev = true;
+          }
+          if (rest.count) push_main(rest);
+        } else if (li.count) {
+          push_main(li);
+        }
         if (!group_last) continue;
         // trailing update of the whole group (K = [g0, g0 + G nb))
         Launch lt;
@@ -506,6 +571,37 @@ void FrontSolver::build_schedule(int method) {
           lt.tile = cnt ? gemm_tile_for(sr / cnt, sc / cnt, tile_threshold_) : 64;
         }
         const int T = lt.tile;
+        if (la) {
+          // the next panel's column block [ge, ge + 64) in 64-wide tiles
+          // first, then the rest of the trailing matrix
+          Launch ln = lt;
+          ln.tile = 64;
+          ln.off = (long long)items_.size();
+          for (int f : big) {
+            const int fs = fr_[f].fs;
+            if (fs <= g0) continue;
+            const int ge = std::min(g0 + G * nb, fs);
+            if (ge >= fs) continue;
+            lower_tiles(f, ge, ge, std::min(ge + 64, fs), 64);
+          }
+          ln.count = (int)(items_.size() - ln.off);
+          if (ln.count) {
+            ln.record_ev = ev_next = n_events++;
+            push_main(ln);
+            split_prev = true;
+          }
+          lt.off = (long long)items_.size();
+          for (int f : big) {
+            const int fs = fr_[f].fs;
+            if (fs <= g0) continue;
+            const int ge = std::min(g0 + G * nb, fs);
+            if (ge + 64 >= fs) continue;
+            lower_tiles(f, ge + 64, ge + 64, fs, T);
+          }
+          lt.count = (int)(items_.size() - lt.off);
+          if (lt.count) push_main(lt);
+          continue;
+        }
         lt.off = (long long)items_.size();
         for (int f : big) {
           const int fs = fr_[f].fs, n = fr_[f].n;
@@ -521,7 +617,7 @@ void FrontSolver::build_schedule(int method) {
             }
         }
         lt.count = (int)(items_.size() - lt.off);
-        if (lt.count) launches_.push_back(lt);
+        if (lt.count) push_main(lt);
         continue;
       }
       Launch lg;
@@ -540,6 +636,32 @@ void FrontSolver::build_schedule(int method) {
         lg.tile = gemm_tile_for(sr / act.size(), sc / act.size(), tile_threshold_);
       }
       const int T = lg.tile;
+      if (la) {
+        // (Cholesky) the next panel's column block in 64-wide tiles first
+        Launch ln = lg;
+        ln.tile = 64;
+        ln.off = (long long)items_.size();
+        for (int f : act) {
+          const int fs = fr_[f].fs, c1 = std::min(c0 + nb, fs);
+          if (c1 >= fs) continue;
+          lower_tiles(f, c1, c1, std::min(c1 + 64, fs), 64);
+        }
+        ln.count = (int)(items_.size() - ln.off);
+        if (ln.count) {
+          ln.record_ev = ev_next = n_events++;
+          push_main(ln);
+          split_prev = true;
+        }
+        lg.off = (long long)items_.size();
+        for (int f : act) {
+          const int fs = fr_[f].fs, c1 = std::min(c0 + nb, fs);
+          if (c1 + 64 >= fs) continue;
+          lower_tiles(f, c1 + 64, c1 + 64, fs, T);
+        }
+        lg.count = (int)(items_.size() - lg.off);
+        if (lg.count) push_main(lg);
+        continue;
+      }
       lg.off = (long long)items_.size();
       for (int f : act) {
         const int fs = fr_[f].fs, n = fr_[f].n;
@@ -554,7 +676,7 @@ void FrontSolver::build_schedule(int method) {
           }
       }
       lg.count = (int)(items_.size() - lg.off);
-      if (lg.count) launches_.push_back(lg);
+      if (lg.count) push_main(lg);
     }
     Launch ls;
     ls.kind = Launch::SCHUR;
@@ -582,7 +704,13 @@ void FrontSolver::build_schedule(int method) {
         }
     }
     ls.count = (int)(items_.size() - ls.off);
-    if (ls.count) launches_.push_back(ls);
+    if (ls.count) push_main(ls);
+  }
+  final_wait_ev_ = pending;
+  while ((int)la_ev_.size() < n_events) {
+    hipEvent_t e;
+    check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+    la_ev_.push_back(e);
   }
   for (Launch& l : launches_) {
     l.prec = level_prec_[l.level];
@@ -672,9 +800,10 @@ void FrontSolver::build_schedule(int method) {
 }
 
 void FrontSolver::enqueue_factor(int method, std::vector<hipEvent_t>* pev) {
-  // The level schedule as a fixed launch sequence on stream_ (also the body
-  // that a graph capture records: no memset nodes, the info flags are
-  // cleared by a fill kernel).
+  // The level schedule as a fixed launch sequence on stream_ and, for the
+  // Cholesky lookahead, side_stream_ (ordered by events; also the body that
+  // a graph capture records: the side stream forks from and joins back into
+  // stream_; no memset nodes, the info flags are cleared by a fill kernel).
   const uhmk_front* F = dF_.as<uhmk_front>();
   const uhmk_item* IT = d_items_.as<uhmk_item>();
   const int* FL = d_flists_.as<int>();
@@ -684,39 +813,43 @@ void FrontSolver::enqueue_factor(int method, std::vector<hipEvent_t>* pev) {
   for (size_t li = 0; li < launches_.size(); ++li) {
     const Launch& l = launches_[li];
     int rc = 0;
-    if (pev) check_hip(hipEventRecord((*pev)[2 * li], stream_), "event");
+    hipStream_t st = l.stream ? side_stream_ : stream_;
+    if (l.wait_ev >= 0) check_hip(hipStreamWaitEvent(st, la_ev_[l.wait_ev], 0), "stream wait");
+    if (pev) check_hip(hipEventRecord((*pev)[2 * li], st), "event");
     void* A = arena_ptr(l.prec);
     switch (l.kind) {
       case Launch::ASSEMBLE:
         rc = uhmk_assemble(F, IT + l.off, l.count, d_srcbeg_.as<int>(), d_srcs_.as<uhmk_source>(), d_inv_.as<int>(),
-                           d_arena_.as<double>(), d_arena32_.as<float>(), l.prec, d_stage_.as<double>(), stream_);
+                           d_arena_.as<double>(), d_arena32_.as<float>(), l.prec, d_stage_.as<double>(), st);
         break;
       case Launch::LU_PANEL:
         rc = uhmk_lu_panel(F, FL + l.off, l.count, l.extra, l.c0, l.nb, A, l.prec, P, info, d_pperm_.as<int>(),
-                           stream_);
+                           st);
         break;
       case Launch::LU_ROWS:
-        rc = uhmk_lu_rows(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, d_pperm_.as<int>(), stream_);
+        rc = uhmk_lu_rows(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, d_pperm_.as<int>(), st);
         break;
       case Launch::LU_FRONT:
-        rc = uhmk_lu_front64(F, FL + l.off, l.count, A, l.prec, P, info, stream_);
+        rc = uhmk_lu_front64(F, FL + l.off, l.count, A, l.prec, P, info, st);
         break;
       case Launch::CHOL_DIAG:
-        rc = uhmk_chol_diag(F, FL + l.off, l.count, l.c0, l.nb, A, l.prec, info, stream_);
+        rc = uhmk_chol_diag(F, FL + l.off, l.count, l.c0, l.nb, A, l.prec, info, st);
         break;
       case Launch::CHOL_TRSM:
-        rc = uhmk_chol_trsm(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, stream_);
+        rc = uhmk_chol_trsm(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, st);
         break;
       case Launch::GEMM:
-        rc = uhmk_gemm_update(F, IT + l.off, l.count, l.c0, l.nb, 0, l.extra, l.tile, A, l.prec, l.mma, stream_);
+        rc = uhmk_gemm_update(F, IT + l.off, l.count, l.c0, l.nb, 0, l.extra, l.tile, A, l.prec, l.mma, st);
         break;
       case Launch::SCHUR:
-        rc = uhmk_gemm_update(F, IT + l.off, l.count, 0, 0, 1, l.extra, l.tile, A, l.prec, l.mma, stream_);
+        rc = uhmk_gemm_update(F, IT + l.off, l.count, 0, 0, 1, l.extra, l.tile, A, l.prec, l.mma, st);
         break;
     }
     check_k(rc, "factor");
-    if (pev) check_hip(hipEventRecord((*pev)[2 * li + 1], stream_), "event");
+    if (pev) check_hip(hipEventRecord((*pev)[2 * li + 1], st), "event");
+    if (l.record_ev >= 0) check_hip(hipEventRecord(la_ev_[l.record_ev], st), "event");
   }
+  if (final_wait_ev_ >= 0) check_hip(hipStreamWaitEvent(stream_, la_ev_[final_wait_ev_], 0), "stream wait");
   if (method == UHM_LU_PIV)
     check_k(uhmk_lu_perm(F, d_fsl_.as<int>(), n_fsl_, P, d_perm_.as<int>(), stream_), "lu perm");
 }
@@ -768,9 +901,13 @@ void FrontSolver::set_tuning(int key, int value) {
     if (value < 0 || value > 64) throw std::runtime_error("set_tuning: whole-front LU limit must be 0..64");
     if (value != front_small_) schedule_method_ = 0;
     front_small_ = value;
+  } else if (key == 5) {
+    if (value != 0 && value != 1) throw std::runtime_error("set_tuning: lookahead is 0 or 1");
+    if (value != lookahead_) schedule_method_ = 0;
+    lookahead_ = value;
   } else {
     throw std::runtime_error("set_tuning: unknown key (1: Cholesky panel group, 2: 128-tile threshold, "
-                             "3: block-solve threshold, 4: whole-front LU limit)");
+                             "3: block-solve threshold, 4: whole-front LU limit, 5: Cholesky lookahead)");
   }
 }
 
@@ -779,13 +916,15 @@ int FrontSolver::get_tuning(int key) const {
   if (key == 2) return tile_threshold_;
   if (key == 3) return block_rhs_;
   if (key == 4) return front_small_;
+  if (key == 5) return lookahead_;
   throw std::runtime_error("get_tuning: unknown key");
 }
 
 double FrontSolver::autotune(int method) {
   // Coordinate search over the schedule knobs on this pattern and these
-  // values: the Cholesky panel group (panels per trailing update), then the
-  // mean tile extent from which a launch uses 128 x 128 GEMM tiles.  Each
+  // values: the Cholesky panel group (panels per trailing update), the
+  // mean tile extent from which a launch uses 128 x 128 GEMM tiles, then the
+  // Cholesky panel lookahead.  Each
   // candidate is timed as the better of two factorizations; the best
   // setting stays in place and its time is returned.
   auto time_it = [&]() {
@@ -823,6 +962,13 @@ double FrontSolver::autotune(int method) {
     }
   }
   set_tuning(2, bt);
+  if (method == UHM_CHOL) {
+    const int bl = lookahead_;
+    set_tuning(5, 1 - bl);
+    const double t = time_it();
+    if (t < best) best = t;
+    else set_tuning(5, bl);
+  }
   use_graph_ = save_graph;
   drop_graphs();
   return best;

@@ -54,6 +54,9 @@ struct Launch {
   int mma = 0;        // gemm / schur products: 0 FP64, 1 FP32
   double flops = 0;   // algorithmic flops of the launch
   double bytes = 0;   // algorithmic HBM bytes of the launch
+  int stream = 0;     // 0: main stream, 1: lookahead stream
+  int wait_ev = -1;   // event the launch's stream waits for first (-1: none)
+  int record_ev = -1; // event recorded on the launch's stream after it (-1: none)
 };
 constexpr int N_LAUNCH_KINDS = 8;
 
@@ -156,7 +159,8 @@ class FrontSolver {
   // mean tile extent from which a GEMM launch uses 128 x 128 tiles, key 3 =
   // number of right-hand sides from which the solves run as block solves
   // (MFMA products over 64-column tiles of right-hand sides), key 4 =
-  // largest LU front (n <= 64) factored by the whole-front kernel (0: none).
+  // largest LU front (n <= 64) factored by the whole-front kernel (0: none),
+  // key 5 = Cholesky panel lookahead (1 on, 0 off).
   void set_tuning(int key, int value);
   int get_tuning(int key) const;
   // Times factorizations of the current values over the knobs and keeps the
@@ -214,6 +218,10 @@ class FrontSolver {
   int tile_threshold_ = 1024;
   int block_rhs_ = 8;
   int front_small_ = 64;  // LU fronts with n <= this (at most 64) factor in one whole-front launch
+  int lookahead_ = 0;     // Cholesky: next panel's diagonal block and TRSM on the lookahead stream
+  hipStream_t side_stream_ = nullptr;
+  std::vector<hipEvent_t> la_ev_;
+  int final_wait_ev_ = -1;
   bool created_ = false;
   hipStream_t stream_ = nullptr;
 
