@@ -72,6 +72,7 @@ __global__ __launch_bounds__(256) void k_sv_gather(const uhmk_front* __restrict_
                                                    const uhmk_source* __restrict__ srcs,
                                                    const int* __restrict__ inv, const int* __restrict__ perm,
                                                    double* __restrict__ x, const double* __restrict__ b, int nrhs) {
+  constexpr int MAXS = 8;  // child sources held in registers; more are added in a second pass
   const uhmk_item it = items[blockIdx.x];
   const uhmk_front fr = F[it.f];
   const int n = fr.n, fs = fr.fs;
@@ -79,8 +80,36 @@ __global__ __launch_bounds__(256) void k_sv_gather(const uhmk_front* __restrict_
   if (i >= n) return;
   const int r = (LU && i < fs) ? perm[fr.poff + i] : i;
   double* __restrict__ xo = x + fr.xoff;
-  for (int c = 0; c < nrhs; ++c) xo[i + (long long)c * n] = b[fr.xoff + r + (long long)c * n];
-  for (int s = src_beg[it.f]; s < src_beg[it.f + 1]; ++s) {
+  // b is zero on the schur rows of fronts without an input stage (right-hand
+  // sides enter at the leaves; refinement places its right-hand side on
+  // factor rows only), so those rows skip the read
+  const bool rb = fr.soff >= 0 || r < fs;
+  const double* __restrict__ bo = b + fr.xoff + r;
+  const double* sp[MAXS];
+  long long sld[MAXS];
+  int ns = 0;
+  const int s0 = src_beg[it.f], s1 = src_beg[it.f + 1];
+  int s = s0;
+  for (; s < s1 && ns < MAXS; ++s) {
+    const uhmk_source sc = srcs[s];
+    if (sc.src < 0) continue;
+    const int ri = inv[sc.inv + r];
+    if (ri < 0) continue;
+    const uhmk_front ch = F[sc.src];
+    sp[ns] = x + ch.xoff + ch.fs + ri;
+    sld[ns] = ch.n;
+    ++ns;
+  }
+  // every column written once: b, then the children in merge order
+#pragma unroll 4
+  for (int c = 0; c < nrhs; ++c) {
+    double acc = rb ? bo[(long long)c * n] : 0.0;
+#pragma unroll
+    for (int q = 0; q < MAXS; ++q)
+      if (q < ns) acc += sp[q][(long long)c * sld[q]];
+    xo[i + (long long)c * n] = acc;
+  }
+  for (; s < s1; ++s) {
     const uhmk_source sc = srcs[s];
     if (sc.src < 0) continue;
     const int ri = inv[sc.inv + r];
