@@ -185,7 +185,7 @@ def run_path(U, m, w, mode, top, args):
                 refine_ms=ref_ms, n_solves=n_solves, stats=stats, tuned=tuned)
 
 
-def run_reseeds(U, m, w, args):
+def run_reseeds(U, m, w, args, n):
     """Re-factorization of new element values on the fixed pattern: the
     factorization and solves are graph-captured once and replayed for each
     reseed (device-generated values), each refined to backward error 1e-12."""
@@ -196,7 +196,7 @@ def run_reseeds(U, m, w, args):
     m.set_refinement(REFINE_KINDS[args.refine or w.get("refine", "classical")](U), 20, 1e-6)
     m.set_graph(True)
     out = []
-    for seed in range(1, w["reseeds"] + 1):
+    for seed in range(1, n + 1):
         m.set_random_params(seed=seed, alpha=w["alpha"], beta=w["beta"], scale_kind=w["scale_kind"], p0=w["p0"],
                             p1=w["p1"])
         m.random_matrix()
@@ -246,6 +246,8 @@ def main():
     ap.add_argument("--top", type=int, default=2, help="tree levels kept in FP64 by the mixed path")
     ap.add_argument("--refine", choices=sorted(REFINE_KINDS), default=None,
                     help="refinement kind (default: the workload's; classical for chol / lu)")
+    ap.add_argument("--reseeds", type=int, default=None,
+                    help="graph-replayed re-factorizations after the timed paths (default: the workload's)")
     ap.add_argument("--tune", action="store_true",
                     help="run the autotuner over the schedule knobs before timing (default: built-in knobs)")
     args = ap.parse_args()
@@ -267,7 +269,8 @@ def main():
     top = w.get("top", args.top)
     r64 = run_path(U, m, w, U.PREC_FP64, top, args)
     rmx = run_path(U, m, w, U.PREC_MIXED, top, args)
-    reseeds = run_reseeds(U, m, w, args) if w.get("reseeds") else None
+    n_reseeds = w.get("reseeds", 0) if args.reseeds is None else args.reseeds
+    reseeds = run_reseeds(U, m, w, args, n_reseeds) if n_reseeds else None
     mixed_ok = rmx["eta"] <= 1e-12
     head = rmx if mixed_ok else r64
 
