@@ -303,7 +303,6 @@ __global__ __launch_bounds__(256) void k_chol_trsm(const uhmk_front* __restrict_
 // stores 16 consecutive rows of C.
 // ------------------------------------------------------------------------
 typedef float f4 __attribute__((ext_vector_type(4)));
-constexpr int KC = 16;
 
 template <typename TM>
 struct Mfma;
@@ -332,7 +331,11 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ 
   constexpr int LD = T + 16;          // row stride of the LDS panels (conflict-free half-wave reads)
   constexpr int NB = T / 32;          // 16x16 blocks per wave and dimension
   constexpr int KR = 256 / T;         // k rows per load pass (one thread per i)
+  // K chunk through LDS: 32 for FP32 operands (half the barriers per MFMA),
+  // 16 for FP64 operands
+  constexpr int KC = sizeof(TM) == 4 ? 32 : 16;
   constexpr int NL = KC / KR;         // loads per thread per operand
+  constexpr int BJ = 256 / KC;        // NN B: columns per load pass
   const uhmk_item it = items[blockIdx.x];
   const uhmk_front fr = F[it.f];
   const int n = fr.n, fs = fr.fs;
@@ -356,8 +359,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ 
   TM ra[NL], rb[NL];
   // A (and NT B): thread t loads row i = t % T of k rows t / T + KR u.
   const int li = tid % T, lk = tid / T;
-  // NN B: thread t loads k = t % 16 of columns t / 16 + 16 u.
-  const int bk = tid & 15, bj = tid >> 4;
+  // NN B: thread t loads k = t % KC of columns t / KC + BJ u.
+  const int bk = tid % KC, bj = tid / KC;
   auto load = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < NL; ++u) {
@@ -373,7 +376,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ 
     } else {
 #pragma unroll
       for (int u = 0; u < NL; ++u) {
-        const int j = bj + 16 * u;
+        const int j = bj + BJ * u;
         rb[u] = uhmd::ldz_as<TM>(M, (k0 + bk) + (long long)(j0 + j) * n, j0 + j < j1 && k0 + bk < kend);
       }
     }
@@ -387,7 +390,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ 
       for (int u = 0; u < NL; ++u) sB[(lk + KR * u) * LD + li] = rb[u];
     } else {
 #pragma unroll
-      for (int u = 0; u < NL; ++u) sB[bk * LD + bj + 16 * u] = rb[u];
+      for (int u = 0; u < NL; ++u) sB[bk * LD + bj + BJ * u] = rb[u];
     }
     __syncthreads();
     if (k0 + KC < kend) load(k0 + KC);
