@@ -65,12 +65,14 @@ __global__ __launch_bounds__(256) void k_block_dots_final(const double* __restri
 
 // w(g, c) = beta w(g, c) + alpha sum_{i < k} coef[i * nrhs + c] V_i(g, c)
 // (beta 1: update; beta 0: w is overwritten).
+// grid (row blocks, nrhs): one column per blockIdx.y.
 __global__ __launch_bounds__(256) void k_basis_combine(const double* __restrict__ V, long long vstride, int k,
                                                        const double* __restrict__ coef, double alpha, double beta,
                                                        double* __restrict__ w, long long nrow, int nrhs) {
-  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= nrow * nrhs) return;
-  const int c = (int)(idx / nrow);
+  const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (g >= nrow) return;
+  const int c = blockIdx.y;
+  const long long idx = (long long)c * nrow + g;
   double s = 0.0;
   for (int i = 0; i < k; ++i) s = fma(coef[i * nrhs + c], V[i * vstride + idx], s);
   w[idx] = beta == 0.0 ? alpha * s : fma(alpha, s, beta * w[idx]);
@@ -79,9 +81,9 @@ __global__ __launch_bounds__(256) void k_basis_combine(const double* __restrict_
 // w(:, c) *= s[c]
 __global__ __launch_bounds__(256) void k_scale_columns(double* __restrict__ w, long long nrow, int nrhs,
                                                        const double* __restrict__ s) {
-  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= nrow * nrhs) return;
-  w[idx] *= s[idx / nrow];
+  const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (g >= nrow) return;
+  w[(long long)blockIdx.y * nrow + g] *= s[blockIdx.y];
 }
 
 inline int ret(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
@@ -109,17 +111,18 @@ int uhmk_block_dots(const double* V, long long vstride, int k, const double* w, 
 
 int uhmk_basis_combine(const double* V, long long vstride, int k, const double* coef, double alpha, double beta,
                        double* w, long long nrow, int nrhs, hipStream_t stream) {
-  const long long tot = nrow * nrhs;
-  if (tot <= 0) return 0;
-  hipLaunchKernelGGL(k_basis_combine, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream, V, vstride, k, coef,
-                     alpha, beta, w, nrow, nrhs);
+  if (nrow <= 0 || nrhs <= 0) return 0;
+  if (nrhs > 65535) return -1012;
+  hipLaunchKernelGGL(k_basis_combine, dim3((unsigned)((nrow + 255) / 256), (unsigned)nrhs), dim3(256), 0, stream, V,
+                     vstride, k, coef, alpha, beta, w, nrow, nrhs);
   return ret(hipGetLastError());
 }
 
 int uhmk_scale_columns(double* w, long long nrow, int nrhs, const double* s, hipStream_t stream) {
-  const long long tot = nrow * nrhs;
-  if (tot <= 0) return 0;
-  hipLaunchKernelGGL(k_scale_columns, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, stream, w, nrow, nrhs, s);
+  if (nrow <= 0 || nrhs <= 0) return 0;
+  if (nrhs > 65535) return -1012;
+  hipLaunchKernelGGL(k_scale_columns, dim3((unsigned)((nrow + 255) / 256), (unsigned)nrhs), dim3(256), 0, stream, w,
+                     nrow, nrhs, s);
   return ret(hipGetLastError());
 }
 

@@ -551,7 +551,19 @@ __global__ __launch_bounds__(256) void k_sv_branch(const uhmk_front* __restrict_
   if (i >= ss || fr.parent < 0) return;
   const uhmk_front P = F[fr.parent];
   const int pi = emap[fr.emap + i];
-  for (int c = 0; c < nrhs; ++c) x[fr.xoff + fs + i + (long long)c * n] = x[P.xoff + pi + (long long)c * P.n];
+  const double* src = x + P.xoff + pi;
+  double* dst = x + fr.xoff + fs + i;
+  int c = 0;
+  // four columns' loads in flight before their stores (parent and child
+  // rows never overlap)
+  for (; c + 4 <= nrhs; c += 4) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = src[(long long)(c + u) * P.n];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) dst[(long long)(c + u) * n] = v[u];
+  }
+  for (; c < nrhs; ++c) dst[(long long)c * n] = src[(long long)c * P.n];
 }
 
 // ------------------------------------------------------------------------

@@ -292,22 +292,37 @@ __global__ __launch_bounds__(256) void k_leaf_residual(long long nrow, const lon
 }
 
 // Global <-> front vectors over the factor rows of every front, all
-// columns: dir 0 g = xf, dir 1 xf = g, dir 2 g += xf.  One workgroup per
-// front.
-__global__ __launch_bounds__(256) void k_front_global_all(const uhmk_front* __restrict__ F,
+// columns: dir 0 g = xf, dir 1 xf = g, dir 2 g += xf.  One thread per
+// global row and 8 columns (grid: rows / 256, columns / 8); the owning
+// front is found by binary search in goff (factor rows are numbered
+// contiguously in front order), so both sides are read and written in
+// runs of consecutive rows.
+constexpr int GC = 8;
+__global__ __launch_bounds__(256) void k_front_global_all(const uhmk_front* __restrict__ F, int nfront,
                                                           const long long* __restrict__ goff,
                                                           double* __restrict__ xf, double* __restrict__ g,
                                                           int nrhs, long long nglob, int dir) {
-  const uhmk_front fr = F[blockIdx.x];
-  const int fs = fr.fs;
-  const long long tot = (long long)fs * nrhs;
-  for (long long idx = threadIdx.x; idx < tot; idx += 256) {
-    const int i = (int)(idx % fs), c = (int)(idx / fs);
-    const long long xo = fr.xoff + i + (long long)c * fr.n;
-    const long long go = goff[blockIdx.x] + i + (long long)c * nglob;
-    if (dir == 0) g[go] = xf[xo];
-    else if (dir == 1) xf[xo] = g[go];
-    else g[go] += xf[xo];
+  const long long row = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (row >= nglob) return;
+  int lo = 0, hi = nfront - 1;  // largest f with goff[f] <= row
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (goff[mid] <= row) lo = mid;
+    else hi = mid - 1;
+  }
+  const uhmk_front fr = F[lo];
+  const long long xo = fr.xoff + (row - goff[lo]);
+  const int c0 = blockIdx.y * GC;
+#pragma unroll
+  for (int u = 0; u < GC; ++u) {
+    const int c = c0 + u;
+    if (c < nrhs) {
+      double* __restrict__ px = xf + xo + (long long)c * fr.n;
+      double* __restrict__ pg = g + row + (long long)c * nglob;
+      if (dir == 0) *pg = *px;
+      else if (dir == 1) *px = *pg;
+      else *pg += *px;
+    }
   }
 }
 
@@ -614,10 +629,11 @@ int uhmk_rowsum_max(long long nrow, const long long* rowptr, const double* val, 
 
 int uhmk_front_global_all(const uhmk_front* fronts, int n_fronts, const long long* goff, double* xf, double* g,
                           int nrhs, long long nglob, int dir, hipStream_t stream) {
-  if (n_fronts <= 0) return 0;
+  if (n_fronts <= 0 || nglob <= 0 || nrhs <= 0) return 0;
   if (dir < 0 || dir > 2) return -1010;
-  hipLaunchKernelGGL(k_front_global_all, dim3(n_fronts), dim3(256), 0, stream, fronts, goff, xf, g, nrhs, nglob,
-                     dir);
+  if ((nrhs + GC - 1) / GC > 65535) return -1011;
+  const dim3 grid((unsigned)((nglob + 255) / 256), (unsigned)((nrhs + GC - 1) / GC));
+  hipLaunchKernelGGL(k_front_global_all, grid, dim3(256), 0, stream, fronts, n_fronts, goff, xf, g, nrhs, nglob, dir);
   return ret(hipGetLastError());
 }
 

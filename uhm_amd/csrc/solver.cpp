@@ -1760,14 +1760,17 @@ int FrontSolver::refine(int method, double tol, int max_iter) {
     eta = e2;
   }
   eta_ = eta;
-  // the refined solution back into the fronts (factor rows, then the
-  // branch pass for the schur rows) so that copy_out sees it
-  check_k(uhmk_front_global_all(F, nf, GO, d_x_.as<double>(), d_xg_.as<double>(), nrhs_, N, 1, stream_),
-          "scatter x");
-  std::vector<SolveStep> br;
-  for (const SolveStep& st : sbwd_)
-    if (st.kind == SolveStep::BRANCH) br.push_back(st);
-  run_solve_steps(br, chol, d_b_.as<double>());
+  if (it > 0) {
+    // the refined solution back into the fronts (factor rows, then the
+    // branch pass for the schur rows) so that copy_out sees it; without a
+    // correction the fronts already hold it
+    check_k(uhmk_front_global_all(F, nf, GO, d_x_.as<double>(), d_xg_.as<double>(), nrhs_, N, 1, stream_),
+            "scatter x");
+    std::vector<SolveStep> br;
+    for (const SolveStep& st : sbwd_)
+      if (st.kind == SolveStep::BRANCH) br.push_back(st);
+    run_solve_steps(br, chol, d_b_.as<double>());
+  }
   check_hip(hipStreamSynchronize(stream_), "refine");
   t_refine_ms_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return it;
