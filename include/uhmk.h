@@ -191,10 +191,14 @@ int uhmk_solve_block_step(const uhmk_front* fronts, const uhmk_item* items, int 
 /* Level-batched off-diagonal products of the block solves, 64 rows x 64
  * right-hand sides per item: dir 0: x[ABL rows] -= ABL z[0:fs) (forward
  * xb update); dir 1: z[ATL rows] -= ATR x[fs:n) (LU backward); dir 2:
- * z[ATL rows] -= ABL^T x[fs:n) (Cholesky backward).  The gemm / gemv
+ * z[ATL rows] -= ABL^T x[fs:n) (Cholesky backward).  ksplit > 1 cuts each
+ * item's K range into ksplit slices (more workgroups for levels with few
+ * long products); the partial tiles (n_items * ksplit * 4096 doubles in
+ * part) are summed in slice order by a second kernel.  The gemm / gemv
  * calls of solve_{lu_piv,chol}_{1,2}_hier on n_rhs columns. */
 int uhmk_solve_block_gemm(const uhmk_front* fronts, const uhmk_item* items, int n_items, int dir,
-                          const void* arena, int prec, double* x, double* z, int nrhs, hipStream_t stream);
+                          const void* arena, int prec, double* x, double* z, int nrhs, int ksplit, double* part,
+                          hipStream_t stream);
 
 /* Backward branch: xb <- the parent's solution rows through emap
  * (Helper_::_branch_rhs, helper.hxx:302-331).  items {f, 0, first row, 0},

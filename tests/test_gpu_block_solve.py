@@ -59,6 +59,9 @@ CASES = [
     ("lu-nb32-9", (5, 4, 4), 3, U.UHM_LU_PIV, None, 32, 9),
     ("lu-mixed-64", (5, 5, 4), 3, U.UHM_LU_PIV, 1, 64, 64),
     ("chol-mixed-130", (6, 5, 5), 3, U.UHM_CHOL, 1, 64, 130),
+    # top fronts of a few hundred rows: the level products split K
+    ("chol-w-split-70", (8, 8, 7), 3, U.UHM_CHOL, None, 64, 70),
+    ("lu-split-40", (8, 7, 7), 3, U.UHM_LU_PIV, None, 64, 40),
 ]
 
 

@@ -161,19 +161,66 @@ template <typename T, bool TA>
 __global__ __launch_bounds__(256, 2) void k_svb_gemm(const uhmk_front* __restrict__ F,
                                                      const uhmk_item* __restrict__ items,
                                                      const T* __restrict__ arena, double* __restrict__ x,
-                                                     double* __restrict__ z, int nrhs, int fwd) {
+                                                     double* __restrict__ z, int nrhs, int fwd, int ks,
+                                                     double* __restrict__ part) {
   __shared__ double sA[2][16 * LD];
   __shared__ double sB[2][16 * LD];
+  // blockIdx.x = item * ks + slice: with ks > 1 the K range is cut into ks
+  // slices whose partial tiles go to part (reduced by k_svb_reduce)
+  const int q = blockIdx.x / ks, sl = blockIdx.x - q * ks;
+  const uhmk_item it = items[q];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs;
+  const int c0 = it.a, r0 = it.b;
+  const int rend = fwd ? n : fs;
+  const int kb0 = fwd ? 0 : fs, ke0 = fwd ? fs : n;
+  const int len = (((ke0 - kb0) + ks - 1) / ks + 15) & ~15;
+  const int kbeg = min(ke0, kb0 + sl * len), kend = min(ke0, kbeg + len);
+  const double* __restrict__ S = (fwd ? z : x) + fr.xoff + (long long)c0 * n;
+  d4 acc[2][2];
+  zero_acc(acc);
+  if (kbeg < kend)
+    tile_product<T, TA>(acc, arena + fr.off, n, r0, rend, kbeg, kend, S, n, c0, nrhs, sA, sB);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (ks == 1) {
+    sub_acc((fwd ? x : z) + fr.xoff, n, r0, rend, c0, nrhs, acc, lane, w);
+    return;
+  }
+  // partial tile, layout [c][i] of the 64 x 64 tile
+  double* __restrict__ P = part + (long long)blockIdx.x * 4096;
+  const int cb = 32 * (w & 1), ib = 32 * (w >> 1);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int c = cb + 16 * a + 4 * g + (lane >> 4), i = ib + 16 * b + (lane & 15);
+        P[c * 64 + i] = acc[a][b][g];
+      }
+}
+
+// dst(r0 + i, c0 + c) -= sum over the ks slices (in order) of the partial
+// tiles of item q; one workgroup per item.
+__global__ __launch_bounds__(256) void k_svb_reduce(const uhmk_front* __restrict__ F,
+                                                    const uhmk_item* __restrict__ items, double* __restrict__ x,
+                                                    double* __restrict__ z, int nrhs, int fwd, int ks,
+                                                    const double* __restrict__ part) {
   const uhmk_item it = items[blockIdx.x];
   const uhmk_front fr = F[it.f];
   const int n = fr.n, fs = fr.fs;
   const int c0 = it.a, r0 = it.b;
   const int rend = fwd ? n : fs;
-  const double* __restrict__ S = (fwd ? z : x) + fr.xoff + (long long)c0 * n;
-  d4 acc[2][2];
-  zero_acc(acc);
-  tile_product<T, TA>(acc, arena + fr.off, n, r0, rend, fwd ? 0 : fs, fwd ? fs : n, S, n, c0, nrhs, sA, sB);
-  sub_acc((fwd ? x : z) + fr.xoff, n, r0, rend, c0, nrhs, acc, threadIdx.x & 63, threadIdx.x >> 6);
+  double* __restrict__ D = (fwd ? x : z) + fr.xoff;
+  const double* __restrict__ P = part + (long long)blockIdx.x * ks * 4096;
+#pragma unroll 4
+  for (int u = 0; u < 16; ++u) {
+    const int e = threadIdx.x + 256 * u, c = e >> 6, i = e & 63;
+    if (r0 + i >= rend || c0 + c >= nrhs) continue;
+    double s = 0.0;
+    for (int k = 0; k < ks; ++k) s += P[(long long)k * 4096 + e];
+    D[(r0 + i) + (long long)(c0 + c) * n] -= s;
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -558,13 +605,15 @@ int launch_step(int mode, int fwd, const uhmk_front* fronts, const uhmk_item* it
 
 template <typename T>
 int launch_gemm(int dir, const uhmk_front* fronts, const uhmk_item* items, int n_items, const T* arena, double* x,
-                double* z, int nrhs, hipStream_t stream) {
+                double* z, int nrhs, int ks, double* part, hipStream_t stream) {
+  const int fwd = dir == 0 ? 1 : 0;
+  const dim3 g((unsigned)((long long)n_items * ks)), b(256);
   if (dir == 2)
-    hipLaunchKernelGGL((k_svb_gemm<T, true>), dim3(n_items), dim3(256), 0, stream, fronts, items, arena, x, z, nrhs,
-                       0);
+    hipLaunchKernelGGL((k_svb_gemm<T, true>), g, b, 0, stream, fronts, items, arena, x, z, nrhs, fwd, ks, part);
   else
-    hipLaunchKernelGGL((k_svb_gemm<T, false>), dim3(n_items), dim3(256), 0, stream, fronts, items, arena, x, z, nrhs,
-                       dir == 0 ? 1 : 0);
+    hipLaunchKernelGGL((k_svb_gemm<T, false>), g, b, 0, stream, fronts, items, arena, x, z, nrhs, fwd, ks, part);
+  if (ks > 1)
+    hipLaunchKernelGGL(k_svb_reduce, dim3(n_items), b, 0, stream, fronts, items, x, z, nrhs, fwd, ks, part);
   return ret(hipGetLastError());
 }
 
@@ -586,13 +635,17 @@ int uhmk_solve_block_step(const uhmk_front* fronts, const uhmk_item* items, int 
 }
 
 int uhmk_solve_block_gemm(const uhmk_front* fronts, const uhmk_item* items, int n_items, int dir,
-                          const void* arena, int prec, double* x, double* z, int nrhs, hipStream_t stream) {
+                          const void* arena, int prec, double* x, double* z, int nrhs, int ksplit, double* part,
+                          hipStream_t stream) {
   if (n_items <= 0) return 0;
   if (dir < 0 || dir > 2) return -1010;
+  if (ksplit < 1 || (ksplit > 1 && !part) || (long long)n_items * ksplit > 0x7fffffffLL) return -1013;
   if (prec == 0)
-    return launch_gemm<double>(dir, fronts, items, n_items, static_cast<const double*>(arena), x, z, nrhs, stream);
+    return launch_gemm<double>(dir, fronts, items, n_items, static_cast<const double*>(arena), x, z, nrhs, ksplit,
+                               part, stream);
   if (prec == 1)
-    return launch_gemm<float>(dir, fronts, items, n_items, static_cast<const float*>(arena), x, z, nrhs, stream);
+    return launch_gemm<float>(dir, fronts, items, n_items, static_cast<const float*>(arena), x, z, nrhs, ksplit,
+                              part, stream);
   return -1006;
 }
 

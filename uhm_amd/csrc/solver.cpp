@@ -1058,6 +1058,22 @@ void FrontSolver::build_block_solve_schedule(int method) {
   auto push = [&](int f, int r) {
     for (int t = 0; t < ntile; ++t) sitems_.push_back(uhmk_item{f, 64 * t, r, 0});
   };
+  // split-K for product launches with few items and long K (top levels):
+  // enough workgroups to fill the chip, every slice at least 256 deep
+  long long part_max = 0;
+  auto split = [&](SolveStep& g, bool fwd) {
+    double ksum = 0;
+    for (int q = g.off; q < g.off + g.cnt; ++q) {
+      const uhmk_front& fr = fr_[sitems_[q].f];
+      ksum += fwd ? fr.fs : fr.n - fr.fs;
+    }
+    const double kmean = g.cnt ? ksum / g.cnt : 0.0;
+    int ks = 1;
+    if (g.cnt < 1024) ks = (int)std::min(8.0, std::ceil(1024.0 / g.cnt));
+    ks = std::max(1, std::min(ks, (int)(kmean / 256.0)));
+    g.ks = ks;
+    if (ks > 1) part_max = std::max(part_max, (long long)g.cnt * ks * 4096);
+  };
   const int nl = (int)levels_.size();
   std::vector<int> mfs(nl, 0);
   for (int L = 0; L < nl; ++L)
@@ -1089,6 +1105,7 @@ void FrontSolver::build_block_solve_schedule(int method) {
     }
     SolveStep g = step(SolveStep::BGEMM, o, L);
     g.mode = 0;
+    split(g, true);
     if (g.cnt) sfwd_.push_back(g);
   }
   for (int L = nl - 1; L >= 0; --L) {
@@ -1106,6 +1123,7 @@ void FrontSolver::build_block_solve_schedule(int method) {
     }
     SolveStep g = step(SolveStep::BGEMM, o, L);
     g.mode = chol ? 2 : 1;
+    split(g, false);
     if (g.cnt) sbwd_.push_back(g);
     for (int k0 = ((mfs[L] - 1) / 64) * 64; k0 >= 0 && mfs[L] > 0; k0 -= 64) {
       o = (int)sitems_.size();
@@ -1121,6 +1139,7 @@ void FrontSolver::build_block_solve_schedule(int method) {
       sbwd_.push_back(st);
     }
   }
+  if (part_max > 0 && (size_t)part_max * sizeof(double) > d_part_.bytes) d_part_.alloc(part_max * sizeof(double));
 }
 
 void FrontSolver::build_solve_schedule(int method) {
@@ -1334,8 +1353,8 @@ void FrontSolver::run_solve_steps(const std::vector<SolveStep>& steps, bool chol
                                    nrhs_, stream_);
         break;
       case SolveStep::BGEMM:
-        rc = uhmk_solve_block_gemm(F, SI + st.off, st.cnt, st.mode, A, st.prec, X, d_z_.as<double>(), nrhs_,
-                                   stream_);
+        rc = uhmk_solve_block_gemm(F, SI + st.off, st.cnt, st.mode, A, st.prec, X, d_z_.as<double>(), nrhs_, st.ks,
+                                   d_part_.as<double>(), stream_);
         break;
     }
     check_k(rc, "solve");

@@ -70,6 +70,7 @@ struct SolveStep {
   int fwd = 0;                     // trsv / cstep: forward (1) or backward (0)
   int k0 = 0;                      // cstep / bstep: first row of the block
   int mode = 0;                    // bstep: 0 LU, 1 Cholesky, 2 Cholesky with W; bgemm: direction
+  int ks = 1;                      // bgemm: K slices (split-K for levels with few, long products)
 };
 
 // Per-kind totals of a profiled factorization (device events per launch).
@@ -267,6 +268,7 @@ class FrontSolver {
   DevBuf d_arena_, d_arena32_, d_piv_, d_x_, d_b_, d_stage_, d_info_;
   DevBuf d_rf_;  // refinement right-hand side (front layout)
   DevBuf d_z_;   // second solve vector (front layout) of the block-step Cholesky solves
+  DevBuf d_part_;  // split-K partial tiles of the block-solve products
   // assembled matrix (CSR) for residuals
   bool csr_built_ = false, vals_dirty_ = true, bg_dirty_ = true;
   long long nnz_ = 0;
