@@ -21,7 +21,8 @@ _lib = None
 
 
 def build():
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    # -B: always recompile (the caller asked for a build)
+    subprocess.run(["make", "-s", "-B", "-C", HERE], check=True)
     return LIB
 
 
