@@ -134,7 +134,7 @@ def run_path(U, m, w, mode, top, args):
     chol = w["method"] == 1
     factor = m.chol_with_free if chol else m.lu_piv_with_free
     solve = m.solve_chol if chol else m.solve_lu_piv
-    name = "fp64" if mode == U.PREC_FP64 else "mixed"
+    name = {U.PREC_FP64: "fp64", U.PREC_MIXED: "mixed", U.PREC_MIXED_BF16: "bf16"}[mode]
     m.set_precision(mode, top)
     m.set_refinement(REFINE_KINDS[args.refine or w.get("refine", "classical")](U), 20, 1e-6)
     tuned = None
@@ -251,6 +251,9 @@ def main():
                     help="refinement kind (default: the workload's; classical for chol / lu)")
     ap.add_argument("--reseeds", type=int, default=None,
                     help="graph-replayed re-factorizations after the timed paths (default: the workload's)")
+    ap.add_argument("--bf16", action="store_true",
+                    help="also time the BF16-Schur mixed path (BF16 operands, FP32 accumulation in the Schur "
+                         "complement products), reported beside the FP32 one")
     ap.add_argument("--tune", action="store_true",
                     help="run the autotuner over the schedule knobs before timing (default: built-in knobs)")
     args = ap.parse_args()
@@ -272,6 +275,7 @@ def main():
     top = w.get("top", args.top)
     r64 = run_path(U, m, w, U.PREC_FP64, top, args)
     rmx = run_path(U, m, w, U.PREC_MIXED, top, args)
+    rbf = run_path(U, m, w, U.PREC_MIXED_BF16, top, args) if args.bf16 else None
     n_reseeds = w.get("reseeds", 0) if args.reseeds is None else args.reseeds
     reseeds = run_reseeds(U, m, w, args, n_reseeds) if n_reseeds else None
     mixed_ok = rmx["eta"] <= 1e-12
@@ -331,6 +335,7 @@ def main():
         "roofline_fp64_path": gemm_roofline(r64["stats"], PEAK_FP64_TFLOPS, "k_gemm (FP64 MFMA)"),
         "mixed": path(rmx),
         "fp64": path(r64),
+        "mixed_bf16": path(rbf) if rbf else None,
         "reseeds": reseeds,
         "cpu_baseline": cpu,
     }

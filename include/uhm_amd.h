@@ -156,7 +156,10 @@ void uhm_get_kernel_stats_(uhm_fort_p* mesh, uhm_fort_double* ms, uhm_fort_doubl
  * levels stay FP64 (their pivot blocks factorize in FP64), all other fronts
  * are stored in FP32, and every Schur / trailing update is an FP32 MFMA
  * product; the solution is then brought to FP64 accuracy with
- * uhm_improve_*_. */
+ * uhm_improve_*_.  mode 2: as mode 1, but the Schur complement products
+ * (ABR -= ABL ATR) take BF16 operands with FP32 accumulation; the panel
+ * updates inside each front stay FP32.  Modes 1 and 2 need the refinement,
+ * mode 2 usually more of it (GMRES-IR, uhm_set_refinement_). */
 void uhm_set_precision_(uhm_fort_p* mesh, uhm_fort_int* mode, uhm_fort_int* top_levels);
 /* Iterative refinement after uhm_solve_*_: x <- x + M^-1 (b - A x), the
  * residual in FP64 against the assembled matrix, M the last factorization,

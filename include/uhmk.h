@@ -112,7 +112,8 @@ int uhmk_chol_trsm(const uhmk_front* fronts, const uhmk_item* items, int n_items
  * ABR -= ABL ATR / ABL ABL^T (schur = 1, K = [0, fs)).  mma 0: FP64
  * products (v_mfma_f64_16x16x4f64, FP64 fronts only); mma 1: FP32
  * products and accumulation (v_mfma_f32_16x16x4f32) on FP32 or FP64
- * fronts.  Replaces linal::dense::gemm / syrk (LINAL/src/dense/gemm.cxx:48-137,
+ * fronts; mma 2: A and B rounded to BF16, FP32 products and accumulation
+ * (v_mfma_f32_16x16x32_bf16) on FP32 or FP64 fronts.  Replaces linal::dense::gemm / syrk (LINAL/src/dense/gemm.cxx:48-137,
  * syrk.cxx:43-123).  items: {f, i0, j0, 0}, tiles cut at fs. */
 int uhmk_gemm_update(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, int schur,
                      int transb, int tile, void* arena, int prec, int mma, hipStream_t stream);

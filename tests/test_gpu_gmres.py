@@ -134,3 +134,52 @@ def test_auto_refinement(lib):
     eta, _ = m.refine_history()
     say("  auto: %d corrections, %d solves, eta %s" % (n, m.refine_solves(), " ".join("%.1e" % e for e in eta)))
     assert m.backward_error() <= ETA
+
+
+def _lu3d(nrhs, seed=10):
+    m = U.Mesh()
+    m.build_synthetic(2, (6, 5, 5), 3, 0)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, nrhs)
+    m.create_matrix_buffer()
+    m.set_random_params(seed=seed, alpha=64.0, beta=0.1, scale_kind=1, p0=0.5)
+    m.random_matrix()
+    m.set_rhs()
+    return m
+
+
+@pytest.mark.parametrize("case", [("chol3d-bf16", _chol3d, U.UHM_CHOL), ("lu3d-bf16", _lu3d, U.UHM_LU_PIV)],
+                         ids=lambda c: c[0])
+@pytest.mark.parametrize("top", [0, 1])
+def test_bf16_schur_refined_to_fp64_accuracy(lib, case, top):
+    """BF16-operand Schur complement products (FP32 accumulation), FP32
+    fronts below the top `top` levels: GMRES-IR brings every right-hand side
+    to backward error <= 1e-12 and to the FP64 path's solution within the
+    condition-aware bar, and the unrefined BF16 solution is measurably less
+    accurate than the FP32-Schur one (the BF16 products really ran)."""
+    name, make, method = case
+    say("case", name, "top", top)
+    nrhs = 6
+    m = make(nrhs)
+    assert _factor(m, method) == 0
+    _solve(m, method)
+    x64 = m.fetch_x().copy()
+    kappa = kappa_1(m)
+    eta0 = {}
+    for mode in (U.PREC_MIXED, U.PREC_MIXED_BF16):
+        m.set_precision(mode, top)
+        assert _factor(m, method) == 0
+        _solve(m, method)
+        m.set_refinement(U.REFINE_GMRES, 30, 1e-4)
+        n = m.improve(method, ETA, 10)
+        eta, _ = m.refine_history()
+        eta0[mode] = eta[0]
+        say("  mode %d: %d corrections, %d solves, eta %s" % (mode, n, m.refine_solves(),
+                                                                " ".join("%.1e" % e for e in eta)))
+        assert m.backward_error() <= ETA
+        x = m.fetch_x()
+        err = (np.abs(x - x64).max(axis=0) / np.abs(x64).max(axis=0)).max()
+        tol = max(1e-10, 100.0 * kappa * ETA)
+        say("  kappa_1 ~ %.2e, forward error vs FP64 path %.3e (bar %.2e)" % (kappa, err, tol))
+        assert err <= tol
+    assert eta0[U.PREC_MIXED_BF16] > 10.0 * eta0[U.PREC_MIXED]

@@ -95,7 +95,7 @@ KERNEL_KINDS = ("assemble", "lu_panel", "lu_rows", "chol_diag", "chol_trsm", "ge
 _RET = {"uhm_get_last_error_": C.c_int}
 
 
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED_BF16 = 0, 1, 2
 REFINE_CLASSICAL, REFINE_GMRES, REFINE_AUTO = 0, 1, 2
 
 
@@ -302,8 +302,9 @@ class Mesh:
         return a.value, b.value
 
     def set_precision(self, mode, top_levels=2):
-        """mode PREC_FP64 (0) or PREC_MIXED (1: FP32 fronts below the top
-        `top_levels` levels, FP32 MFMA Schur updates)."""
+        """mode PREC_FP64 (0), PREC_MIXED (1: FP32 fronts below the top
+        `top_levels` levels, FP32 MFMA Schur updates) or PREC_MIXED_BF16 (2:
+        as 1 with BF16-operand, FP32-accumulate Schur complement products)."""
         _call("uhm_set_precision_", C.byref(self.handle), _i(mode), _i(top_levels))
 
     def improve(self, method, tol=1e-12, max_iter=20):

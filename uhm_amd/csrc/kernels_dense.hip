@@ -425,6 +425,166 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ 
     }
 }
 
+// ------------------------------------------------------------------------
+// The same update with BF16 operands and FP32 accumulation
+// (v_mfma_f32_16x16x32_bf16): A and B are rounded once to BF16 (nearest
+// even) on their way to LDS, the products and sums are FP32, and C is updated in
+// its storage type.  Both LDS panels are stored [row][k] with k contiguous,
+// so that a lane fetches its eight consecutive k of one row with one
+// ds_read_b128; rows are LDK = 40 BF16 (80 bytes) apart, which spreads the
+// sixteen rows a quarter-wave reads over all 64 banks.  One K chunk of 32 is
+// one MFMA k-step.
+// Operand layout: lane l supplies row l % 16 and k = 8 (l / 16) .. +7 of both
+// operands (any k order common to the two operands gives the same sum);
+// accumulator register r of lane l holds D[4 (l / 16) + r][l % 16].  As in
+// k_gemm the product is formed transposed (D = B^T A^T), so a 16-lane group
+// stores 16 consecutive rows of C.
+// Staging: A (and NT B, whose rows are also contiguous in memory) -- thread
+// t loads row t % T at KPT consecutive k and writes them packed; NN B
+// (columns contiguous in memory) -- thread t loads the k pair 2 (t % 16),
+// +1 of columns t / 16 + 16 u and writes it as one dword.
+// ------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// One rounding to BF16, nearest even, from either storage type (an FP64
+// value is not rounded to FP32 first).
+template <typename TS>
+__device__ __forceinline__ __bf16 to_bf16(TS v) {
+  return (__bf16)v;
+}
+
+template <bool TRANSB, int T, typename TS>
+__global__ __launch_bounds__(256, 2) void k_gemm_bf16(const uhmk_front* __restrict__ F,
+                                                   const uhmk_item* __restrict__ items, int c0, int nb, int schur,
+                                                   TS* __restrict__ arena) {
+  constexpr int KC = 32;             // k per chunk: one MFMA k-step
+  constexpr int LDK = KC + 8;        // BF16 row stride of the LDS panels
+  constexpr int NB = T / 32;         // 16x16 blocks per wave and dimension
+  constexpr int KPT = KC * T / 256;  // A / NT B: consecutive k per thread (16 or 8)
+  constexpr int JP = T / 16;         // NN B: columns per thread
+  static_assert(KPT % 8 == 0, "A rows are written in groups of eight");
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs;
+  const int kbeg = schur ? 0 : c0;
+  const int kend = schur ? fs : min(c0 + nb, fs);
+  const int i0 = it.a, j0 = it.b;
+  const int i1 = min(i0 + T, i0 < fs ? fs : n);
+  const int j1 = min(j0 + T, j0 < fs ? fs : n);
+  TS* __restrict__ M = arena + fr.off;
+  __shared__ __align__(16) __bf16 sA[T * LDK];
+  __shared__ __align__(16) __bf16 sB[T * LDK];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int iw = (w & 1) * (T / 2), jw = (w >> 1) * (T / 2);
+
+  f4 acc[NB][NB];
+#pragma unroll
+  for (int a = 0; a < NB; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[a][b] = f4{0, 0, 0, 0};
+
+  TS ra[KPT], rb[KPT];
+  const int li = tid % T, lk = (tid / T) * KPT;    // A / NT B: row, first k
+  const int bp = 2 * (tid % 16), bj = tid / 16;     // NN B: k pair, first column
+  const bool a_ok = i0 + li < i1, bt_ok = j0 + li < j1;
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < KPT; ++u) {
+      const int k = k0 + lk + u;
+      ra[u] = uhmd::ldz_as<TS>(M, (i0 + li) + (long long)k * n, a_ok && k < kend);
+    }
+    if (TRANSB) {
+#pragma unroll
+      for (int u = 0; u < KPT; ++u) {
+        const int k = k0 + lk + u;
+        rb[u] = uhmd::ldz_as<TS>(M, (j0 + li) + (long long)k * n, bt_ok && k < kend);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < JP; ++u) {
+        const int j = j0 + bj + 16 * u;
+        const long long base = (long long)j * n + k0 + bp;
+        rb[2 * u] = uhmd::ldz_as<TS>(M, base, j < j1 && k0 + bp < kend);
+        rb[2 * u + 1] = uhmd::ldz_as<TS>(M, base + 1, j < j1 && k0 + bp + 1 < kend);
+      }
+    }
+  };
+  if (kbeg < kend) load(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += KC) {
+#pragma unroll
+    for (int h = 0; h < KPT / 8; ++h) {
+      bf16x8 v;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] = to_bf16(ra[8 * h + e]);
+      *reinterpret_cast<bf16x8*>(&sA[li * LDK + lk + 8 * h]) = v;
+    }
+    if (TRANSB) {
+#pragma unroll
+      for (int h = 0; h < KPT / 8; ++h) {
+        bf16x8 v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = to_bf16(rb[8 * h + e]);
+        *reinterpret_cast<bf16x8*>(&sB[li * LDK + lk + 8 * h]) = v;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < JP; ++u) {
+        bf16x2 v;
+        v[0] = to_bf16(rb[2 * u]);
+        v[1] = to_bf16(rb[2 * u + 1]);
+        *reinterpret_cast<bf16x2*>(&sB[(bj + 16 * u) * LDK + bp]) = v;
+      }
+    }
+    __syncthreads();
+    if (k0 + KC < kend) load(k0 + KC);
+    {
+      const int kq = 8 * (lane >> 4), cl = lane & 15;
+      bf16x8 bo[NB], ao[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) bo[b] = *reinterpret_cast<const bf16x8*>(&sB[(jw + b * 16 + cl) * LDK + kq]);
+#pragma unroll
+      for (int a = 0; a < NB; ++a) ao[a] = *reinterpret_cast<const bf16x8*>(&sA[(iw + a * 16 + cl) * LDK + kq]);
+#pragma unroll
+      for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bo[b], ao[a], acc[a][b], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < NB; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int i = i0 + iw + a * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = j0 + jw + b * 16 + 4 * (lane >> 4) + r;
+        if (i < i1 && j < j1) {
+          TS* c = M + i + (long long)j * n;
+          *c = (TS)((double)*c - (double)acc[a][b][r]);
+        }
+      }
+    }
+}
+
+template <typename TS>
+int launch_gemm_bf16(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, int schur,
+                     int transb, int tile, TS* arena, hipStream_t stream) {
+  dim3 g(n_items), b(256);
+  if (tile == 128) {
+    if (transb) hipLaunchKernelGGL((k_gemm_bf16<true, 128, TS>), g, b, 0, stream, fronts, items, c0, nb, schur, arena);
+    else hipLaunchKernelGGL((k_gemm_bf16<false, 128, TS>), g, b, 0, stream, fronts, items, c0, nb, schur, arena);
+  } else if (tile == 64) {
+    if (transb) hipLaunchKernelGGL((k_gemm_bf16<true, 64, TS>), g, b, 0, stream, fronts, items, c0, nb, schur, arena);
+    else hipLaunchKernelGGL((k_gemm_bf16<false, 64, TS>), g, b, 0, stream, fronts, items, c0, nb, schur, arena);
+  } else {
+    return -1004;
+  }
+  return 0;
+}
+
 template <typename TS, typename TM>
 int launch_gemm(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, int schur,
                 int transb, int tile, TS* arena, hipStream_t stream) {
@@ -490,6 +650,12 @@ int uhmk_gemm_update(const uhmk_front* fronts, const uhmk_item* items, int n_ite
   else if (prec == 1 && mma == 1)
     rc = launch_gemm<float, float>(fronts, items, n_items, c0, nb, schur, transb, tile, static_cast<float*>(arena),
                                    stream);
+  else if (prec == 0 && mma == 2)
+    rc = launch_gemm_bf16<double>(fronts, items, n_items, c0, nb, schur, transb, tile, static_cast<double*>(arena),
+                                  stream);
+  else if (prec == 1 && mma == 2)
+    rc = launch_gemm_bf16<float>(fronts, items, n_items, c0, nb, schur, transb, tile, static_cast<float*>(arena),
+                                 stream);
   else
     return -1006;
   if (rc) return rc;

@@ -67,7 +67,7 @@ void FrontSolver::layout_arenas() {
   // the FP32 arena; each offset is 512-byte aligned.
   const int nl = (int)levels_.size();
   level_prec_.assign(nl, 0);
-  if (prec_mode_ == 1)
+  if (prec_mode_ != 0)
     for (int L = 0; L < nl - top_fp64_; ++L) level_prec_[L] = 1;
   long long a64 = 0, a32 = 0;
   for (int f = 0; f < (int)fr_.size(); ++f) {
@@ -87,7 +87,8 @@ void FrontSolver::layout_arenas() {
 }
 
 void FrontSolver::set_precision(int mode, int top_levels) {
-  if (mode != 0 && mode != 1) throw std::runtime_error("set_precision: mode must be 0 (FP64) or 1 (mixed)");
+  if (mode < 0 || mode > 2)
+    throw std::runtime_error("set_precision: mode must be 0 (FP64), 1 (mixed FP32) or 2 (mixed, BF16 Schur)");
   if (top_levels < 0) throw std::runtime_error("set_precision: top_levels must be >= 0");
   if (mode == prec_mode_ && top_levels == top_fp64_) return;
   prec_mode_ = mode;
@@ -714,7 +715,9 @@ void FrontSolver::build_schedule(int method) {
   }
   for (Launch& l : launches_) {
     l.prec = level_prec_[l.level];
-    l.mma = (prec_mode_ == 0) ? 0 : 1;
+    // mode 2: the Schur complement products take BF16 operands; the panel
+    // (trailing) updates, which form the factor columns themselves, stay FP32
+    l.mma = prec_mode_ == 0 ? 0 : (prec_mode_ == 2 && l.kind == Launch::SCHUR ? 2 : 1);
   }
   // Algorithmic work of every launch: the flops / bytes the operation needs,
   // not what a kernel happens to execute (diagonal SYRK tiles count their

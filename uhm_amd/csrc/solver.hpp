@@ -51,7 +51,7 @@ struct Launch {
   int level = 0;
   int tile = 64;      // gemm / schur tile edge
   int prec = 0;       // storage of the level's fronts: 0 FP64, 1 FP32
-  int mma = 0;        // gemm / schur products: 0 FP64, 1 FP32
+  int mma = 0;        // gemm / schur products: 0 FP64, 1 FP32, 2 BF16 operands (FP32 sums)
   double flops = 0;   // algorithmic flops of the launch
   double bytes = 0;   // algorithmic HBM bytes of the launch
   int stream = 0;     // 0: main stream, 1: lookahead stream
