@@ -193,7 +193,10 @@ void uhm_get_refine_solves_(uhm_fort_p* mesh, uhm_fort_int* n);
  * the next panel's diagonal block and TRSM run on a second stream beside
  * the current panel's trailing update (default 0: off; the stream
  * dependencies cost more than the overlap wins on the measured meshes, the
- * autotuner tries it). */
+ * autotuner tries it), key 6 = Cholesky diagonal blocks inside a panel
+ * group factored by the workgroup of the previous panel update that
+ * updates them, without a launch of their own (default 1; 0: one diagonal
+ * launch per panel; the factors are the same bits either way). */
 void uhm_set_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value);
 void uhm_get_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value);
 /* Autotuner: times factorizations of the current values over the knobs

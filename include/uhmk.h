@@ -107,6 +107,16 @@ int uhmk_chol_diag(const uhmk_front* fronts, const int* flist, int n_fronts, int
 int uhmk_chol_trsm(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, void* arena,
                    int prec, hipStream_t stream);
 
+/* Cholesky panel update as uhmk_gemm_update (schur = 0, NT, tile 64,
+ * nb = 64) in which the workgroup of each item flagged c = 1 -- the tile
+ * (c1, c1), c1 = min(c0 + nb, fs), i.e. the next panel's diagonal block,
+ * listed before the other items -- then factors that block as
+ * uhmk_chol_diag(c0 = c1) does, from its updated values in LDS: the next
+ * panel's diagonal launch is folded into this one (same results, bit for
+ * bit).  mma 0 (FP64 fronts) or 1 (FP32 products); info as uhmk_chol_diag. */
+int uhmk_chol_gemm_diag(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, void* arena,
+                        int prec, int mma, int* info, hipStream_t stream);
+
 /* C -= A op(B) on tile x tile blocks (tile 64 or 128) on MFMA: panel
  * update (schur = 0, K = [c0, c0+nb)) or final Schur complement
  * ABR -= ABL ATR / ABL ABL^T (schur = 1, K = [0, fs)).  mma 0: FP64
@@ -200,6 +210,15 @@ int uhmk_solve_block_step(const uhmk_front* fronts, const uhmk_item* items, int 
 int uhmk_solve_block_gemm(const uhmk_front* fronts, const uhmk_item* items, int n_items, int dir,
                           const void* arena, int prec, double* x, double* z, int nrhs, int ksplit, double* part,
                           hipStream_t stream);
+
+/* uhmk_solve_gather and uhmk_solve_branch for many right-hand sides: the
+ * same results, bit for bit, with items of 64 rows ({f, 0, first row, 0})
+ * and one wavefront per item and group of 8 columns. */
+int uhmk_solve_gather_cols(const uhmk_front* fronts, const uhmk_item* items, int n_items, int lu,
+                           const int* src_beg, const uhmk_source* srcs, const int* inv, const int* perm, double* x,
+                           const double* b, int nrhs, hipStream_t stream);
+int uhmk_solve_branch_cols(const uhmk_front* fronts, const uhmk_item* items, int n_items, const int* emap, double* x,
+                           int nrhs, hipStream_t stream);
 
 /* Backward branch: xb <- the parent's solution rows through emap
  * (Helper_::_branch_rhs, helper.hxx:302-331).  items {f, 0, first row, 0},

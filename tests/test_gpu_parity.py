@@ -381,3 +381,59 @@ def test_cholesky_lookahead_matches_in_order(lib, prec, group):
     m.check_chol()
     if prec == U.PREC_FP64:
         assert m.backward_error() <= ETA_TOL
+
+
+def _chol_mesh(seed, alpha):
+    m = U.Mesh()
+    m.build_synthetic(2, (8, 8, 7), 3, 0)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    m.create_matrix_buffer()
+    m.set_random_params(seed=seed, alpha=alpha, scale_kind=1, p0=0.5)
+    m.random_matrix()
+    m.set_rhs()
+    return m
+
+
+@pytest.mark.parametrize("prec", [U.PREC_FP64, U.PREC_MIXED])
+def test_cholesky_fused_diagonal_same_bits(lib, prec):
+    """Diagonal blocks factored by the workgroup of the panel update that
+    produced them (no diagonal launch inside a panel group) give the same
+    front values, bit for bit, as one diagonal launch per panel, over the
+    part of a front the Cholesky path defines: the lower triangle, and W in
+    the upper triangles of the 64 x 64 diagonal blocks of ATL (the rest of
+    the upper triangle is never assembled or read and holds stale values)."""
+    def defined(m, f):
+        G = m.fetch_front(f)
+        fs = m.front_info(f)["fs"]
+        return [np.tril(G)] + [G[k0:min(k0 + 64, fs), k0:min(k0 + 64, fs)].copy() for k0 in range(0, fs, 64)]
+
+    m = _chol_mesh(19, 64.0)
+    m.set_precision(prec, 1)
+    say("case fused diagonal prec", prec, "max fs", max(m.front_info(f)["fs"] for f in range(m.n_fronts())))
+    m.set_tuning(6, 0)
+    assert m.chol_with_free() == 0
+    ref = [defined(m, f) for f in range(m.n_fronts())]
+    m.set_tuning(6, 1)
+    assert m.get_tuning(6) == 1
+    for rep in range(2):
+        assert m.chol_with_free() == 0
+        for f in range(m.n_fronts()):
+            got = defined(m, f)
+            assert all(np.array_equal(a, b) for a, b in zip(got, ref[f])), (rep, f)
+    m.solve_chol()
+    m.check_chol()
+    if prec == U.PREC_FP64:
+        assert m.backward_error() <= ETA_TOL
+
+
+def test_cholesky_fused_diagonal_reports_failure(lib):
+    """A matrix that is not positive definite fails with the same element
+    reported whether or not the diagonal blocks are fused."""
+    got = []
+    for fuse in (0, 1):
+        m = _chol_mesh(23, -4.0)
+        m.set_tuning(6, fuse)
+        got.append(m.chol_with_free())
+    say("  info unfused %d fused %d" % tuple(got))
+    assert got[0] != 0 and got[0] == got[1]

@@ -91,36 +91,28 @@ struct Chol16Step<16> {
 // the lower triangle of a front is used); the panel TRSM and the solves
 // read it from there (its diagonal is 1 / L(k,k)).  Padding beyond kb is
 // an identity block, so it never produces a non-positive pivot.
+// LDS of the diagonal-block factorization.
+struct DiagLds {
+  double sL[TILE * LDW];
+  double sW[TILE * LDW];
+  double sS[4][16 * 17];
+  int s_first;
+};
+
+// Factorization of the block held in S.sL (lower triangle of the kb x kb
+// block, identity beyond kb, zeros above the diagonal; S.sW zero,
+// S.s_first zero, all visible to the workgroup), then L and W back to the
+// front at A (the block's first entry, column stride n) and the first
+// non-positive pivot (c0 + its index + 1) to info[f].  Called by all 256
+// threads of the workgroup.
 template <typename T>
-__global__ __launch_bounds__(256) void k_chol_diag(const uhmk_front* __restrict__ F,
-                                                   const int* __restrict__ flist, int c0, int nb,
-                                                   T* __restrict__ arena, int* __restrict__ info) {
-  __shared__ double sL[TILE * LDW];
-  __shared__ double sW[TILE * LDW];
-  __shared__ double sS[4][16 * 17];
-  __shared__ int s_first;
-  const int f = flist[blockIdx.x];
-  const uhmk_front fr = F[f];
-  const int n = fr.n, fs = fr.fs;
-  const int kb = min(nb, fs - c0);
-  T* __restrict__ A = arena + fr.off + c0 + (long long)c0 * n;
+__device__ __forceinline__ void chol_diag_lds(DiagLds& S, T* __restrict__ A, long long n, int kb, int c0,
+                                              int* __restrict__ info, int f) {
+  double* __restrict__ sL = S.sL;
+  double* __restrict__ sW = S.sW;
+  auto& sS = S.sS;
+  int& s_first = S.s_first;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  {
-    double v[16];
-#pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
-      v[u] = uhmd::ldz(A, r + (long long)c * n, r < kb && c < kb && c <= r);
-    }
-#pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
-      sL[r * LDW + c] = (r >= kb && r == c) ? 1.0 : (c <= r ? v[u] : 0.0);
-      sW[r * LDW + c] = 0.0;
-    }
-  }
-  if (tid == 0) s_first = 0;
-  __syncthreads();
   for (int B = 0; B < 4; ++B) {
     const int b0 = 16 * B;
     if (w == 0 && lane < 16) {
@@ -213,6 +205,36 @@ __global__ __launch_bounds__(256) void k_chol_diag(const uhmk_front* __restrict_
     }
   }
   if (tid == 0 && s_first && info[f] == 0) info[f] = c0 + s_first;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_chol_diag(const uhmk_front* __restrict__ F,
+                                                   const int* __restrict__ flist, int c0, int nb,
+                                                   T* __restrict__ arena, int* __restrict__ info) {
+  __shared__ DiagLds S;
+  const int f = flist[blockIdx.x];
+  const uhmk_front fr = F[f];
+  const int n = fr.n, fs = fr.fs;
+  const int kb = min(nb, fs - c0);
+  T* __restrict__ A = arena + fr.off + c0 + (long long)c0 * n;
+  const int tid = threadIdx.x;
+  {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
+      v[u] = uhmd::ldz(A, r + (long long)c * n, r < kb && c < kb && c <= r);
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
+      S.sL[r * LDW + c] = (r >= kb && r == c) ? 1.0 : (c <= r ? v[u] : 0.0);
+      S.sW[r * LDW + c] = 0.0;
+    }
+  }
+  if (tid == 0) S.s_first = 0;
+  __syncthreads();
+  chol_diag_lds<T>(S, A, n, kb, c0, info, f);
 }
 
 // Cholesky panel rows [b, b+64) below the diagonal block: X <- X L11^-T =
@@ -323,10 +345,17 @@ struct Mfma<float> {
   __device__ __forceinline__ static int col(int r, int lane) { return 4 * (lane >> 4) + r; }
 };
 
-template <bool TRANSB, int T, typename TS, typename TM>
+// DIAG (Cholesky panel updates, NT, T = 64): the workgroup of an item with
+// c == 1 -- the tile (c1, c1) that is the next panel's diagonal block, c1 =
+// min(c0 + nb, fs), listed first so that it starts first -- keeps its
+// updated lower triangle (rounded to TS, as a store and reload would) in
+// LDS and factors it there (chol_diag_lds, the k_chol_diag computation), so
+// the next panel needs no diagonal launch.  Its LDS is shared with the
+// product's panels, which are dead by then.
+template <bool TRANSB, int T, typename TS, typename TM, bool DIAG = false>
 __global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ F,
                                               const uhmk_item* __restrict__ items, int c0, int nb, int schur,
-                                              TS* __restrict__ arena) {
+                                              TS* __restrict__ arena, int* __restrict__ info = nullptr) {
   typedef typename Mfma<TM>::acc_t acc_t;
   constexpr int LD = T + 16;          // row stride of the LDS panels (conflict-free half-wave reads)
   constexpr int NB = T / 32;          // 16x16 blocks per wave and dimension
@@ -345,8 +374,12 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ 
   const int i1 = min(i0 + T, i0 < fs ? fs : n);
   const int j1 = min(j0 + T, j0 < fs ? fs : n);
   TS* __restrict__ M = arena + fr.off;
-  __shared__ TM sA[KC * LD];
-  __shared__ TM sB[KC * LD];
+  static_assert(!DIAG || (TRANSB && T == TILE), "fused diagonal blocks: Cholesky updates on 64-wide tiles");
+  constexpr size_t PANEL_BYTES = 2 * KC * LD * sizeof(TM);
+  constexpr size_t LDS_BYTES = (DIAG && sizeof(DiagLds) > PANEL_BYTES) ? sizeof(DiagLds) : PANEL_BYTES;
+  __shared__ __align__(16) unsigned char smem[LDS_BYTES];
+  TM* __restrict__ sA = reinterpret_cast<TM*>(smem);
+  TM* __restrict__ sB = sA + KC * LD;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int iw = (w & 1) * (T / 2), jw = (w >> 1) * (T / 2);
 
@@ -409,6 +442,36 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ 
     }
     __syncthreads();
   }
+  if (DIAG && it.c == 1) {
+    // the next panel's diagonal block: tile (c1, c1), kb = i1 - i0 rows
+    DiagLds& S = *reinterpret_cast<DiagLds*>(smem);
+    const int kb = i1 - i0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
+      S.sL[r * LDW + c] = (r >= kb && r == c) ? 1.0 : 0.0;
+      S.sW[r * LDW + c] = 0.0;
+    }
+    if (tid == 0) S.s_first = 0;
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int r = iw + a * 16 + (lane & 15);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int c = jw + b * 16 + Mfma<TM>::col(q, lane);
+          if (r < kb && c <= r) {
+            const TS* e = M + (i0 + r) + (long long)(j0 + c) * n;
+            S.sL[r * LDW + c] = (double)(TS)((double)*e - (double)acc[a][b][q]);
+          }
+        }
+      }
+    __syncthreads();
+    chol_diag_lds<TS>(S, M + i0 + (long long)j0 * n, n, kb, i0, info, it.f);
+    return;
+  }
 #pragma unroll
   for (int a = 0; a < NB; ++a)
 #pragma unroll
@@ -423,6 +486,14 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ 
         }
       }
     }
+}
+
+template <typename TS, typename TM>
+int launch_gemm_diag(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, TS* arena,
+                     int* info, hipStream_t stream) {
+  hipLaunchKernelGGL((k_gemm<true, 64, TS, TM, true>), dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb, 0,
+                     arena, info);
+  return 0;
 }
 
 // ------------------------------------------------------------------------
@@ -634,6 +705,23 @@ int uhmk_chol_trsm(const uhmk_front* fronts, const uhmk_item* items, int n_items
                        static_cast<float*>(arena));
   else
     return -1006;
+  return ret(hipGetLastError());
+}
+
+int uhmk_chol_gemm_diag(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, void* arena,
+                        int prec, int mma, int* info, hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  if (nb != TILE) return -1000;
+  int rc;
+  if (prec == 0 && mma == 0)
+    rc = launch_gemm_diag<double, double>(fronts, items, n_items, c0, nb, static_cast<double*>(arena), info, stream);
+  else if (prec == 0 && mma == 1)
+    rc = launch_gemm_diag<double, float>(fronts, items, n_items, c0, nb, static_cast<double*>(arena), info, stream);
+  else if (prec == 1 && mma == 1)
+    rc = launch_gemm_diag<float, float>(fronts, items, n_items, c0, nb, static_cast<float*>(arena), info, stream);
+  else
+    return -1006;
+  if (rc) return rc;
   return ret(hipGetLastError());
 }
 

@@ -120,6 +120,108 @@ __global__ __launch_bounds__(256) void k_sv_gather(const uhmk_front* __restrict_
   }
 }
 
+// The gather for many right-hand sides: one wavefront per 64-row chunk of a
+// front and group of GCW columns, the group's columns held in registers, so
+// that every lane has GCW (1 + sources) loads in flight and a small front
+// occupies one wavefront, not an idle workgroup.  Workgroup w takes column
+// group w % ng of item w / ng: the groups of an item run side by side, so the
+// wavefronts in flight sweep a few fronts' contiguous n x n_rhs blocks
+// rather than a sliver of every front's block.  Same sums in the same order
+// as k_sv_gather: b, then the children in merge order.
+constexpr int GCW = 8;
+
+template <bool LU>
+__global__ __launch_bounds__(64) void k_sv_gather_cols(const uhmk_front* __restrict__ F,
+                                                       const uhmk_item* __restrict__ items,
+                                                       const int* __restrict__ src_beg,
+                                                       const uhmk_source* __restrict__ srcs,
+                                                       const int* __restrict__ inv, const int* __restrict__ perm,
+                                                       double* __restrict__ x, const double* __restrict__ b,
+                                                       int nrhs, int ng) {
+  constexpr int MAXS = 8;
+  const uhmk_item it = items[blockIdx.x / ng];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs;
+  const int i = it.b + threadIdx.x;
+  if (i >= n) return;
+  const int c0 = (blockIdx.x % ng) * GCW;
+  const int r = (LU && i < fs) ? perm[fr.poff + i] : i;
+  double* __restrict__ xo = x + fr.xoff + i + (long long)c0 * n;
+  const bool rb = fr.soff >= 0 || r < fs;
+  const double* __restrict__ bo = b + fr.xoff + r + (long long)c0 * n;
+  const double* sp[MAXS];
+  long long sld[MAXS];
+  int ns = 0;
+  const int s0 = src_beg[it.f], s1 = src_beg[it.f + 1];
+  int s = s0;
+  for (; s < s1 && ns < MAXS; ++s) {
+    const uhmk_source sc = srcs[s];
+    if (sc.src < 0) continue;
+    const int ri = inv[sc.inv + r];
+    if (ri < 0) continue;
+    const uhmk_front ch = F[sc.src];
+    sp[ns] = x + ch.xoff + ch.fs + ri + (long long)c0 * ch.n;
+    sld[ns] = ch.n;
+    ++ns;
+  }
+  double acc[GCW];
+  if (c0 + GCW <= nrhs) {
+#pragma unroll
+    for (int u = 0; u < GCW; ++u) acc[u] = rb ? bo[(long long)u * n] : 0.0;
+#pragma unroll
+    for (int q = 0; q < MAXS; ++q)
+      if (q < ns) {
+#pragma unroll
+        for (int u = 0; u < GCW; ++u) acc[u] += sp[q][(long long)u * sld[q]];
+      }
+  } else {
+    const int nc = nrhs - c0;
+#pragma unroll
+    for (int u = 0; u < GCW; ++u) acc[u] = (rb && u < nc) ? bo[(long long)u * n] : 0.0;
+    for (int q = 0; q < ns; ++q)
+#pragma unroll
+      for (int u = 0; u < GCW; ++u)
+        if (u < nc) acc[u] += sp[q][(long long)u * sld[q]];
+  }
+  for (; s < s1; ++s) {
+    const uhmk_source sc = srcs[s];
+    if (sc.src < 0) continue;
+    const int ri = inv[sc.inv + r];
+    if (ri < 0) continue;
+    const uhmk_front ch = F[sc.src];
+    const double* __restrict__ xc = x + ch.xoff + ch.fs + ri + (long long)c0 * ch.n;
+#pragma unroll
+    for (int u = 0; u < GCW; ++u)
+      if (c0 + u < nrhs) acc[u] += xc[(long long)u * ch.n];
+  }
+#pragma unroll
+  for (int u = 0; u < GCW; ++u)
+    if (c0 + u < nrhs) xo[(long long)u * n] = acc[u];
+}
+
+// The branch for many right-hand sides, mapped as k_sv_gather_cols: item
+// {f, 0, first of 64 rows of xb, 0}.
+__global__ __launch_bounds__(64) void k_sv_branch_cols(const uhmk_front* __restrict__ F,
+                                                       const uhmk_item* __restrict__ items,
+                                                       const int* __restrict__ emap, double* __restrict__ x,
+                                                       int nrhs, int ng) {
+  const uhmk_item it = items[blockIdx.x / ng];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs, ss = n - fs;
+  const int i = it.b + threadIdx.x;
+  if (i >= ss || fr.parent < 0) return;
+  const int c0 = (blockIdx.x % ng) * GCW;
+  const uhmk_front P = F[fr.parent];
+  const double* __restrict__ src = x + P.xoff + emap[fr.emap + i] + (long long)c0 * P.n;
+  double* __restrict__ dst = x + fr.xoff + fs + i + (long long)c0 * n;
+  double v[GCW];
+#pragma unroll
+  for (int u = 0; u < GCW; ++u) v[u] = (c0 + u < nrhs) ? src[(long long)u * P.n] : 0.0;
+#pragma unroll
+  for (int u = 0; u < GCW; ++u)
+    if (c0 + u < nrhs) dst[(long long)u * n] = v[u];
+}
+
 // Broadcast of lane L's double to the wavefront (v_readlane).
 template <int L>
 __device__ __forceinline__ double bcast(double v) {
@@ -794,6 +896,31 @@ int uhmk_solve_chol_step(const uhmk_front* fronts, const uhmk_item* items, int n
   } else {
     return -1006;
   }
+  return ret(hipGetLastError());
+}
+
+int uhmk_solve_gather_cols(const uhmk_front* fronts, const uhmk_item* items, int n_items, int lu,
+                           const int* src_beg, const uhmk_source* srcs, const int* inv, const int* perm, double* x,
+                           const double* b, int nrhs, hipStream_t stream) {
+  if (n_items <= 0 || nrhs <= 0) return 0;
+  const long long ng = (nrhs + GCW - 1) / GCW, nb = ng * n_items;
+  if (nb > 0x7fffffffLL) return -1011;
+  if (lu)
+    hipLaunchKernelGGL(k_sv_gather_cols<true>, dim3((unsigned)nb), dim3(64), 0, stream, fronts, items, src_beg, srcs,
+                       inv, perm, x, b, nrhs, (int)ng);
+  else
+    hipLaunchKernelGGL(k_sv_gather_cols<false>, dim3((unsigned)nb), dim3(64), 0, stream, fronts, items, src_beg, srcs,
+                       inv, perm, x, b, nrhs, (int)ng);
+  return ret(hipGetLastError());
+}
+
+int uhmk_solve_branch_cols(const uhmk_front* fronts, const uhmk_item* items, int n_items, const int* emap, double* x,
+                           int nrhs, hipStream_t stream) {
+  if (n_items <= 0 || nrhs <= 0) return 0;
+  const long long ng = (nrhs + GCW - 1) / GCW, nb = ng * n_items;
+  if (nb > 0x7fffffffLL) return -1011;
+  hipLaunchKernelGGL(k_sv_branch_cols, dim3((unsigned)nb), dim3(64), 0, stream, fronts, items, emap, x, nrhs,
+                     (int)ng);
   return ret(hipGetLastError());
 }
 

@@ -340,16 +340,18 @@ __global__ __launch_bounds__(256) void k_leaf_residual(long long nrow, const lon
 
 // Global <-> front vectors over the factor rows of every front, all
 // columns: dir 0 g = xf, dir 1 xf = g, dir 2 g += xf.  One thread per
-// global row and 8 columns (grid: rows / 256, columns / 8); the owning
-// front is found by binary search in goff (factor rows are numbered
-// contiguously in front order), so both sides are read and written in
-// runs of consecutive rows.
+// global row and 8 columns; the owning front is found by binary search in
+// goff (factor rows are numbered contiguously in front order), so both
+// sides are read and written in runs of consecutive rows.  Workgroup w
+// takes column group w % ng of rows 256 (w / ng) ...: the column groups of a
+// row range run side by side, so the workgroups in flight sweep a few
+// fronts' contiguous blocks rather than a sliver of every front's block.
 constexpr int GC = 8;
 __global__ __launch_bounds__(256) void k_front_global_all(const uhmk_front* __restrict__ F, int nfront,
                                                           const long long* __restrict__ goff,
                                                           double* __restrict__ xf, double* __restrict__ g,
-                                                          int nrhs, long long nglob, int dir) {
-  const long long row = (long long)blockIdx.x * 256 + threadIdx.x;
+                                                          int nrhs, long long nglob, int dir, int ng) {
+  const long long row = (long long)(blockIdx.x / ng) * 256 + threadIdx.x;
   if (row >= nglob) return;
   int lo = 0, hi = nfront - 1;  // largest f with goff[f] <= row
   while (lo < hi) {
@@ -359,7 +361,7 @@ __global__ __launch_bounds__(256) void k_front_global_all(const uhmk_front* __re
   }
   const uhmk_front fr = F[lo];
   const long long xo = fr.xoff + (row - goff[lo]);
-  const int c0 = blockIdx.y * GC;
+  const int c0 = (blockIdx.x % ng) * GC;
 #pragma unroll
   for (int u = 0; u < GC; ++u) {
     const int c = c0 + u;
@@ -684,9 +686,10 @@ int uhmk_front_global_all(const uhmk_front* fronts, int n_fronts, const long lon
                           int nrhs, long long nglob, int dir, hipStream_t stream) {
   if (n_fronts <= 0 || nglob <= 0 || nrhs <= 0) return 0;
   if (dir < 0 || dir > 2) return -1010;
-  if ((nrhs + GC - 1) / GC > 65535) return -1011;
-  const dim3 grid((unsigned)((nglob + 255) / 256), (unsigned)((nrhs + GC - 1) / GC));
-  hipLaunchKernelGGL(k_front_global_all, grid, dim3(256), 0, stream, fronts, n_fronts, goff, xf, g, nrhs, nglob, dir);
+  const long long ng = (nrhs + GC - 1) / GC, nb = ng * ((nglob + 255) / 256);
+  if (nb > 0x7fffffffLL) return -1011;
+  hipLaunchKernelGGL(k_front_global_all, dim3((unsigned)nb), dim3(256), 0, stream, fronts, n_fronts, goff, xf, g, nrhs,
+                     nglob, dir, (int)ng);
   return ret(hipGetLastError());
 }
 

@@ -387,7 +387,7 @@ void FrontSolver::build_schedule(int method) {
   // rest of the update.  Events order the two streams; every main-stream
   // launch after a lookahead pair waits for it (pending).
   int n_events = 0, pending = -1, ev_next = -1;
-  bool split_prev = false;
+  bool split_prev = false, fused_prev = false;
   auto push_main = [&](Launch l) {
     l.stream = 0;
     if (pending >= 0) {
@@ -465,6 +465,9 @@ void FrontSolver::build_schedule(int method) {
       lp.nb = nb;
       lp.level = L;
       lp.kind = chol ? Launch::CHOL_DIAG : Launch::LU_PANEL;
+      // the previous panel's update factored every diagonal block of this one
+      const bool diag_done = fused_prev;
+      fused_prev = false;
       lp.off = (long long)flists_.size();
       lp.count = (int)act.size();
       lp.extra = maxrows;
@@ -501,7 +504,7 @@ void FrontSolver::build_schedule(int method) {
         }
         launches_.back().record_ev = pending = n_events++;
       } else {
-        push_main(lp);
+        if (!diag_done) push_main(lp);
         if (lr.count) push_main(lr);
       }
       split_prev = false;
@@ -549,6 +552,18 @@ void FrontSolver::build_schedule(int method) {
           }
           if (rest.count) push_main(rest);
         } else if (li.count) {
+          if (fuse_diag_ && !la && !group_last && nb == 64) {
+            // the next panel's diagonal tiles (c1, c1) first, flagged: their
+            // workgroups factor the block after updating it
+            auto diag_tile = [&](const uhmk_item& it) {
+              const int c1 = std::min(c0 + nb, fr_[it.f].fs);
+              return it.a == c1 && it.b == c1;
+            };
+            std::stable_partition(items_.begin() + li.off, items_.end(), diag_tile);
+            for (long long q = li.off; q < (long long)items_.size() && diag_tile(items_[q]); ++q) items_[q].c = 1;
+            li.fuse = 1;
+            fused_prev = true;
+          }
           push_main(li);
         }
         if (!group_last) continue;
@@ -842,7 +857,10 @@ void FrontSolver::enqueue_factor(int method, std::vector<hipEvent_t>* pev) {
         rc = uhmk_chol_trsm(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, st);
         break;
       case Launch::GEMM:
-        rc = uhmk_gemm_update(F, IT + l.off, l.count, l.c0, l.nb, 0, l.extra, l.tile, A, l.prec, l.mma, st);
+        if (l.fuse)
+          rc = uhmk_chol_gemm_diag(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, l.mma, info, st);
+        else
+          rc = uhmk_gemm_update(F, IT + l.off, l.count, l.c0, l.nb, 0, l.extra, l.tile, A, l.prec, l.mma, st);
         break;
       case Launch::SCHUR:
         rc = uhmk_gemm_update(F, IT + l.off, l.count, 0, 0, 1, l.extra, l.tile, A, l.prec, l.mma, st);
@@ -908,9 +926,14 @@ void FrontSolver::set_tuning(int key, int value) {
     if (value != 0 && value != 1) throw std::runtime_error("set_tuning: lookahead is 0 or 1");
     if (value != lookahead_) schedule_method_ = 0;
     lookahead_ = value;
+  } else if (key == 6) {
+    if (value != 0 && value != 1) throw std::runtime_error("set_tuning: diagonal fusion is 0 or 1");
+    if (value != fuse_diag_) schedule_method_ = 0;
+    fuse_diag_ = value;
   } else {
     throw std::runtime_error("set_tuning: unknown key (1: Cholesky panel group, 2: 128-tile threshold, "
-                             "3: block-solve threshold, 4: whole-front LU limit, 5: Cholesky lookahead)");
+                             "3: block-solve threshold, 4: whole-front LU limit, 5: Cholesky lookahead, "
+                             "6: Cholesky diagonal blocks fused into the panel updates)");
   }
 }
 
@@ -920,6 +943,7 @@ int FrontSolver::get_tuning(int key) const {
   if (key == 3) return block_rhs_;
   if (key == 4) return front_small_;
   if (key == 5) return lookahead_;
+  if (key == 6) return fuse_diag_;
   throw std::runtime_error("get_tuning: unknown key");
 }
 
@@ -1081,11 +1105,14 @@ void FrontSolver::build_block_solve_schedule(int method) {
   std::vector<int> mfs(nl, 0);
   for (int L = 0; L < nl; ++L)
     for (int f : levels_[L]) mfs[L] = std::max(mfs[L], fr_[f].fs);
+  // gather and branch: 64-row items, one wavefront per item and column group
   for (int L = 0; L < nl; ++L) {
     int o = (int)sitems_.size();
     for (int f : levels_[L])
-      for (int r = 0; r < fr_[f].n; r += 256) sitems_.push_back(uhmk_item{f, 0, r, 0});
-    sfwd_.push_back(step(SolveStep::GATHER, o, L));
+      for (int r = 0; r < fr_[f].n; r += 64) sitems_.push_back(uhmk_item{f, 0, r, 0});
+    SolveStep ga = step(SolveStep::GATHER, o, L);
+    ga.cols = 1;
+    sfwd_.push_back(ga);
     for (int k0 = 0; k0 < mfs[L]; k0 += 64) {
       o = (int)sitems_.size();
       for (int f : levels_[L]) {
@@ -1115,8 +1142,9 @@ void FrontSolver::build_block_solve_schedule(int method) {
     int o = (int)sitems_.size();
     for (int f : levels_[L])
       if (fr_[f].parent >= 0)
-        for (int r = 0; r < fr_[f].n - fr_[f].fs; r += 256) sitems_.push_back(uhmk_item{f, 0, r, 0});
+        for (int r = 0; r < fr_[f].n - fr_[f].fs; r += 64) sitems_.push_back(uhmk_item{f, 0, r, 0});
     SolveStep br = step(SolveStep::BRANCH, o, L);
+    br.cols = 1;
     if (br.cnt) sbwd_.push_back(br);
     o = (int)sitems_.size();
     for (int f : levels_[L]) {
@@ -1333,8 +1361,9 @@ void FrontSolver::run_solve_steps(const std::vector<SolveStep>& steps, bool chol
     int rc = 0;
     switch (st.kind) {
       case SolveStep::GATHER:
-        rc = uhmk_solve_gather(F, SI + st.off, st.cnt, chol ? 0 : 1, d_srcbeg_.as<int>(), d_srcs_.as<uhmk_source>(),
-                               d_inv_.as<int>(), d_perm_.as<int>(), X, rhs, nrhs_, stream_);
+rc = (st.cols ? uhmk_solve_gather_cols : uhmk_solve_gather)(
+            F, SI + st.off, st.cnt, chol ? 0 : 1, d_srcbeg_.as<int>(), d_srcs_.as<uhmk_source>(), d_inv_.as<int>(),
+            d_perm_.as<int>(), X, rhs, nrhs_, stream_);
         break;
       case SolveStep::TRSV:
         rc = uhmk_solve_trsv(F, SF + st.off, st.cnt, st.max_fs, chol ? (factor_nb_ == 64 ? 2 : 1) : 0, st.fwd, st.sb,
@@ -1349,7 +1378,8 @@ void FrontSolver::run_solve_steps(const std::vector<SolveStep>& steps, bool chol
                                   stream_);
         break;
       case SolveStep::BRANCH:
-        rc = uhmk_solve_branch(F, SI + st.off, st.cnt, d_emap_.as<int>(), X, nrhs_, stream_);
+        rc = (st.cols ? uhmk_solve_branch_cols : uhmk_solve_branch)(F, SI + st.off, st.cnt, d_emap_.as<int>(), X,
+                                                                    nrhs_, stream_);
         break;
       case SolveStep::BSTEP:
         rc = uhmk_solve_block_step(F, SI + st.off, st.cnt, st.mode, st.fwd, st.k0, A, st.prec, X, d_z_.as<double>(),

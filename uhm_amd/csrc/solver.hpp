@@ -57,6 +57,7 @@ struct Launch {
   int stream = 0;     // 0: main stream, 1: lookahead stream
   int wait_ev = -1;   // event the launch's stream waits for first (-1: none)
   int record_ev = -1; // event recorded on the launch's stream after it (-1: none)
+  int fuse = 0;       // gemm: items with c == 1 factor the next panel's diagonal block
 };
 constexpr int N_LAUNCH_KINDS = 8;
 
@@ -65,7 +66,8 @@ struct SolveStep {
   enum Kind { GATHER, TRSV, GEMV, BRANCH, CSTEP, BSTEP, BGEMM } kind = GATHER;
   int off = 0, cnt = 0;            // items (gather / gemv / branch) or front list (trsv)
   int max_fs = 0, sb = 0, sb_size = 0;  // trsv
-  int cols = 0;                    // gemv: row chunks (0) or column chunks (1)
+  int cols = 0;                    // gemv: row chunks (0) or column chunks (1); gather / branch:
+                                   // 256-row items (0) or 64-row items x column groups (1)
   int prec = 0;                    // storage of the level's fronts
   int fwd = 0;                     // trsv / cstep: forward (1) or backward (0)
   int k0 = 0;                      // cstep / bstep: first row of the block
@@ -220,6 +222,7 @@ class FrontSolver {
   int block_rhs_ = 8;
   int front_small_ = 64;  // LU fronts with n <= this (at most 64) factor in one whole-front launch
   int lookahead_ = 0;     // Cholesky: next panel's diagonal block and TRSM on the lookahead stream
+  int fuse_diag_ = 1;     // Cholesky: next panel's diagonal block factored by the update's diagonal tile
   hipStream_t side_stream_ = nullptr;
   std::vector<hipEvent_t> la_ev_;
   int final_wait_ev_ = -1;
