@@ -221,11 +221,13 @@ def run_reseeds(U, m, w, args, n):
     return out
 
 
-# HBM bytes per k_gemm launch from rocprofv3 FETCH_SIZE / WRITE_SIZE passes,
-# calibrated against a copy kernel of known size
+# HBM bytes per k_gemm launch (the gemm + schur launches: pure products, not
+# the panel updates that also factor diagonal blocks) from rocprofv3
+# FETCH_SIZE / WRITE_SIZE passes over `bench.py --traffic-probe`, calibrated
+# against a copy of known size
 # (profiles/k_gemm_chol_q4_16cube_hbm_traffic.md): mixed path of the chol
 # workload only; other paths and workloads report null.
-GEMM_TRAFFIC = {("chol", "mixed"): 180.6e6}
+GEMM_TRAFFIC = {("chol", "mixed"): 539.4e6}
 
 
 def gemm_roofline(stats, peak, kernel, traffic=None):
@@ -236,6 +238,30 @@ def gemm_roofline(stats, peak, kernel, traffic=None):
     return {"kernel": kernel, "bound": "mfma", "achieved": round(achieved, 4), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4), "traffic": traffic, "launches": g_n,
             "avg_launch_us": round(g_ms / g_n * 1e3, 3) if g_n else None}
+
+
+def traffic_probe(U, m, w, top):
+    """The program of the HBM-traffic counter passes: a device-to-device
+    copy of 512 MiB (the runtime's copy kernel, 512 MiB read and 512 MiB
+    written) whose counter readings calibrate FETCH_SIZE / WRITE_SIZE, then
+    two mixed-path factorizations; the second one's k_gemm dispatches are
+    the measured ones."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = ctypes.c_size_t(512 << 20)
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), nbytes) != 0:
+            raise SystemExit("hipMalloc failed")
+    if hip.hipMemcpy(dst, src, nbytes, 3) != 0 or hip.hipDeviceSynchronize() != 0:  # device to device
+        raise SystemExit("calibration copy failed")
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    m.set_precision(U.PREC_MIXED, top)
+    factor = m.chol_with_free if w["method"] == 1 else m.lu_piv_with_free
+    for _ in range(2):
+        info = factor()
+    device_sync()
+    log("traffic probe: 2 factorizations, info %d" % info)
 
 
 def main():
@@ -254,6 +280,10 @@ def main():
     ap.add_argument("--bf16", action="store_true",
                     help="also time the BF16-Schur mixed path (BF16 operands, FP32 accumulation in the Schur "
                          "complement products), reported beside the FP32 one")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="counter-pass program instead of the bench: a copy kernel of known size (calibration: "
+                         "512 MiB read, 512 MiB written), then two factorizations of the mixed path (the second "
+                         "is the measured one); run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--tune", action="store_true",
                     help="run the autotuner over the schedule knobs before timing (default: built-in knobs)")
     args = ap.parse_args()
@@ -267,6 +297,9 @@ def main():
     t0 = time.perf_counter()
     m = build_mesh(U, w, seed=3)
     U.set_hier_block_size(args.nb)
+    if args.traffic_probe:
+        traffic_probe(U, m, w, w.get("top", args.top))
+        return
     ndof = m.get_n_dof()
     flops, _, _, _ = m.estimate_cost(w["method"])
     log("mesh %s: ndof %d, fronts %d, %.1f GFLOP, setup %.1f s"

@@ -145,8 +145,9 @@ void uhm_get_timing_(uhm_fort_p* mesh, uhm_fort_double* factor_ms, uhm_fort_doub
  * totals of the last profiled one, per kernel kind in the order assemble,
  * lu_panel, lu_rows, chol_diag, chol_trsm, gemm (panel update), schur
  * (final Schur complement), lu_front (whole-front LU of fronts with
- * n <= 64): milliseconds, algorithmic flops, algorithmic bytes and launch
- * counts (arrays of 8). */
+ * n <= 64), gemm_diag (Cholesky panel update that also factors the next
+ * panel's diagonal blocks): milliseconds, algorithmic flops, algorithmic
+ * bytes and launch counts (arrays of 9). */
 void uhm_set_profile_(uhm_fort_p* mesh, uhm_fort_int* on);
 void uhm_get_kernel_stats_(uhm_fort_p* mesh, uhm_fort_double* ms, uhm_fort_double* flops, uhm_fort_double* bytes,
                            uhm_fort_int* launches);

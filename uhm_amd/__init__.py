@@ -91,7 +91,8 @@ _SIGS = {
     "uhm_get_refine_history_": [_P, _D, _I, _D],
 }
 
-KERNEL_KINDS = ("assemble", "lu_panel", "lu_rows", "chol_diag", "chol_trsm", "gemm", "schur", "lu_front")
+KERNEL_KINDS = ("assemble", "lu_panel", "lu_rows", "chol_diag", "chol_trsm", "gemm", "schur", "lu_front",
+                "gemm_diag")
 _RET = {"uhm_get_last_error_": C.c_int}
 
 

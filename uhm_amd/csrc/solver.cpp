@@ -561,7 +561,7 @@ void FrontSolver::build_schedule(int method) {
             };
             std::stable_partition(items_.begin() + li.off, items_.end(), diag_tile);
             for (long long q = li.off; q < (long long)items_.size() && diag_tile(items_[q]); ++q) items_[q].c = 1;
-            li.fuse = 1;
+            li.kind = Launch::GEMM_DIAG;
             fused_prev = true;
           }
           push_main(li);
@@ -790,10 +790,12 @@ void FrontSolver::build_schedule(int method) {
         }
         break;
       case Launch::GEMM:
+      case Launch::GEMM_DIAG:
       case Launch::SCHUR:
         for (int q = 0; q < l.count; ++q) {
           const uhmk_item& it = items_[l.off + q];
           const int fs = fr_[it.f].fs, n = fr_[it.f].n;
+          if (l.kind == Launch::GEMM_DIAG && it.c == 1) fl += flop_chol(std::min(l.nb, fs - it.a));
           const double K = (l.kind == Launch::SCHUR) ? fs : std::min(l.c0 + l.nb, fs) - l.c0;
           const int i1 = tile_end(it.a, fs, n, l.tile), j1 = tile_end(it.b, fs, n, l.tile);
           double cnt = 0;
@@ -857,10 +859,10 @@ void FrontSolver::enqueue_factor(int method, std::vector<hipEvent_t>* pev) {
         rc = uhmk_chol_trsm(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, st);
         break;
       case Launch::GEMM:
-        if (l.fuse)
-          rc = uhmk_chol_gemm_diag(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, l.mma, info, st);
-        else
-          rc = uhmk_gemm_update(F, IT + l.off, l.count, l.c0, l.nb, 0, l.extra, l.tile, A, l.prec, l.mma, st);
+        rc = uhmk_gemm_update(F, IT + l.off, l.count, l.c0, l.nb, 0, l.extra, l.tile, A, l.prec, l.mma, st);
+        break;
+      case Launch::GEMM_DIAG:
+        rc = uhmk_chol_gemm_diag(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, l.mma, info, st);
         break;
       case Launch::SCHUR:
         rc = uhmk_gemm_update(F, IT + l.off, l.count, 0, 0, 1, l.extra, l.tile, A, l.prec, l.mma, st);

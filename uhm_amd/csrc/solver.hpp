@@ -43,7 +43,9 @@ struct DevBuf {
 
 // One launch of the level schedule.
 struct Launch {
-  enum Kind { ASSEMBLE, LU_PANEL, LU_ROWS, CHOL_DIAG, CHOL_TRSM, GEMM, SCHUR, LU_FRONT } kind;
+  // GEMM_DIAG: a Cholesky panel update whose diagonal tiles then factor the
+  // next panel's diagonal block (items with c == 1)
+  enum Kind { ASSEMBLE, LU_PANEL, LU_ROWS, CHOL_DIAG, CHOL_TRSM, GEMM, SCHUR, LU_FRONT, GEMM_DIAG } kind;
   int c0 = 0, nb = 0;
   long long off = 0;  // offset into the item (or front-list) array
   int count = 0;
@@ -57,9 +59,8 @@ struct Launch {
   int stream = 0;     // 0: main stream, 1: lookahead stream
   int wait_ev = -1;   // event the launch's stream waits for first (-1: none)
   int record_ev = -1; // event recorded on the launch's stream after it (-1: none)
-  int fuse = 0;       // gemm: items with c == 1 factor the next panel's diagonal block
 };
-constexpr int N_LAUNCH_KINDS = 8;
+constexpr int N_LAUNCH_KINDS = 9;
 
 // One launch of the level solves.
 struct SolveStep {
