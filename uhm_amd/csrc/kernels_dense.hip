@@ -336,6 +336,28 @@ __global__ __launch_bounds__(256) void k_chol_trsm(const uhmk_front* __restrict_
 // ------------------------------------------------------------------------
 typedef float f4 __attribute__((ext_vector_type(4)));
 
+// 16 bytes of storage type T (four floats or two doubles) at any address
+// aligned to T: one global_load_dwordx4; t is the aligned register type.
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  static constexpr int N = 4;
+  typedef float t __attribute__((ext_vector_type(4)));
+  typedef float u __attribute__((ext_vector_type(4), aligned(4)));
+};
+template <>
+struct Vec16<double> {
+  static constexpr int N = 2;
+  typedef double t __attribute__((ext_vector_type(2)));
+  typedef double u __attribute__((ext_vector_type(2), aligned(8)));
+};
+// N values of type TM in registers, stored to LDS in one ds_write
+template <typename TM, int N>
+struct VecM {
+  typedef TM t __attribute__((ext_vector_type(N)));
+};
+
 template <typename TM>
 struct Mfma;
 template <>
@@ -404,13 +426,53 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ 
   const int li = tid % T, lk = tid / T;
   // NN B: thread t loads k = t % KC of columns t / KC + BJ u.
   const int bk = tid % KC, bj = tid / KC;
-  auto load = [&](int k0) {
+  // A (and NT B) in 16-byte loads when storage and products have the same
+  // type: VW consecutive rows per load (four floats or two doubles), thread
+  // t rows VW (t % RG) .. +VW-1 of k rows t / RG + VKS u, written to LDS in
+  // one ds_write (FP64 storage with FP32 products measured slower this way
+  // and keeps the element loads)
+  constexpr bool VEC = sizeof(TS) == sizeof(TM);
+  constexpr int VW = Vec16<TS>::N, RG = T / VW, VKS = 256 / RG, NV = KC / VKS;
+  typedef typename Vec16<TS>::t vs_t;
+  typedef typename Vec16<TS>::u vsu_t;
+  typedef typename VecM<TM, VW>::t vm_t;
+  const int vr = VW * (tid % RG), vk = tid / RG;
+  vm_t va[VEC ? NV : 1], vb[VEC ? NV : 1];
+  auto ldv = [&](int r0, int rend, int k) -> vm_t {
+    const TS* p = M + r0 + (long long)k * n;
+    vm_t v;
+    if (k < kend && r0 + VW - 1 < rend) {
+      const vs_t x = *reinterpret_cast<const vsu_t*>(p);
 #pragma unroll
-    for (int u = 0; u < NL; ++u) {
-      const int k = k0 + lk + KR * u;
-      ra[u] = uhmd::ldz_as<TM>(M, (i0 + li) + (long long)k * n, i0 + li < i1 && k < kend);
+      for (int e = 0; e < VW; ++e) v[e] = (TM)x[e];
+      return v;
+    }
+#pragma unroll
+    for (int e = 0; e < VW; ++e) v[e] = (TM)0;
+    if (k < kend) {
+#pragma unroll
+      for (int e = 0; e < VW; ++e)
+        if (r0 + e < rend) v[e] = (TM)p[e];
+    }
+    return v;
+  };
+  auto load = [&](int k0) {
+    if constexpr (VEC) {
+#pragma unroll
+      for (int u = 0; u < NV; ++u) va[u] = ldv(i0 + vr, i1, k0 + vk + VKS * u);
+      if (TRANSB) {
+#pragma unroll
+        for (int u = 0; u < NV; ++u) vb[u] = ldv(j0 + vr, j1, k0 + vk + VKS * u);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < NL; ++u) {
+        const int k = k0 + lk + KR * u;
+        ra[u] = uhmd::ldz_as<TM>(M, (i0 + li) + (long long)k * n, i0 + li < i1 && k < kend);
+      }
     }
     if (TRANSB) {
+      if constexpr (VEC) return;
 #pragma unroll
       for (int u = 0; u < NL; ++u) {
         const int k = k0 + lk + KR * u;
@@ -426,11 +488,21 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ 
   };
   if (kbeg < kend) load(kbeg);
   for (int k0 = kbeg; k0 < kend; k0 += KC) {
+    if constexpr (VEC) {
 #pragma unroll
-    for (int u = 0; u < NL; ++u) sA[(lk + KR * u) * LD + li] = ra[u];
+      for (int u = 0; u < NV; ++u) *reinterpret_cast<vm_t*>(&sA[(vk + VKS * u) * LD + vr]) = va[u];
+    } else {
+#pragma unroll
+      for (int u = 0; u < NL; ++u) sA[(lk + KR * u) * LD + li] = ra[u];
+    }
     if (TRANSB) {
+      if constexpr (VEC) {
 #pragma unroll
-      for (int u = 0; u < NL; ++u) sB[(lk + KR * u) * LD + li] = rb[u];
+        for (int u = 0; u < NV; ++u) *reinterpret_cast<vm_t*>(&sB[(vk + VKS * u) * LD + vr]) = vb[u];
+      } else {
+#pragma unroll
+        for (int u = 0; u < NL; ++u) sB[(lk + KR * u) * LD + li] = rb[u];
+      }
     } else {
 #pragma unroll
       for (int u = 0; u < NL; ++u) sB[bk * LD + bj + BJ * u] = rb[u];
