@@ -554,6 +554,23 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ 
     chol_diag_lds<TS>(S, M + i0 + (long long)j0 * n, n, kb, i0, info, it.f);
     return;
   }
+  if (i1 - i0 == T && j1 - j0 == T) {
+    // whole tile inside the front: no per-entry bounds checks, so the
+    // read-modify-writes issue back to back
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const int i = i0 + iw + a * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int j = j0 + jw + b * 16 + Mfma<TM>::col(r, lane);
+          TS* c = M + i + (long long)j * n;
+          *c = (TS)((double)*c - (double)acc[a][b][r]);
+        }
+      }
+    return;
+  }
 #pragma unroll
   for (int a = 0; a < NB; ++a)
 #pragma unroll
