@@ -323,8 +323,11 @@ __global__ __launch_bounds__(256) void k_chol_trsm(const uhmk_front* __restrict_
 // min(c0+nb, fs)) for a panel update or [0, fs) for the final Schur
 // complement (schur = 1).  Tiles never straddle fs, so the extent is
 // min(i0 + T, fs) or min(i0 + T, n).  Four waves each own a (T/2) x (T/2)
-// quarter; K advances in chunks of 16 through LDS while the next chunk is
-// prefetched into registers.  TS is the storage type of the fronts, TM the
+// quarter; K advances in chunks of 16 (FP64 products) or 32 (FP32 products)
+// through LDS while the next chunk is prefetched into registers.  Workgroups
+// take their items through xcd_item; tiles that lie wholly inside the front
+// update C without per-entry bounds checks.  TS is the storage type of the
+// fronts, TM the
 // MFMA operand type: double -> v_mfma_f64_16x16x4f64, float ->
 // v_mfma_f32_16x16x4f32 (FP32 products and accumulation, operands rounded
 // to FP32 on their way to LDS).
