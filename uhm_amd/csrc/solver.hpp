@@ -176,7 +176,7 @@ class FrontSolver {
     if (g != chol_group_) schedule_method_ = 0;
     chol_group_ = g;
   }
-  int chol_group() const { return chol_group_ ? chol_group_ : (prec_mode_ ? 4 : 8); }
+  int chol_group() const { return chol_group_ ? chol_group_ : 8; }
   void set_profile(bool on) { profile_ = on; }
   // Graph mode: the factorization's launch sequence (and each solve's) is
   // captured once per pattern / method / precision and replayed, for
@@ -218,7 +218,7 @@ class FrontSolver {
   Mesh_* mesh_;
   int nrhs_;
   int nb_ = 64;
-  int chol_group_ = 0;  // 0: by precision (FP64 8, mixed 4, the autotuner's usual picks)
+  int chol_group_ = 0;  // 0: 8 panels (measured best or tied on both paths with fused diagonal blocks)
   int tile_threshold_ = 1024;
   int block_rhs_ = 8;
   int front_small_ = 64;  // LU fronts with n <= this (at most 64) factor in one whole-front launch
