@@ -135,7 +135,8 @@ CASES = [
     ("q3-3d-lu-deep", 2, (6, 6, 5), 3, U.UHM_LU_PIV, 64.0, 0.1, 1, 1.0, 0.0),
     ("varp-2d-lu", 3, (24, 24), 8, U.UHM_LU_PIV, -1.0, 0.1, 0, 0.0, 0.0),
     ("q2-2d-chol", 1, (20, 12), 2, U.UHM_CHOL, 9.0, 0.0, 0, 0.0, 0.0),
-    # fronts above 768 rows: 128-wide GEMM tiles and multi-panel pivoting
+    # fronts above 768 rows: multi-panel pivoting (128-wide GEMM tiles: the
+    # tile-threshold knob in test_schedule_knobs_preserve_results)
     ("q3-3d-chol-8cube", 2, (8, 8, 8), 3, U.UHM_CHOL, 64.0, 0.0, 1, 0.5, 0.0),
     ("q3-3d-lu-8cube", 2, (8, 8, 8), 3, U.UHM_LU_PIV, 64.0, 0.1, 1, 1.0, 0.0),
     # root fronts above 1024 rows: super-block triangular solves

@@ -981,7 +981,7 @@ double FrontSolver::autotune(int method) {
     set_chol_group(bg);
   }
   int bt = tile_threshold_;
-  for (int th : {128, 192, 256, 384, 512, 1024}) {
+  for (int th : {512, 1024, 2048, 4096, 8192}) {
     if (th == bt) continue;
     set_tuning(2, th);
     const double t = time_it();

@@ -219,7 +219,7 @@ class FrontSolver {
   int nrhs_;
   int nb_ = 64;
   int chol_group_ = 0;  // 0: 8 panels (measured best or tied on both paths with fused diagonal blocks)
-  int tile_threshold_ = 1024;
+  int tile_threshold_ = 8192;  // mean tile extent from which GEMM launches use 128 x 128 tiles
   int block_rhs_ = 8;
   int front_small_ = 64;  // LU fronts with n <= this (at most 64) factor in one whole-front launch
   int lookahead_ = 0;     // Cholesky: next panel's diagonal block and TRSM on the lookahead stream
