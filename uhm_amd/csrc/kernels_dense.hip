@@ -263,45 +263,53 @@ __global__ __launch_bounds__(256) void k_chol_trsm(const uhmk_front* __restrict_
   const int kb = min(nb, fs - c0);
   T* __restrict__ A = arena + fr.off;
   __shared__ double sWt[TILE * LDT];  // sWt[k][j] = W(j, k)
-  __shared__ double sXt[TILE * LDT];  // sXt[k][i] = X(i, k)
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int iw = (w & 1) * 32, jw = (w >> 1) * 32, cl = lane & 15, kq = lane >> 4;
   const T* __restrict__ D = A + c0 + (long long)c0 * n;
-  double vw[16], vx[16];
+  double vw[16];
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
     const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
     vw[u] = uhmd::ldz(D, r + (long long)c * n, r < kb && c < kb && r <= c);
-    const int row = it.b + r;
-    vx[u] = uhmd::ldz(A, row + (long long)(c0 + c) * n, row < n && c < kb);
+  }
+  // X straight into the MFMA operand registers (no LDS copy): lane l holds
+  // X(iw + 16 a + l % 16, 4 s + l / 16) for k-step s, 16 consecutive rows
+  // per lane group
+  double xo[2][TILE / 4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int row = it.b + iw + 16 * a + cl;
+#pragma unroll
+    for (int ks = 0; ks < TILE / 4; ++ks) {
+      const int k = 4 * ks + kq;
+      xo[a][ks] = uhmd::ldz(A, row + (long long)(c0 + k) * n, row < n && k < kb);
+    }
   }
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
     const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
     // block entry (r, c), r <= c: W(c, r) (r < c) or L(c, c) (r == c, inverted below)
     sWt[r * LDT + c] = vw[u];
-    sXt[c * LDT + r] = vx[u];
   }
   __syncthreads();
   if (tid < kb) sWt[tid * LDT + tid] = 1.0 / sWt[tid * LDT + tid];
   __syncthreads();
-  const int iw = (w & 1) * 32, jw = (w >> 1) * 32, cl = lane & 15;
   d4 acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int k0 = 0; k0 < TILE; k0 += 4) {
-    const int kr = k0 + (lane >> 4);
-    double wo[2], xo[2];
+  for (int ks = 0; ks < TILE / 4; ++ks) {
+    const int kr = 4 * ks + kq;
+    double wo[2];
 #pragma unroll
     for (int b = 0; b < 2; ++b) wo[b] = sWt[kr * LDT + jw + 16 * b + cl];
 #pragma unroll
-    for (int a = 0; a < 2; ++a) xo[a] = sXt[kr * LDT + iw + 16 * a + cl];
-#pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(wo[b], xo[a], acc[a][b], 0, 0, 0);
+      for (int b = 0; b < 2; ++b)
+        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(wo[b], xo[a][ks], acc[a][b], 0, 0, 0);
   }
   // accumulator register r of lane l: D(j, i) with j = 4 r + l / 16, i = l % 16
 #pragma unroll
