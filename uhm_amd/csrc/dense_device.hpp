@@ -29,6 +29,16 @@ __device__ __forceinline__ TM ldz_as(const T* p, long long idx, bool ok) {
   }
 }
 
+// Item of workgroup b of a launch over n items, the items dealt out to the
+// eight XCDs in contiguous runs (workgroup b runs on XCD b % 8, and XCD x
+// takes items [x n / 8, (x + 1) n / 8) in order): tiles that share panels
+// sit next to each other in the item list, so each XCD's L2 keeps its own
+// panels instead of all eight fetching every panel.  A bijection on [0, n).
+__device__ __forceinline__ int xcd_item(int b, int n) {
+  const int x = b & 7, l = b >> 3, q = n >> 3, r = n & 7;
+  return x < r ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
+}
+
 // Broadcast of a double from lane (l & ~3) + KQ to the four lanes of its
 // quad (DPP quad_perm, no LDS round trip).
 template <int KQ>

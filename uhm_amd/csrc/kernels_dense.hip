@@ -91,16 +91,6 @@ struct Chol16Step<16> {
 // the lower triangle of a front is used); the panel TRSM and the solves
 // read it from there (its diagonal is 1 / L(k,k)).  Padding beyond kb is
 // an identity block, so it never produces a non-positive pivot.
-// Item of workgroup b of a launch over n items, the items dealt out to the
-// eight XCDs in contiguous runs (workgroup b runs on XCD b % 8, and XCD x
-// takes items [x n / 8, (x + 1) n / 8) in order): tiles that share panels
-// sit next to each other in the item list, so each XCD's L2 keeps its own
-// panels instead of all eight fetching every panel.  A bijection on [0, n).
-__device__ __forceinline__ int xcd_item(int b, int n) {
-  const int x = b & 7, l = b >> 3, q = n >> 3, r = n & 7;
-  return x < r ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
-}
-
 // LDS of the diagonal-block factorization.
 struct DiagLds {
   double sL[TILE * LDW];
@@ -408,7 +398,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ 
   constexpr int KC = sizeof(TM) == 4 ? 32 : 16;
   constexpr int NL = KC / KR;         // loads per thread per operand
   constexpr int BJ = 256 / KC;        // NN B: columns per load pass
-  const uhmk_item it = items[xcd_item(blockIdx.x, gridDim.x)];
+  const uhmk_item it = items[uhmd::xcd_item(blockIdx.x, gridDim.x)];
   const uhmk_front fr = F[it.f];
   const int n = fr.n, fs = fr.fs;
   const int kbeg = schur ? 0 : c0;
@@ -645,7 +635,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_bf16(const uhmk_front* __restri
   constexpr int KPT = KC * T / 256;  // A / NT B: consecutive k per thread (16 or 8)
   constexpr int JP = T / 16;         // NN B: columns per thread
   static_assert(KPT % 8 == 0, "A rows are written in groups of eight");
-  const uhmk_item it = items[xcd_item(blockIdx.x, gridDim.x)];
+  const uhmk_item it = items[uhmd::xcd_item(blockIdx.x, gridDim.x)];
   const uhmk_front fr = F[it.f];
   const int n = fr.n, fs = fr.fs;
   const int kbeg = schur ? 0 : c0;
