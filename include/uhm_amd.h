@@ -54,7 +54,10 @@ void uhm_get_hier_block_size_(uhm_fort_int* size);        /* fort.cxx:66  */
  * counter is never incremented (UHM/src/matrix/uhm/common.cxx:44-50). */
 void uhm_reset_flop_(void);                               /* fort.cxx:69  */
 void uhm_get_flop_(uhm_fort_double* flop);                /* fort.cxx:72  */
-/* Device bytes held by the fronts (all fronts stay resident in HBM). */
+/* Device bytes held by the library (front arenas, maps, solve and
+ * refinement buffers of every mesh of the process; all fronts stay resident
+ * in HBM) and their high-water mark, as the reference's process-global
+ * buffer_used (UHM/src/matrix/uhm/common.cxx:42-44). */
 void uhm_get_buffer_used_(uhm_fort_double* buffer);       /* fort.cxx:75  */
 void uhm_get_max_buffer_used_(uhm_fort_double* buffer);   /* fort.cxx:78  */
 void uhm_is_multithreading_enable_(uhm_fort_int* flag);   /* fort.cxx:81  */

@@ -61,6 +61,9 @@ def load():
     lib.orc_solve.argtypes = [C.c_void_p, C.c_int]
     lib.orc_get_front.argtypes = [C.c_void_p, C.c_int, _D]
     lib.orc_get_pivots.argtypes = [C.c_void_p, C.c_int, _I]
+    lib.orc_get_margins.argtypes = [C.c_void_p, C.c_int, _D]
+    lib.orc_get_input.argtypes = [C.c_void_p, C.c_int, _D]
+    lib.orc_assemble.argtypes = [C.c_void_p, C.c_int, _D]
     lib.orc_get_x_global.argtypes = [C.c_void_p, _D, C.c_int]
     lib.orc_flops.restype = C.c_double
     lib.orc_flops.argtypes = [C.c_void_p, C.c_int]
@@ -206,6 +209,33 @@ class OracleMesh:
         out = np.zeros(max(1, fs), np.int32)
         _lib.orc_get_pivots(self.h, e, _ip(out))
         return out[:fs].copy()
+
+    def margins(self, e):
+        """Relative margin (|best| - |next|) / |best| of every pivot choice
+        of the last LU factorization of front e."""
+        fs = self.element(e)["fs"]
+        out = np.zeros(max(1, fs), np.float64)
+        _lib.orc_get_margins(self.h, e, _dp(out))
+        return out[:fs].copy()
+
+    def _square(self, e):
+        el = self.element(e)
+        n = el["fs"] + el["ss"]
+        return np.zeros((n, n), np.float64, order="F")
+
+    def input(self, e):
+        """Assembly input of front e (its copy-in, front order)."""
+        out = self._square(e)
+        _lib.orc_get_input(self.h, e, _dp(out))
+        return out
+
+    def assemble(self, e):
+        """Extend-add of front e (merge_A: 0 + own input + children's ABR in
+        children order), without factoring it; its children must have been
+        factored."""
+        out = self._square(e)
+        _lib.orc_assemble(self.h, e, _dp(out))
+        return out
 
     def x_global(self, ndof):
         out = np.zeros((ndof, self.nrhs), np.float64, order="F")

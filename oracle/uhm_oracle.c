@@ -78,6 +78,7 @@ typedef struct {
   int** piv;
   double** X;   /* n x nrhs */
   double** B;   /* n x nrhs (leaf right-hand sides) */
+  double** marg; /* per front, fs relative pivot margins (LU): (|best| - |second|) / |best| */
   int info;
 } orc;
 
@@ -311,6 +312,7 @@ void orc_free(void* h) {
     if (o->piv) free(o->piv[e]);
     if (o->X) free(o->X[e]);
     if (o->B) free(o->B[e]);
+    if (o->marg) free(o->marg[e]);
   }
   for (int i = 0; i < o->n_nodes; ++i) free(o->owner[i]);
   free(o->nodes);
@@ -331,6 +333,7 @@ void orc_free(void* h) {
   free(o->lvl_elem);
   free(o->F);
   free(o->K);
+  free(o->marg);
   free(o->piv);
   free(o->X);
   free(o->B);
@@ -421,18 +424,27 @@ static void alloc_numeric(orc* o, int nrhs) {
   o->piv = (int**)calloc(o->n_elem, sizeof(int*));
   o->X = (double**)calloc(o->n_elem, sizeof(double*));
   o->B = (double**)calloc(o->n_elem, sizeof(double*));
+  o->marg = (double**)calloc(o->n_elem, sizeof(double*));
 }
 
 /* Per-front storage is allocated on first use, so a sample of levels of a
- * large mesh fits in host memory. */
+ * large mesh fits in host memory.  The assembly input K exists only for
+ * fronts that receive a copy-in (the leaves); the others start from zero. */
 static void ensure(orc* o, int e) {
   if (o->F[e]) return;
   size_t n = (size_t)(o->fs[e] + o->ss[e]);
-  o->K[e] = (double*)calloc(n * n + 1, sizeof(double));
   o->piv[e] = (int*)calloc(o->fs[e] + 1, sizeof(int));
+  o->marg[e] = (double*)calloc(o->fs[e] + 1, sizeof(double));
   o->X[e] = (double*)calloc(n * o->nrhs + 1, sizeof(double));
   o->B[e] = (double*)calloc(n * o->nrhs + 1, sizeof(double));
   o->F[e] = (double*)calloc(n * n + 1, sizeof(double));
+}
+
+static void ensure_input(orc* o, int e) {
+  ensure(o, e);
+  if (o->K[e]) return;
+  size_t n = (size_t)(o->fs[e] + o->ss[e]);
+  o->K[e] = (double*)calloc(n * n + 1, sizeof(double));
 }
 
 void orc_init_numeric(void* h, int nrhs) { alloc_numeric((orc*)h, nrhs); }
@@ -463,6 +475,7 @@ int orc_copy_in(void* h, int e, int nn, const int* nods, int side, int m, const 
   ensure(o, e);
   int n = o->fs[e] + o->ss[e];
   if (side == 1) {
+    ensure_input(o, e);
     for (int j = 0; j < m; ++j)
       for (int i = 0; i < m; ++i) o->K[e][map[i] + (size_t)map[j] * n] += buf[i + (size_t)j * m];
   } else {
@@ -477,7 +490,7 @@ int orc_copy_in(void* h, int e, int nn, const int* nods, int side, int m, const 
 void orc_set_front(void* h, int e, const double* F) {
   orc* o = (orc*)h;
   size_t n = (size_t)(o->fs[e] + o->ss[e]);
-  ensure(o, e);
+  ensure_input(o, e);
   memcpy(o->K[e], F, n * n * sizeof(double));
 }
 void orc_set_rhs_front(void* h, int e, const double* b) {
@@ -486,74 +499,123 @@ void orc_set_rhs_front(void* h, int e, const double* b) {
   ensure(o, e);
   memcpy(o->B[e], b, n * o->nrhs * sizeof(double));
 }
+/* the assembly input of front e (n x n, front order; zero without copy-in) */
+void orc_get_input(void* h, int e, double* out) {
+  orc* o = (orc*)h;
+  size_t n = (size_t)(o->fs[e] + o->ss[e]);
+  ensure(o, e);
+  if (o->K[e]) memcpy(out, o->K[e], n * n * sizeof(double));
+  else memset(out, 0, n * n * sizeof(double));
+}
 
-/* ---- dense kernels on column-major views (ld = front size) */
+/* ---- dense kernels on column-major views (ld = front size)
+ * Every entry accumulates its updates in the order of the textbook loops
+ * (ascending inner index p), whatever the blocking and the thread count, so
+ * a result does not depend on either.  par: OpenMP inside the kernel (one
+ * front at a time); fronts processed in parallel pass 0. */
 #define AT(A, ld, i, j) ((A)[(size_t)(i) + (size_t)(j) * (ld)])
+#define KB_ 64   /* inner-index block */
+#define IB_ 256  /* row block */
+static inline int imin(int a, int b) { return a < b ? a : b; }
+
+/* c[i0..ie) -= sum over p in [p0, pe) of a_p[i] * b[p], p ascending */
+static inline void axpy_block(double* c, int i0, int ie, const double* A, int lda, int p0, int pe, const double* b,
+                              int bstride) {
+  int p = p0;
+  for (; p + 4 <= pe; p += 4) {
+    const double b0 = b[(size_t)p * bstride], b1 = b[(size_t)(p + 1) * bstride];
+    const double b2 = b[(size_t)(p + 2) * bstride], b3 = b[(size_t)(p + 3) * bstride];
+    const double* a0 = A + (size_t)p * lda;
+    const double* a1 = a0 + lda;
+    const double* a2 = a1 + lda;
+    const double* a3 = a2 + lda;
+    for (int i = i0; i < ie; ++i) {
+      double t = c[i];
+      t -= a0[i] * b0;
+      t -= a1[i] * b1;
+      t -= a2[i] * b2;
+      t -= a3[i] * b3;
+      c[i] = t;
+    }
+  }
+  for (; p < pe; ++p) {
+    const double bp = b[(size_t)p * bstride];
+    const double* a = A + (size_t)p * lda;
+    for (int i = i0; i < ie; ++i) c[i] -= a[i] * bp;
+  }
+}
 
 /* C[m x n] -= A[m x k] * B[k x n] */
-static void gemm_sub(int m, int n, int k, const double* A, int lda, const double* B, int ldb, double* Cm, int ldc) {
-  for (int j = 0; j < n; ++j)
-    for (int p = 0; p < k; ++p) {
-      const double b = AT(B, ldb, p, j);
-      if (b == 0.0) continue;
-      const double* a = A + (size_t)p * lda;
-      double* c = Cm + (size_t)j * ldc;
-      for (int i = 0; i < m; ++i) c[i] -= a[i] * b;
-    }
+static void gemm_sub(int m, int n, int k, const double* A, int lda, const double* B, int ldb, double* Cm, int ldc,
+                     int par) {
+  for (int p0 = 0; p0 < k; p0 += KB_) {
+    const int pe = imin(p0 + KB_, k);
+#pragma omp parallel for schedule(static) if (par)
+    for (int j = 0; j < n; ++j)
+      for (int i0 = 0; i0 < m; i0 += IB_)
+        axpy_block(Cm + (size_t)j * ldc, i0, imin(i0 + IB_, m), A, lda, p0, pe, B + (size_t)j * ldb, 1);
+  }
 }
 
 /* C[m x n] -= A[m x k] * A2[n x k]^T, lower part only when lower */
 static void gemm_nt_sub(int m, int n, int k, const double* A, int lda, const double* A2, int ld2, double* Cm, int ldc,
-                        int lower) {
-  for (int j = 0; j < n; ++j)
-    for (int p = 0; p < k; ++p) {
-      const double b = AT(A2, ld2, j, p);
-      if (b == 0.0) continue;
-      const double* a = A + (size_t)p * lda;
-      double* c = Cm + (size_t)j * ldc;
-      for (int i = lower ? j : 0; i < m; ++i) c[i] -= a[i] * b;
-    }
-}
-
-/* B = L^-1 B, L unit lower k x k */
-static void trsm_llnu(int k, int n, const double* L, int ldl, double* B, int ldb) {
-  for (int j = 0; j < n; ++j) {
-    double* b = B + (size_t)j * ldb;
-    for (int p = 0; p < k; ++p) {
-      const double bp = b[p];
-      if (bp == 0.0) continue;
-      for (int i = p + 1; i < k; ++i) b[i] -= AT(L, ldl, i, p) * bp;
+                        int lower, int par) {
+  for (int p0 = 0; p0 < k; p0 += KB_) {
+    const int pe = imin(p0 + KB_, k);
+#pragma omp parallel for schedule(dynamic, 8) if (par)
+    for (int j = 0; j < n; ++j) {
+      const int ib = lower ? j : 0;
+      for (int i0 = ib; i0 < m; i0 += IB_)
+        axpy_block(Cm + (size_t)j * ldc, i0, imin(i0 + IB_, m), A, lda, p0, pe, A2 + j, ld2);
     }
   }
 }
 
-/* B = B * U^-1, U upper non-unit k x k, B m x k */
-static void trsm_runn(int m, int k, const double* U, int ldu, double* B, int ldb) {
-  for (int j = 0; j < k; ++j) {
-    double* bj = B + (size_t)j * ldb;
-    for (int p = 0; p < j; ++p) {
-      const double u = AT(U, ldu, p, j);
-      if (u == 0.0) continue;
-      const double* bp = B + (size_t)p * ldb;
-      for (int i = 0; i < m; ++i) bj[i] -= bp[i] * u;
+/* B = L^-1 B, L unit lower k x k: column j runs p = 0 .. k-1,
+ * b[i] -= L[i, p] b[p] for i > p; blocked over p (diagonal block, then the
+ * rows below it), which keeps that order for every entry. */
+static void trsm_llnu(int k, int n, const double* L, int ldl, double* B, int ldb, int par) {
+  for (int p0 = 0; p0 < k; p0 += KB_) {
+    const int pe = imin(p0 + KB_, k);
+#pragma omp parallel for schedule(static) if (par)
+    for (int j = 0; j < n; ++j) {
+      double* b = B + (size_t)j * ldb;
+      for (int p = p0; p < pe; ++p) {
+        const double bp = b[p];
+        for (int i = p + 1; i < pe; ++i) b[i] -= AT(L, ldl, i, p) * bp;
+      }
+      for (int i0 = pe; i0 < k; i0 += IB_) axpy_block(b, i0, imin(i0 + IB_, k), L, ldl, p0, pe, b, 1);
     }
-    const double d = AT(U, ldu, j, j);
-    for (int i = 0; i < m; ++i) bj[i] /= d;
   }
 }
 
-/* B = B * L^-T, L lower non-unit k x k, B m x k */
-static void trsm_rltn(int m, int k, const double* L, int ldl, double* B, int ldb) {
-  for (int j = 0; j < k; ++j) {
-    double* bj = B + (size_t)j * ldb;
-    for (int p = 0; p < j; ++p) {
-      const double l = AT(L, ldl, j, p);
-      if (l == 0.0) continue;
-      const double* bp = B + (size_t)p * ldb;
-      for (int i = 0; i < m; ++i) bj[i] -= bp[i] * l;
+/* B = B * U^-1, U upper non-unit k x k, B m x k: row by row, column j gets
+ * b[i, j] -= b[i, p] U[p, j] for p < j ascending, then / U[j, j]. */
+static void trsm_runn(int m, int k, const double* U, int ldu, double* B, int ldb, int par) {
+#pragma omp parallel for schedule(static) if (par)
+  for (int i0 = 0; i0 < m; i0 += 64) {
+    const int ie = imin(i0 + 64, m);
+    for (int j = 0; j < k; ++j) {
+      double* bj = B + (size_t)j * ldb;
+      axpy_block(bj, i0, ie, B, ldb, 0, j, U + (size_t)j * ldu, 1);
+      const double d = AT(U, ldu, j, j);
+      for (int i = i0; i < ie; ++i) bj[i] /= d;
     }
-    const double d = AT(L, ldl, j, j);
-    for (int i = 0; i < m; ++i) bj[i] /= d;
+  }
+}
+
+/* B = B * L^-T, L lower non-unit k x k, B m x k (as trsm_runn with
+ * U[p, j] = L[j, p]) */
+static void trsm_rltn(int m, int k, const double* L, int ldl, double* B, int ldb, int par) {
+#pragma omp parallel for schedule(static) if (par)
+  for (int i0 = 0; i0 < m; i0 += 64) {
+    const int ie = imin(i0 + 64, m);
+    for (int j = 0; j < k; ++j) {
+      double* bj = B + (size_t)j * ldb;
+      axpy_block(bj, i0, ie, B, ldb, 0, j, L + j, ldl);
+      const double d = AT(L, ldl, j, j);
+      for (int i = i0; i < ie; ++i) bj[i] /= d;
+    }
   }
 }
 
@@ -569,20 +631,26 @@ static void swap_rows(double* A, int ld, int r1, int r2, int c0, int c1) {
 
 /* Unblocked GEPP on the panel rows [k0, m) x cols [k0, k0+kb) of A (leading
  * dimension ld), swaps applied inside the panel columns only; relative
- * pivots p[k] = pivot row - k.  Returns 1 + first zero-pivot row or 0. */
-static int getf2_panel(double* A, int ld, int k0, int kb, int m, int* p) {
+ * pivots p[k] = pivot row - k; marg[k] (if not null) = relative margin of
+ * the chosen pivot over the next largest candidate.  Returns 1 + first
+ * zero-pivot row or 0. */
+static int getf2_panel(double* A, int ld, int k0, int kb, int m, int* p, double* marg, int par) {
   int info = 0;
   for (int j = k0; j < k0 + kb; ++j) {
     int pr = j;
-    double best = -1.0;
+    double best = -1.0, second = 0.0;
     for (int i = j; i < m; ++i) {
       double a = fabs(AT(A, ld, i, j));
       if (a > best) {
+        if (best > second) second = best;
         best = a;
         pr = i;
+      } else if (a > second) {
+        second = a;
       }
     }
     p[j] = pr - j;
+    if (marg) marg[j] = best > 0.0 ? (best - second) / best : 0.0;
     swap_rows(A, ld, j, pr, k0, k0 + kb);
     const double d = AT(A, ld, j, j);
     if (d != 0.0) {
@@ -590,6 +658,7 @@ static int getf2_panel(double* A, int ld, int k0, int kb, int m, int* p) {
     } else if (!info) {
       info = j + 1;
     }
+#pragma omp parallel for schedule(static) if (par && (m - j) * (k0 + kb - j) > 32768)
     for (int c = j + 1; c < k0 + kb; ++c) {
       const double u = AT(A, ld, j, c);
       if (u == 0.0) continue;
@@ -601,34 +670,34 @@ static int getf2_panel(double* A, int ld, int k0, int kb, int m, int* p) {
 
 /* lu_piv_hier: blocked LU of ATL with tile b (pivots inside ATL), then
  * ATR <- L^-1 P ATR, ABL <- ABL U^-1, ABR -= ABL ATR. */
-static int front_lu(double* F, int n, int fs, int b, int* p) {
+static int front_lu(double* F, int n, int fs, int b, int* p, double* marg, int par) {
   int info = 0;
   for (int k0 = 0; k0 < fs; k0 += b) {
     int kb = fs - k0 < b ? fs - k0 : b;
-    int r = getf2_panel(F, n, k0, kb, fs, p);
+    int r = getf2_panel(F, n, k0, kb, fs, p, marg, par);
     if (r && !info) info = r;
     for (int j = k0; j < k0 + kb; ++j) {
       int pr = j + p[j];
       swap_rows(F, n, j, pr, 0, k0);        /* apply_pivots on [A10; A20] */
       swap_rows(F, n, j, pr, k0 + kb, fs);  /* apply_pivots on [A12; A22] */
     }
-    trsm_llnu(kb, fs - k0 - kb, &AT(F, n, k0, k0), n, &AT(F, n, k0, k0 + kb), n);
+    trsm_llnu(kb, fs - k0 - kb, &AT(F, n, k0, k0), n, &AT(F, n, k0, k0 + kb), n, par);
     gemm_sub(fs - k0 - kb, fs - k0 - kb, kb, &AT(F, n, k0 + kb, k0), n, &AT(F, n, k0, k0 + kb), n,
-             &AT(F, n, k0 + kb, k0 + kb), n);
+             &AT(F, n, k0 + kb, k0 + kb), n, par);
   }
   const int ss = n - fs;
   if (fs && ss) {
     for (int j = 0; j < fs; ++j) swap_rows(F, n, j, j + p[j], fs, n); /* apply_pivots(p, ATR) */
-    trsm_llnu(fs, ss, F, n, &AT(F, n, 0, fs), n);
-    trsm_runn(ss, fs, F, n, &AT(F, n, fs, 0), n);
-    gemm_sub(ss, ss, fs, &AT(F, n, fs, 0), n, &AT(F, n, 0, fs), n, &AT(F, n, fs, fs), n);
+    trsm_llnu(fs, ss, F, n, &AT(F, n, 0, fs), n, par);
+    trsm_runn(ss, fs, F, n, &AT(F, n, fs, 0), n, par);
+    gemm_sub(ss, ss, fs, &AT(F, n, fs, 0), n, &AT(F, n, 0, fs), n, &AT(F, n, fs, fs), n, par);
   }
   return info;
 }
 
 /* chol_hier: blocked Cholesky of ATL (lower) with tile b, then
  * ABL <- ABL L^-T, ABR -= ABL ABL^T (lower). */
-static int front_chol(double* F, int n, int fs, int b) {
+static int front_chol(double* F, int n, int fs, int b, int par) {
   int info = 0;
   for (int k0 = 0; k0 < fs; k0 += b) {
     int kb = fs - k0 < b ? fs - k0 : b;
@@ -644,27 +713,32 @@ static int front_chol(double* F, int n, int fs, int b) {
       }
     }
     int m2 = fs - k0 - kb;
-    trsm_rltn(m2, kb, &AT(F, n, k0, k0), n, &AT(F, n, k0 + kb, k0), n);
-    gemm_nt_sub(m2, m2, kb, &AT(F, n, k0 + kb, k0), n, &AT(F, n, k0 + kb, k0), n, &AT(F, n, k0 + kb, k0 + kb), n, 1);
+    trsm_rltn(m2, kb, &AT(F, n, k0, k0), n, &AT(F, n, k0 + kb, k0), n, par);
+    gemm_nt_sub(m2, m2, kb, &AT(F, n, k0 + kb, k0), n, &AT(F, n, k0 + kb, k0), n, &AT(F, n, k0 + kb, k0 + kb), n, 1,
+                par);
   }
   const int ss = n - fs;
   if (fs && ss) {
-    trsm_rltn(ss, fs, F, n, &AT(F, n, fs, 0), n);
-    gemm_nt_sub(ss, ss, fs, &AT(F, n, fs, 0), n, &AT(F, n, fs, 0), n, &AT(F, n, fs, fs), n, 1);
+    trsm_rltn(ss, fs, F, n, &AT(F, n, fs, 0), n, par);
+    gemm_nt_sub(ss, ss, fs, &AT(F, n, fs, 0), n, &AT(F, n, fs, 0), n, &AT(F, n, fs, fs), n, 1, par);
   }
   return info;
 }
 
 /* op_merge(A): parent (zeroed, or its own copy-in) += children's ABR in
  * children order, through the mapper runs (helper.hxx:191-222). */
-static void merge_A(orc* o, int e) {
+static void merge_A(orc* o, int e, int par) {
   const int n = o->fs[e] + o->ss[e];
-  memcpy(o->F[e], o->K[e], (size_t)n * n * sizeof(double));
+  if (o->K[e]) memcpy(o->F[e], o->K[e], (size_t)n * n * sizeof(double));
+  else memset(o->F[e], 0, (size_t)n * n * sizeof(double));
   for (int k = o->cptr[e]; k < o->cptr[e + 1]; ++k) {
     int c = o->cidx[k];
     int nc = o->fs[c] + o->ss[c], fc = o->fs[c];
     int* runs = (int*)malloc((4 * o->nsch[c] + 4) * sizeof(int));
     int nr = orc_mapper(o, c, runs);
+    /* column runs are independent: each parent entry still receives the
+     * children in children order */
+#pragma omp parallel for schedule(dynamic, 1) if (par)
     for (int jr = 0; jr < nr; ++jr)
       for (int ir = 0; ir < nr; ++ir) {
         int ioc = runs[4 * ir], jo = runs[4 * jr];
@@ -678,21 +752,39 @@ static void merge_A(orc* o, int e) {
   }
 }
 
+/* Merged input of front e (0 + own copy-in + children's ABR, merge order)
+ * into out, without factoring; its children must have been factored. */
+void orc_assemble(void* h, int e, double* out) {
+  orc* o = (orc*)h;
+  ensure(o, e);
+  merge_A(o, e, 1);
+  size_t n = (size_t)(o->fs[e] + o->ss[e]);
+  memcpy(out, o->F[e], n * n * sizeof(double));
+}
+
 /* Levels [L0, L1] (leaves are level 0); every child of those fronts must
- * have been factored. */
+ * have been factored.  A level with at least as many fronts as threads
+ * runs its fronts in parallel; a smaller one runs them one after another,
+ * each with OpenMP inside (the results are the same bits either way). */
 int orc_factor_levels(void* h, int method, int b, int L0, int L1) {
   orc* o = (orc*)h;
   o->info = 0;
+  int nth = 1;
+#ifdef _OPENMP
+  nth = omp_get_max_threads();
+#endif
   for (int L = L0; L <= L1 && L < o->n_levels; ++L) {
     const int lo = o->lvl_ptr[L], hi = o->lvl_ptr[L + 1];
     for (int t = lo; t < hi; ++t) ensure(o, o->lvl_elem[t]);
+    const int outer = (hi - lo) >= nth;
     int first = hi;
-#pragma omp parallel for schedule(dynamic, 1) reduction(min : first)
+#pragma omp parallel for schedule(dynamic, 1) reduction(min : first) if (outer)
     for (int t = lo; t < hi; ++t) {
       int e = o->lvl_elem[t];
-      merge_A(o, e);
+      merge_A(o, e, !outer);
       int n = o->fs[e] + o->ss[e];
-      int r = (method == 1) ? front_chol(o->F[e], n, o->fs[e], b) : front_lu(o->F[e], n, o->fs[e], b, o->piv[e]);
+      int r = (method == 1) ? front_chol(o->F[e], n, o->fs[e], b, !outer)
+                            : front_lu(o->F[e], n, o->fs[e], b, o->piv[e], o->marg[e], !outer);
       if (r && t < first) first = t;
     }
     if (first < hi && !o->info) o->info = o->lvl_elem[first] + 1;
@@ -705,9 +797,14 @@ int orc_factor(void* h, int method, int b) { return orc_factor_levels(h, method,
 /* Forward (leaf -> root) then backward (root -> leaf) solves. */
 void orc_solve(void* h, int method) {
   orc* o = (orc*)h;
+  int nth = 1;
+#ifdef _OPENMP
+  nth = omp_get_max_threads();
+#endif
   for (int e = 0; e < o->n_elem; ++e) ensure(o, e);
   for (int L = 0; L < o->n_levels; ++L) {
-#pragma omp parallel for schedule(dynamic, 1)
+    const int outer = (o->lvl_ptr[L + 1] - o->lvl_ptr[L]) >= nth;
+#pragma omp parallel for schedule(dynamic, 1) if (outer)
     for (int t = o->lvl_ptr[L]; t < o->lvl_ptr[L + 1]; ++t) {
       int e = o->lvl_elem[t];
       int n = o->fs[e] + o->ss[e], fs = o->fs[e];
@@ -726,6 +823,7 @@ void orc_solve(void* h, int method) {
         free(runs);
       }
       const double* F = o->F[e];
+#pragma omp parallel for schedule(static) if (!outer)
       for (int r = 0; r < o->nrhs; ++r) {
         double* xr = x + (size_t)r * n;
         if (method != 1)
@@ -744,7 +842,8 @@ void orc_solve(void* h, int method) {
     }
   }
   for (int L = o->n_levels - 1; L >= 0; --L) {
-#pragma omp parallel for schedule(dynamic, 1)
+    const int outer = (o->lvl_ptr[L + 1] - o->lvl_ptr[L]) >= nth;
+#pragma omp parallel for schedule(dynamic, 1) if (outer)
     for (int t = o->lvl_ptr[L]; t < o->lvl_ptr[L + 1]; ++t) {
       int e = o->lvl_elem[t];
       int n = o->fs[e] + o->ss[e], fs = o->fs[e];
@@ -763,6 +862,7 @@ void orc_solve(void* h, int method) {
         free(runs);
       }
       const double* F = o->F[e];
+#pragma omp parallel for schedule(static) if (!outer)
       for (int r = 0; r < o->nrhs; ++r) {
         double* xr = x + (size_t)r * n;
         if (method == 1) {
@@ -802,6 +902,12 @@ void orc_get_pivots(void* h, int e, int* out) {
   orc* o = (orc*)h;
   ensure(o, e);
   memcpy(out, o->piv[e], o->fs[e] * sizeof(int));
+}
+/* relative pivot margins of the last LU factorization of front e */
+void orc_get_margins(void* h, int e, double* out) {
+  orc* o = (orc*)h;
+  ensure(o, e);
+  memcpy(out, o->marg[e], o->fs[e] * sizeof(double));
 }
 /* solution in the global numbering: x[offset(node) + d] */
 void orc_get_x_global(void* h, double* out, int ndof) {
@@ -872,5 +978,5 @@ int orc_max_threads(void) {
 }
 
 /* Dense single-front entry points for the golden-vector tests. */
-int orc_dense_lu(double* F, int n, int fs, int b, int* p) { return front_lu(F, n, fs, b, p); }
-int orc_dense_chol(double* F, int n, int fs, int b) { return front_chol(F, n, fs, b); }
+int orc_dense_lu(double* F, int n, int fs, int b, int* p) { return front_lu(F, n, fs, b, p, NULL, 1); }
+int orc_dense_chol(double* F, int n, int fs, int b) { return front_chol(F, n, fs, b, 1); }

@@ -69,6 +69,35 @@ def test_graph_replays_match_eager(lib, method):
         assert np.array_equal(x2, x3)
 
 
+@pytest.mark.parametrize("method", [U.UHM_CHOL, U.UHM_LU_PIV])
+def test_graph_replays_new_values_without_recapture(lib, method):
+    """Re-factorization on a fixed pattern: graph mode stays on while the
+    element values are redrawn, so the factorization and solve graphs
+    captured on the first seed are replayed on the new values.  Every reseed
+    matches, bit for bit, an eager run of a second mesh built with the same
+    seed, and its refined solution reaches backward error 1e-12."""
+    chol = method == U.UHM_CHOL
+    beta = 0.0 if chol else 0.1
+    m = _mesh(method, 5, nrhs=2)
+    m.set_precision(U.PREC_MIXED, 1)
+    m.set_graph(True)
+    _run(m, method)                      # captures the graphs on seed 5
+    for seed in (9, 10, 11):
+        say("case graph reseed method", method, "seed", seed)
+        m.set_random_params(seed=seed, alpha=64.0, beta=beta, scale_kind=1, p0=0.5)
+        m.random_matrix()
+        m.set_rhs()
+        F1, x1 = _run(m, method)         # replayed graphs
+        ref = _mesh(method, seed, nrhs=2)
+        ref.set_precision(U.PREC_MIXED, 1)
+        F0, x0 = _run(ref, method)       # eager
+        for f in range(m.n_fronts()):
+            assert np.array_equal(F0[f], F1[f]), (seed, f)
+        assert np.array_equal(x0, x1), seed
+        m.improve(method, 1e-12, 30)
+        assert m.backward_error() <= 1e-12, (seed, m.backward_error())
+
+
 def test_graph_mixed_refinement(lib):
     m = _mesh(U.UHM_CHOL, 6)
     m.set_precision(U.PREC_MIXED, 1)

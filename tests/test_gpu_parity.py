@@ -384,13 +384,13 @@ def test_cholesky_lookahead_matches_in_order(lib, prec, group):
         assert m.backward_error() <= ETA_TOL
 
 
-def _chol_mesh(seed, alpha):
+def _chol_mesh(seed, alpha, beta=0.0):
     m = U.Mesh()
     m.build_synthetic(2, (8, 8, 7), 3, 0)
     m.lock()
     m.create_matrix_without_buffer(U.UHM_REAL, 1)
     m.create_matrix_buffer()
-    m.set_random_params(seed=seed, alpha=alpha, scale_kind=1, p0=0.5)
+    m.set_random_params(seed=seed, alpha=alpha, beta=beta, scale_kind=1, p0=0.5)
     m.random_matrix()
     m.set_rhs()
     return m
@@ -430,10 +430,13 @@ def test_cholesky_fused_diagonal_same_bits(lib, prec):
 
 def test_cholesky_fused_diagonal_reports_failure(lib):
     """A matrix that is not positive definite fails with the same element
-    reported whether or not the diagonal blocks are fused."""
+    reported whether or not the diagonal blocks are fused.  Element matrices
+    M^T M + 50 S (no shift): the Cholesky path reads their lower triangle,
+    i.e. the symmetric matrix whose strict lower part carries 50 S, whose
+    eigenvalues (~ +-50 sqrt(2 m)) swamp those of M^T M (<= ~4 m) for m = 64."""
     got = []
     for fuse in (0, 1):
-        m = _chol_mesh(23, -4.0)
+        m = _chol_mesh(23, 0.0, 50.0)
         m.set_tuning(6, fuse)
         got.append(m.chol_with_free())
     say("  info unfused %d fused %d" % tuple(got))

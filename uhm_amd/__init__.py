@@ -486,3 +486,17 @@ def get_hier_block_size():
     r = C.c_int(0)
     _call("uhm_get_hier_block_size_", C.byref(r))
     return r.value
+
+
+def buffer_used():
+    """Device bytes the library holds now (every mesh of the process)."""
+    r = C.c_double(0)
+    _call("uhm_get_buffer_used_", C.byref(r))
+    return r.value
+
+
+def max_buffer_used():
+    """High-water mark of buffer_used()."""
+    r = C.c_double(0)
+    _call("uhm_get_max_buffer_used_", C.byref(r))
+    return r.value

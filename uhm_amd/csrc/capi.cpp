@@ -88,8 +88,8 @@ void uhm_set_hier_block_size_(uhm_fort_int* b) {
 void uhm_get_hier_block_size_(uhm_fort_int* b) { *b = g_block; }
 void uhm_reset_flop_(void) { g_flop = 0.0; }
 void uhm_get_flop_(uhm_fort_double* f) { *f = g_flop; }
-void uhm_get_buffer_used_(uhm_fort_double* b) { *b = 0.0; }
-void uhm_get_max_buffer_used_(uhm_fort_double* b) { *b = 0.0; }
+void uhm_get_buffer_used_(uhm_fort_double* b) { *b = (double)uhm::device_bytes_used(); }
+void uhm_get_max_buffer_used_(uhm_fort_double* b) { *b = (double)uhm::device_bytes_max(); }
 void uhm_is_multithreading_enable_(uhm_fort_int* f) { *f = 1; }
 void uhm_is_hier_matrix_enable_(uhm_fort_int* f) { *f = 1; }
 void uhm_timer_(uhm_fort_double* t) {

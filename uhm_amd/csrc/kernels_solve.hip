@@ -734,11 +734,13 @@ __global__ __launch_bounds__(256) void k_gen_elements(const uhmk_front* __restri
     double s = 1.0;
     if (scale_kind == 1) s = exp(p0 * rng_normal(seed, eid, 2u, 0u));
     else if (scale_kind == 2) s = pow(10.0, p0 + (p1 - p0) * rng_uniform(seed, eid, 2u, 0u));
+    // alpha < 0: the shift is |alpha| times the element size m
+    const double shift = alpha < 0.0 ? -alpha * (double)m : alpha;
     for (int idx = tid; idx < m * m; idx += 256) {
       const int i = idx % m, j = idx / m;
       double acc = 0.0;
       for (int k = 0; k < m; ++k) acc += sM[k + i * ldm] * sM[k + j * ldm];
-      if (i == j) acc += alpha;
+      if (i == j) acc += shift;
       if (beta != 0.0) acc += beta * rng_normal(seed, eid, 1u, (unsigned)idx);
       stage[fr.soff + mp[i] + (long long)mp[j] * n] = s * acc;
     }

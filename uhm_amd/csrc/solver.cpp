@@ -5,6 +5,7 @@
 #include <chrono>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -15,8 +16,19 @@
 namespace uhm {
 
 // ------------------------------------------------------------------ DevBuf
+// Device bytes held by every DevBuf of the process and their high-water
+// mark: the accounting of the reference's process-global buffer_used
+// (UHM/src/matrix/uhm/common.cxx:42-44), kept with atomics.
+static std::atomic<long long> g_dev_bytes{0}, g_dev_bytes_max{0};
+
+long long device_bytes_used() { return g_dev_bytes.load(); }
+long long device_bytes_max() { return g_dev_bytes_max.load(); }
+
 void DevBuf::release() {
-  if (p) (void)hipFree(p);
+  if (p) {
+    (void)hipFree(p);
+    g_dev_bytes -= (long long)bytes;
+  }
   p = nullptr;
   bytes = 0;
 }
@@ -31,6 +43,14 @@ void DevBuf::alloc(size_t b) {
                              hipGetErrorString(e));
   }
   bytes = b;
+  const long long now = (g_dev_bytes += (long long)b);
+  long long prev = g_dev_bytes_max.load();
+  while (now > prev && !g_dev_bytes_max.compare_exchange_weak(prev, now)) {
+  }
+}
+
+void DevBuf::reserve(size_t b) {
+  if (!p || bytes < b) alloc(b);
 }
 
 template <class T>
@@ -1663,10 +1683,12 @@ void FrontSolver::gmres_correction(bool chol) {
   const long long N = ndof_;
   const int nr = nrhs_, m = gmres_m_;
   const long long vs = N * nr;
-  d_V_.alloc((size_t)(m + 1) * vs * sizeof(double));
-  d_dpart_.alloc(std::max(1LL, uhmk_block_dots_scratch(N, nr)) * sizeof(double));
-  d_dout_.alloc((size_t)(m + 1) * nr * sizeof(double));
-  d_coef_.alloc((size_t)(m + 1) * nr * sizeof(double));
+  // grow-only: the basis and the reduction buffers are sized once per
+  // pattern and restart length, not reallocated per correction
+  d_V_.reserve((size_t)(m + 1) * vs * sizeof(double));
+  d_dpart_.reserve(std::max(1LL, uhmk_block_dots_scratch(N, nr)) * sizeof(double));
+  d_dout_.reserve((size_t)(m + 1) * nr * sizeof(double));
+  d_coef_.reserve((size_t)(m + 1) * nr * sizeof(double));
   double* V = d_V_.as<double>();
   double* dout = d_dout_.as<double>();
   double* coef = d_coef_.as<double>();

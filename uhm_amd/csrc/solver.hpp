@@ -34,7 +34,8 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf() { release(); }
   void release();
-  void alloc(size_t b);
+  void alloc(size_t b);    // exactly b bytes (old contents dropped)
+  void reserve(size_t b);  // at least b bytes; keeps a large enough buffer
   template <class T>
   T* as() const { return static_cast<T*>(p); }
   template <class T>
@@ -83,6 +84,10 @@ struct KernelStats {
   double bytes[N_LAUNCH_KINDS] = {0};
   int launches[N_LAUNCH_KINDS] = {0};
 };
+
+// Device bytes held by the library's buffers now and at most so far.
+long long device_bytes_used();
+long long device_bytes_max();
 
 class FrontSolver {
  public:
