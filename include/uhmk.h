@@ -298,6 +298,18 @@ int uhmk_front_global_all(const uhmk_front* fronts, int n_fronts, const long lon
 long long uhmk_block_dots_scratch(long long nrow, int nrhs);
 int uhmk_block_dots(const double* V, long long vstride, int k, const double* w, long long nrow, int nrhs,
                     double* partial, double* out, hipStream_t stream);
+/* uhmk_block_dots without the caller's workspace: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops): it calls no hipMalloc / hipFree
+ * and keeps its partial sums in a fixed device array of
+ * uhmk_block_dots_noalloc_max() doubles (4 MiB; one copy per GPU, held until
+ * the process ends).  Same results as uhmk_block_dots, bit for bit.  Returns
+ * -1013 when uhmk_block_dots_scratch(nrow, nrhs) exceeds that maximum,
+ * -1012 for k > 32 or nrhs > 65535.  Calls that overlap on one GPU (two
+ * streams, two host threads) share the array and race. */
+long long uhmk_block_dots_noalloc_max(void);
+int uhmk_block_dots_noalloc(const double* V, long long vstride, int k, const double* w, long long nrow, int nrhs,
+                            double* out, hipStream_t stream);
 int uhmk_basis_combine(const double* V, long long vstride, int k, const double* coef, double alpha, double beta,
                        double* w, long long nrow, int nrhs, hipStream_t stream);
 int uhmk_scale_columns(double* w, long long nrow, int nrhs, const double* s, hipStream_t stream);

@@ -10,6 +10,8 @@
 // whose reference body is a stub.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "uhmk.h"
 
 namespace {
@@ -88,6 +90,40 @@ __global__ __launch_bounds__(256) void k_scale_columns(double* __restrict__ w, l
 
 inline int ret(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
 
+// Partial sums of uhmk_block_dots_noalloc: a fixed device array instead of
+// a caller's workspace, for callers that cannot allocate.  Every GPU holds
+// its own copy; the host reaches it only through the address that
+// hipGetSymbolAddress returns on that device (kept per device below).
+constexpr long long NOALLOC_DOUBLES = 1LL << 19;  // 4 MiB
+__device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+constexpr int MAX_DEVICES = 64;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+std::once_flag g_scratch_once[MAX_DEVICES];
+hipError_t g_scratch_err[MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current; nullptr (and *err set) on failure.
+double* scratch_ptr(hipError_t* err) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    *err = e;
+    return nullptr;
+  }
+  if (dev < 0 || dev >= MAX_DEVICES) {
+    *err = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  std::call_once(g_scratch_once[dev], [dev] {
+    void* p = nullptr;
+    g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (g_scratch_err[dev] == hipSuccess) g_scratch_ptr[dev] = static_cast<double*>(p);
+  });
+  *err = g_scratch_err[dev];
+  return g_scratch_ptr[dev];
+}
+
 }  // namespace
 
 extern "C" {
@@ -107,6 +143,21 @@ int uhmk_block_dots(const double* V, long long vstride, int k, const double* w, 
   hipLaunchKernelGGL(k_block_dots_final, dim3((unsigned)((k * nrhs + 255) / 256)), dim3(256), 0, stream, partial,
                      (int)nblk, k, nrhs, out);
   return ret(hipGetLastError());
+}
+
+long long uhmk_block_dots_noalloc_max(void) { return NOALLOC_DOUBLES; }
+
+int uhmk_block_dots_noalloc(const double* V, long long vstride, int k, const double* w, long long nrow, int nrhs,
+                            double* out, hipStream_t stream) {
+  if (k <= 0 || nrow <= 0 || nrhs <= 0) return 0;
+  if (k > KMAX || nrhs > 65535) return -1012;
+  if (uhmk_block_dots_scratch(nrow, nrhs) > NOALLOC_DOUBLES) return -1013;
+  hipError_t err = hipSuccess;
+  double* partial = scratch_ptr(&err);
+  if (!partial) return ret(err != hipSuccess ? err : hipErrorInvalidSymbol);
+  // every partial slot that k_block_dots_final reads is written first by
+  // k_block_dots in the same stream order, so nothing needs clearing
+  return uhmk_block_dots(V, vstride, k, w, nrow, nrhs, partial, out, stream);
 }
 
 int uhmk_basis_combine(const double* V, long long vstride, int k, const double* coef, double alpha, double beta,
