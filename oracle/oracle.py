@@ -64,6 +64,7 @@ def load():
     lib.orc_get_margins.argtypes = [C.c_void_p, C.c_int, _D]
     lib.orc_get_input.argtypes = [C.c_void_p, C.c_int, _D]
     lib.orc_assemble.argtypes = [C.c_void_p, C.c_int, _D]
+    lib.orc_set_factored.argtypes = [C.c_void_p, C.c_int, _D]
     lib.orc_get_x_global.argtypes = [C.c_void_p, _D, C.c_int]
     lib.orc_flops.restype = C.c_double
     lib.orc_flops.argtypes = [C.c_void_p, C.c_int]
@@ -228,6 +229,12 @@ class OracleMesh:
         out = self._square(e)
         _lib.orc_get_input(self.h, e, _dp(out))
         return out
+
+    def set_factored(self, e, F):
+        """Working values of front e (e.g. factored on the GPU), read by
+        assemble() of its parent."""
+        F = np.asfortranarray(F, dtype=np.float64)
+        _lib.orc_set_factored(self.h, e, _dp(F))
 
     def assemble(self, e):
         """Extend-add of front e (merge_A: 0 + own input + children's ABR in

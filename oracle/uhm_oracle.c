@@ -499,6 +499,14 @@ void orc_set_rhs_front(void* h, int e, const double* b) {
   ensure(o, e);
   memcpy(o->B[e], b, n * o->nrhs * sizeof(double));
 }
+/* replace the working values of front e (n x n), e.g. by factored values
+ * computed elsewhere, as the children's input of orc_assemble */
+void orc_set_factored(void* h, int e, const double* F) {
+  orc* o = (orc*)h;
+  size_t n = (size_t)(o->fs[e] + o->ss[e]);
+  ensure(o, e);
+  memcpy(o->F[e], F, n * n * sizeof(double));
+}
 /* the assembly input of front e (n x n, front order; zero without copy-in) */
 void orc_get_input(void* h, int e, double* out) {
   orc* o = (orc*)h;
