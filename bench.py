@@ -83,50 +83,41 @@ def build_mesh(U, w, seed):
     return m
 
 
-def cpu_baseline(U, m, w, target_gflop=470.0):
-    """The CPU oracle (C restatement of the reference algorithm, OpenMP over
-    the fronts of a level) on the bottom levels of the same tree, with fresh
-    element matrices of the same distribution drawn by numpy.  The sample
-    stops at the level that would take it past target_gflop (default
-    workload: levels 0-11, 469.7 of 494.8 GFLOP, about 20 s on 16 threads;
-    the root alone, one front on one thread, would add about 5 s more for
-    the last 5 % of the flops).  target_gflop None: the whole tree."""
-    import numpy as np
+def cpu_baseline(U, m, w, repeats=3):
+    """The CPU oracle (the C restatement of the reference algorithm in
+    oracle/: BLAS-free blocked loops, OpenMP over the fronts of a level, or
+    inside a front for levels with fewer fronts than threads) factorizing
+    the whole tree of the same matrices the GPU factorized (leaf stages and
+    right-hand sides fetched from the device), tile b = 256; the best of
+    `repeats` runs, then one solve, whose solution is returned for the
+    forward errors of the GPU paths."""
     from oracle.oracle import OracleMesh, set_threads
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     threads = max(1, min(threads, 16, os.cpu_count() or 1))
     set_threads(threads)
-    c = m.export_connectivity()
-    om = OracleMesh.from_connectivity(c)
-    om.init_numeric(1)
-    levels = om.levels()
-    acc, L1 = 0.0, 0
-    for L in range(len(levels)):
-        fl = om.level_flops(w["method"], L, L)
-        if target_gflop is not None and L > 0 and acc + fl > target_gflop * 1e9:
-            break
-        acc += fl
-        L1 = L
-    rng = np.random.default_rng(1)
-    eptr, enode = c["eptr"], c["enode"]
-    leaves = [e for e in range(len(c["parent"])) if eptr[e + 1] > eptr[e]]
-    for e in leaves:
-        nl = enode[eptr[e]:eptr[e + 1]]
-        k = len(nl)
-        M = rng.standard_normal((k, k))
-        K = M.T @ M + w["alpha"] * np.eye(k)
-        if w["beta"]:
-            K += w["beta"] * rng.standard_normal((k, k))
-        om.copy_in(e, nl, 1, K)
-    t0 = time.perf_counter()
-    info = om.factor_levels(w["method"], 256, 0, L1)
-    dt = time.perf_counter() - t0
-    return dict(value=round(acc / dt / 1e12, 5), unit="TFLOP/s", cores=threads, kind="port",
-                sample="oracle factorization of tree levels 0-%d (%d fronts, %.1f GFLOP of %.1f) of the same "
-                       "workload, tile b=256, %d OpenMP threads, %.2f s, info %d"
-                       % (L1, sum(len(levels[L]) for L in range(L1 + 1)), acc / 1e9,
-                          om.flops(w["method"]) / 1e9, threads, dt, info))
+    om = OracleMesh.from_connectivity(m.export_connectivity())
+    om.init_numeric(m.n_rhs)
+    for f in range(m.n_fronts()):
+        fi = m.front_info(f)
+        if fi["leaf"]:
+            om.set_front(fi["id"], m.fetch_stage(f))
+            om.set_rhs_front(fi["id"], m.fetch_rhs_front(f))
+    flops = om.flops(w["method"])
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        info = om.factor(w["method"], 256)
+        times.append(time.perf_counter() - t0)
+    om.solve(w["method"])
+    x = om.x_global(m.get_n_dof())
+    dt = min(times)
+    out = dict(value=round(flops / dt / 1e12, 5), unit="TFLOP/s", cores=threads, kind="port",
+               sample="BLAS-free oracle (C restatement of the reference's lu_piv_hier / chol_hier, blocked loops, "
+                      "OpenMP) factorizing the whole tree (%d fronts, %.1f GFLOP) of the same device-generated "
+                      "matrices, tile b=256, %d threads, best of %d: %s s, info %d"
+                      % (m.n_fronts(), flops / 1e9, threads, repeats, ", ".join("%.2f" % t for t in times), info))
+    return out, x
 
 
 def run_path(U, m, w, mode, top, args):
@@ -173,6 +164,7 @@ def run_path(U, m, w, mode, top, args):
     hist, ref_ms = m.refine_history()
     n_solves = m.refine_solves()
     eta = m.backward_error()
+    x = m.fetch_x()
     f_ms, s_ms = m.timing()
     log("[%s] factor+solve+refine %.2f ms (factor %.2f, solve %.2f, refine %.2f ms, %d corrections, %d solves), "
         "backward error %s" % (name, t_sol, f_ms, s_ms, ref_ms, n_corr, n_solves,
@@ -186,7 +178,7 @@ def run_path(U, m, w, mode, top, args):
             log("  %-10s %5d launches %9.3f ms  %8.3f TFLOP/s  %8.1f GB/s"
                 % (k, n_k, ms_k, fl_k / ms_k / 1e9 if ms_k else 0, by_k / ms_k / 1e6 if ms_k else 0))
     return dict(ms_step=ms_step, t_sol=t_sol, eta=eta, hist=[float(e) for e in hist], n_corr=n_corr, solve_ms=s_ms,
-                refine_ms=ref_ms, n_solves=n_solves, stats=stats, tuned=tuned)
+                refine_ms=ref_ms, n_solves=n_solves, stats=stats, tuned=tuned, x=x)
 
 
 def run_reseeds(U, m, w, args, n):
@@ -241,6 +233,20 @@ def gemm_roofline(stats, peak, kernel, traffic=None):
             "avg_launch_us": round(g_ms / g_n * 1e3, 3) if g_n else None}
 
 
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
+
+
+def assemble_roofline(stats):
+    """Extend-add (k_assemble) against HBM bandwidth: algorithmic bytes
+    (every stored parent entry written once, its own stage and each child's
+    ABR entry read once) over the summed launch durations."""
+    ms, _, by, n = stats["assemble"]
+    achieved = by / (ms * 1e-3) / 1e9 if ms else 0.0
+    return {"kernel": "k_assemble (extend-add, gather form)", "bound": "hbm", "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+            "launches": n, "avg_launch_us": round(ms / n * 1e3, 3) if n else None}
+
+
 def traffic_probe(U, m, w, top):
     """The program of the HBM-traffic counter passes: a device-to-device
     copy of 512 MiB (the runtime's copy kernel, 512 MiB read and 512 MiB
@@ -272,6 +278,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="chol")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the compact lines of the other configurations (16^3 Q3 LU, 256^2 var-p, 24^3 Q3)")
     ap.add_argument("--nb", type=int, default=64, help="panel width")
     ap.add_argument("--top", type=int, default=2, help="tree levels kept in FP64 by the mixed path")
     ap.add_argument("--refine", choices=sorted(REFINE_KINDS), default=None,
@@ -280,7 +288,8 @@ def main():
                     help="graph-replayed re-factorizations after the timed paths (default: the workload's)")
     ap.add_argument("--bf16", action="store_true",
                     help="also time the BF16-Schur mixed path (BF16 operands, FP32 accumulation in the Schur "
-                         "complement products), reported beside the FP32 one")
+                         "complement products), reported beside the FP32 one (always on for the default "
+                         "workload)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="counter-pass program instead of the bench: a copy kernel of known size (calibration: "
                          "512 MiB read, 512 MiB written), then two factorizations of the mixed path (the second "
@@ -294,51 +303,30 @@ def main():
         sys.exit(2)
 
     import uhm_amd as U
-    w = WORKLOADS[args.workload]
-    t0 = time.perf_counter()
-    m = build_mesh(U, w, seed=3)
     U.set_hier_block_size(args.nb)
     if args.traffic_probe:
+        w = WORKLOADS[args.workload]
+        m = build_mesh(U, w, seed=3)
         traffic_probe(U, m, w, w.get("top", args.top))
         return
-    ndof = m.get_n_dof()
-    flops, _, _, _ = m.estimate_cost(w["method"])
-    log("mesh %s: ndof %d, fronts %d, %.1f GFLOP, setup %.1f s"
-        % (args.workload, ndof, m.n_fronts(), flops / 1e9, time.perf_counter() - t0))
 
-    top = w.get("top", args.top)
-    r64 = run_path(U, m, w, U.PREC_FP64, top, args)
-    rmx = run_path(U, m, w, U.PREC_MIXED, top, args)
-    rbf = run_path(U, m, w, U.PREC_MIXED_BF16, top, args) if args.bf16 else None
-    n_reseeds = w.get("reseeds", 0) if args.reseeds is None else args.reseeds
-    reseeds = run_reseeds(U, m, w, args, n_reseeds) if n_reseeds else None
+    # the headline workload in full, with the CPU baseline on its matrices
+    res = measure(U, args.workload, args, args.steps, args.warmup, bf16=args.bf16 or args.workload == "chol")
+    w, m, flops = res["w"], res["m"], res["flops"]
+    r64, rmx, rbf = res["fp64"], res["mixed"], res["bf16"]
     mixed_ok = rmx["eta"] <= 1e-12
     head = rmx if mixed_ok else r64
-
-    cpu = None
+    cpu, x_oracle = None, None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(U, m, w)
+            cpu, x_oracle = cpu_baseline(U, m, w)
             log("cpu baseline:", cpu["value"], cpu["unit"], cpu["sample"])
         except Exception as e:  # the baseline is reported, never the product
             log("cpu baseline failed:", repr(e))
-
-    def tflops(r):
-        return round(flops / (r["ms_step"] * 1e-3) / 1e12, 4)
-
-    def path(r):
-        return {"factor_tflops": tflops(r), "factor_ms": round(r["ms_step"], 4),
-                "time_to_1e-12_ms": round(r["t_sol"], 3) if r["eta"] <= 1e-12 else None,
-                "backward_error": r["eta"], "backward_error_history": r["hist"], "corrections": r["n_corr"],
-                "refine_solves": r["n_solves"],
-                "solve_ms": round(r["solve_ms"], 3), "refine_ms": round(r["refine_ms"], 3), "schedule": r["tuned"],
-                "kernels": {k: {"ms": round(v[0], 4), "gflop": round(v[1] / 1e9, 3),
-                                "gbytes": round(v[2] / 1e9, 4), "launches": v[3]}
-                            for k, v in r["stats"].items() if v[3]}}
-
+    top = w.get("top", args.top)
     out = {
         "metric": METRIC,
-        "value": tflops(head),
+        "value": _tflops(flops, head),
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -351,7 +339,7 @@ def main():
         "data": "synthetic (Philox element matrices generated on the device)",
         "config": {
             "workload": w["name"],
-            "ndof": ndof,
+            "ndof": m.get_n_dof(),
             "fronts": m.n_fronts(),
             "factor_gflop": round(flops / 1e9, 3),
             "precision_path": ("mixed: FP32 fronts and FP32 MFMA Schur updates below the top %d tree levels, "
@@ -361,19 +349,119 @@ def main():
             "panel_width": args.nb,
             "n_rhs": w.get("nrhs", 1),
         },
-        "time_to_1e-12_ms": {"mixed": path(rmx)["time_to_1e-12_ms"], "fp64": path(r64)["time_to_1e-12_ms"],
-                             "speedup": round(r64["t_sol"] / rmx["t_sol"], 3) if mixed_ok else None},
+        "time_to_1e-12_ms": _ttt(res),
         "roofline": gemm_roofline(head["stats"], PEAK_FP32_TFLOPS if mixed_ok else PEAK_FP64_TFLOPS,
                                   "k_gemm (Schur / trailing update, %s MFMA)" % ("FP32" if mixed_ok else "FP64"),
                                   GEMM_TRAFFIC.get((args.workload, "mixed" if mixed_ok else "fp64"))),
         "roofline_fp64_path": gemm_roofline(r64["stats"], PEAK_FP64_TFLOPS, "k_gemm (FP64 MFMA)"),
-        "mixed": path(rmx),
-        "fp64": path(r64),
-        "mixed_bf16": path(rbf) if rbf else None,
-        "reseeds": reseeds,
+        "roofline_extend_add": assemble_roofline(head["stats"]),
+        "mixed": _path(flops, rmx, r64, x_oracle),
+        "fp64": _path(flops, r64, r64, x_oracle),
+        "mixed_bf16": _path(flops, rbf, r64, x_oracle) if rbf else None,
+        "reseeds": res["reseeds"],
         "cpu_baseline": cpu,
     }
+    del m, res
+    # the other configurations, compact (driver-timed in the same run)
+    extra = {}
+    if args.workload == "chol" and not args.no_extra:
+        for name in ("lu", "varp", "q3big"):
+            try:
+                rx = measure(U, name, args, EXTRA_STEPS, 1, bf16=(name == "q3big"))
+                extra[name] = _compact(rx)
+                del rx
+            except Exception as e:  # reported, the headline stands
+                log("config %s failed: %r" % (name, e))
+                extra[name] = {"error": repr(e)}
+    out["configs"] = extra or None
     print(json.dumps(out), flush=True)
+
+
+EXTRA_STEPS = 3
+
+
+def measure(U, name, args, steps, warmup, bf16=False):
+    """Mesh, FP64 and mixed paths (and the BF16-Schur path if asked) of one
+    workload, plus its graph-replayed reseeds."""
+    import copy
+    w = WORKLOADS[name]
+    a = copy.copy(args)
+    a.steps, a.warmup = steps, warmup
+    t0 = time.perf_counter()
+    m = build_mesh(U, w, seed=3)
+    flops, _, _, _ = m.estimate_cost(w["method"])
+    log("mesh %s: ndof %d, fronts %d, %.1f GFLOP, setup %.1f s"
+        % (name, m.get_n_dof(), m.n_fronts(), flops / 1e9, time.perf_counter() - t0))
+    top = w.get("top", args.top)
+    r64 = run_path(U, m, w, U.PREC_FP64, top, a)
+    rmx = run_path(U, m, w, U.PREC_MIXED, top, a)
+    rbf = run_path(U, m, w, U.PREC_MIXED_BF16, top, a) if bf16 else None
+    n_reseeds = w.get("reseeds", 0) if args.reseeds is None else args.reseeds
+    reseeds = run_reseeds(U, m, w, a, n_reseeds) if n_reseeds else None
+    return dict(name=name, w=w, m=m, flops=flops, fp64=r64, mixed=rmx, bf16=rbf, reseeds=reseeds)
+
+
+def _rel(a, b):
+    import numpy as np
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def _tflops(flops, r):
+    return round(flops / (r["ms_step"] * 1e-3) / 1e12, 4)
+
+
+def _ttt(res):
+    """Time from the start of factorization to backward error <= 1e-12 for
+    each path, and the FP64 path's time over each low-precision path's."""
+    r64 = res["fp64"]
+    out = {"fp64": round(r64["t_sol"], 3) if r64["eta"] <= 1e-12 else None}
+    for key, r in (("mixed", res["mixed"]), ("mixed_bf16", res["bf16"])):
+        if r is None:
+            continue
+        ok = r["eta"] <= 1e-12
+        out[key] = round(r["t_sol"], 3) if ok else None
+        out["speedup" if key == "mixed" else "speedup_bf16"] = round(r64["t_sol"] / r["t_sol"], 3) if ok else None
+    return out
+
+
+def _path(flops, r, r64, x_oracle):
+    """One path's record: factor throughput, time to 1e-12, backward error
+    history, forward errors against the FP64 path and the oracle, per-kind
+    kernel statistics."""
+    return {"factor_tflops": _tflops(flops, r), "factor_ms": round(r["ms_step"], 4),
+            "time_to_1e-12_ms": round(r["t_sol"], 3) if r["eta"] <= 1e-12 else None,
+            "backward_error": r["eta"], "backward_error_history": r["hist"],
+            "forward_error_vs_fp64": _rel(r["x"], r64["x"]) if r is not r64 else 0.0,
+            "forward_error_vs_oracle": _rel(r["x"], x_oracle) if x_oracle is not None else None,
+            "corrections": r["n_corr"], "refine_solves": r["n_solves"],
+            "solve_ms": round(r["solve_ms"], 3), "refine_ms": round(r["refine_ms"], 3), "schedule": r["tuned"],
+            "kernels": {k: {"ms": round(v[0], 4), "gflop": round(v[1] / 1e9, 3),
+                            "gbytes": round(v[2] / 1e9, 4), "launches": v[3]}
+                        for k, v in r["stats"].items() if v[3]}}
+
+
+def _compact(res):
+    """Short record of a secondary configuration."""
+    w, flops, r64 = res["w"], res["flops"], res["fp64"]
+    out = {"workload": w["name"], "ndof": res["m"].get_n_dof(), "fronts": res["m"].n_fronts(),
+           "factor_gflop": round(flops / 1e9, 3), "n_rhs": w.get("nrhs", 1), "steps": EXTRA_STEPS,
+           "time_to_1e-12_ms": _ttt(res)}
+    for key, r in (("fp64", r64), ("mixed", res["mixed"]), ("mixed_bf16", res["bf16"])):
+        if r is None:
+            continue
+        out[key] = {"factor_ms": round(r["ms_step"], 4), "factor_tflops": _tflops(flops, r),
+                    "solve_ms": round(r["solve_ms"], 3), "refine_ms": round(r["refine_ms"], 3),
+                    "corrections": r["n_corr"], "refine_solves": r["n_solves"], "backward_error": r["eta"],
+                    "forward_error_vs_fp64": _rel(r["x"], r64["x"]) if r is not r64 else 0.0,
+                    "roofline_gemm": gemm_roofline(r["stats"], PEAK_FP64_TFLOPS if key == "fp64" else PEAK_FP32_TFLOPS,
+                                                   "k_gemm"),
+                    "roofline_extend_add": assemble_roofline(r["stats"])}
+    if res["reseeds"]:
+        rs = res["reseeds"]
+        out["reseeds"] = {"count": len(rs), "max_eta": max(r["eta"] for r in rs),
+                          "mean_ms": round(sum(r["ms"] for r in rs) / len(rs), 3),
+                          "corrections": [r["corrections"] for r in rs]}
+    return out
 
 
 if __name__ == "__main__":
