@@ -294,6 +294,8 @@ def main():
                     help="counter-pass program instead of the bench: a copy kernel of known size (calibration: "
                          "512 MiB read, 512 MiB written), then two factorizations of the mixed path (the second "
                          "is the measured one); run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
+                    help="schedule knob (uhm_set_tuning_) applied before timing, repeatable")
     ap.add_argument("--tune", action="store_true",
                     help="run the autotuner over the schedule knobs before timing (default: built-in knobs)")
     args = ap.parse_args()
@@ -392,6 +394,9 @@ def measure(U, name, args, steps, warmup, bf16=False):
     flops, _, _, _ = m.estimate_cost(w["method"])
     log("mesh %s: ndof %d, fronts %d, %.1f GFLOP, setup %.1f s"
         % (name, m.get_n_dof(), m.n_fronts(), flops / 1e9, time.perf_counter() - t0))
+    for kv in args.set:
+        k, v = kv.split("=")
+        m.set_tuning(int(k), int(v))
     top = w.get("top", args.top)
     r64 = run_path(U, m, w, U.PREC_FP64, top, a)
     rmx = run_path(U, m, w, U.PREC_MIXED, top, a)

@@ -149,8 +149,9 @@ void uhm_get_timing_(uhm_fort_p* mesh, uhm_fort_double* factor_ms, uhm_fort_doub
  * lu_panel, lu_rows, chol_diag, chol_trsm, gemm (panel update), schur
  * (final Schur complement), lu_front (whole-front LU of fronts with
  * n <= 64), gemm_diag (Cholesky panel update that also factors the next
- * panel's diagonal blocks): milliseconds, algorithmic flops, algorithmic
- * bytes and launch counts (arrays of 9). */
+ * panel's diagonal blocks), chol_group (a Cholesky panel group in one
+ * persistent launch): milliseconds, algorithmic flops, algorithmic bytes
+ * and launch counts (arrays of 10). */
 void uhm_set_profile_(uhm_fort_p* mesh, uhm_fort_int* on);
 void uhm_get_kernel_stats_(uhm_fort_p* mesh, uhm_fort_double* ms, uhm_fort_double* flops, uhm_fort_double* bytes,
                            uhm_fort_int* launches);
@@ -201,7 +202,12 @@ void uhm_get_refine_solves_(uhm_fort_p* mesh, uhm_fort_int* n);
  * autotuner tries it), key 6 = Cholesky diagonal blocks inside a panel
  * group factored by the workgroup of the previous panel update that
  * updates them, without a launch of their own (default 1; 0: one diagonal
- * launch per panel; the factors are the same bits either way). */
+ * launch per panel; the factors are the same bits either way), key 7 =
+ * Cholesky panel groups (key 1, at most 16 panels) each in one persistent
+ * launch whose workgroups resolve the panel-to-panel dependencies among
+ * themselves, left-looking inside the group: 0 never (the per-panel
+ * launches that keys 5 and 6 shape), 1 on every tree level, 2 (default) on
+ * the levels with at most 16 fronts, where they measured faster. */
 void uhm_set_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value);
 void uhm_get_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value);
 /* Autotuner: times factorizations of the current values over the knobs

@@ -117,6 +117,26 @@ int uhmk_chol_trsm(const uhmk_front* fronts, const uhmk_item* items, int n_items
 int uhmk_chol_gemm_diag(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, void* arena,
                         int prec, int mma, int* info, hipStream_t stream);
 
+/* Cholesky panel group in one persistent launch: the columns [c0, ge),
+ * ge = min(c0 + ncol, fs), of every front listed, factored left-looking in
+ * 64-column panels -- per panel the diagonal block and its inverse (as
+ * uhmk_chol_diag), the panel rows below it (as uhmk_chol_trsm) and the
+ * updates inside the group (as uhmk_gemm_update with K = 64 panels) -- with
+ * the dependencies between panels resolved inside the launch (the steps of
+ * linal::dense::chol, LINAL/src/dense/chol.cxx:46-92, for the columns of
+ * one group; the trailing matrix right of the group is left to
+ * uhmk_gemm_update).  The rows [c0, n) of a front are cut into 64-row
+ * tiles (cut at fs too), numbered from 0; items {f, first tile r, number
+ * of tiles, flag slot s}: each tile r < P (the panels of the group) an
+ * item of its own, listed before the front's other items, which cover the
+ * remaining tiles in increasing r.  flags: ints, zero before the
+ * launch; flags[0] counts waits that gave up (nonzero: the results are
+ * invalid), flags[s + r] the progress of tile r (at most 16 per front).
+ * ncol a multiple of 64, at most 1024.  mma 0 (FP64 fronts) or 1 (FP32
+ * products); info as uhmk_chol_diag. */
+int uhmk_chol_group(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int ncol, void* arena,
+                    int prec, int mma, int* flags, int* info, hipStream_t stream);
+
 /* C -= A op(B) on tile x tile blocks (tile 64 or 128) on MFMA: panel
  * update (schur = 0, K = [c0, c0+nb)) or final Schur complement
  * ABR -= ABL ATR / ABL ABL^T (schur = 1, K = [0, fs)).  mma 0: FP64

@@ -412,6 +412,7 @@ def test_cholesky_fused_diagonal_same_bits(lib, prec):
     m = _chol_mesh(19, 64.0)
     m.set_precision(prec, 1)
     say("case fused diagonal prec", prec, "max fs", max(m.front_info(f)["fs"] for f in range(m.n_fronts())))
+    m.set_tuning(7, 0)  # per-panel launches (the path keys 5 and 6 shape)
     m.set_tuning(6, 0)
     assert m.chol_with_free() == 0
     ref = [defined(m, f) for f in range(m.n_fronts())]
@@ -437,7 +438,47 @@ def test_cholesky_fused_diagonal_reports_failure(lib):
     got = []
     for fuse in (0, 1):
         m = _chol_mesh(23, 0.0, 50.0)
+        m.set_tuning(7, 0)
         m.set_tuning(6, fuse)
         got.append(m.chol_with_free())
-    say("  info unfused %d fused %d" % tuple(got))
-    assert got[0] != 0 and got[0] == got[1]
+    m = _chol_mesh(23, 0.0, 50.0)
+    m.set_tuning(7, 1)
+    got.append(m.chol_with_free())
+    say("  info unfused %d fused %d group launches %d" % tuple(got))
+    assert got[0] != 0 and got[0] == got[1] == got[2]
+
+
+@pytest.mark.parametrize("prec", [U.PREC_FP64, U.PREC_MIXED])
+@pytest.mark.parametrize("group", [1, 3, 8, 16])
+def test_cholesky_group_launch_matches_panel_path(lib, prec, group):
+    """Each panel group in one persistent launch (key 7, default; the
+    workgroups of one front's row tiles wait for each other's diagonal
+    blocks inside the launch, left-looking within the group) against the
+    per-panel launches (key 7 = 0, right-looking): the same factors up to
+    rounding (the update order inside a group differs), run after run the
+    same bits, and the FP64 path's backward error at the FP64 bound."""
+    m = _chol_mesh(29, 64.0)
+    m.set_precision(prec, 1)
+    m.set_tuning(1, group)
+    say("case group launch prec", prec, "group", group)
+    m.set_tuning(7, 0)
+    assert m.chol_with_free() == 0
+    nf = m.n_fronts()
+    ref = [np.tril(m.fetch_front(f)) for f in range(nf)]
+    m.set_tuning(7, 1)  # every level (the default, 2, only the levels with few fronts)
+    assert m.chol_with_free() == 0
+    got = [np.tril(m.fetch_front(f)) for f in range(nf)]
+    tol = 1e-12 if prec == U.PREC_FP64 else 2e-5
+    worst = max(float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)) for a, b in zip(got, ref))
+    say("  worst front deviation %.2e" % worst)
+    assert worst <= tol
+    assert m.chol_with_free() == 0
+    again = [np.tril(m.fetch_front(f)) for f in range(nf)]
+    assert all(np.array_equal(a, b) for a, b in zip(again, got))
+    m.solve_chol()
+    m.check_chol()
+    if prec == U.PREC_FP64:
+        assert m.backward_error() <= ETA_TOL
+    else:
+        m.improve(U.UHM_CHOL, 1e-12, 30)
+        assert m.backward_error() <= 1e-12

@@ -9,6 +9,8 @@
 //                                      lu_piv/decompose.cxx:148-149, chol/decompose.cxx:95-96
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "dense_device.hpp"
 #include "uhmk.h"
 
@@ -78,6 +80,7 @@ struct Chol16Step<16> {
   __device__ __forceinline__ static void run(double (&)[16], int, int&) {}
 };
 
+
 // Cholesky of the kb x kb diagonal panel block and its inverse, one
 // workgroup of four wavefronts per front, by 16 x 16 blocks.  Block step B:
 // lanes 0..15 of wavefront 0 factor the diagonal block (v_readlane
@@ -105,7 +108,16 @@ struct DiagLds {
 // front at A (the block's first entry, column stride n) and the first
 // non-positive pivot (c0 + its index + 1) to info[f].  Called by all 256
 // threads of the workgroup.
-template <typename T>
+// Store of one front entry: plain, or (COH) coherent across the XCDs'
+// caches for readers inside the same launch (agent-scope relaxed atomic:
+// the store carries sc1, no cache maintenance).
+template <bool COH, typename T>
+__device__ __forceinline__ void st_front(T* p, T v) {
+  if constexpr (COH) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else *p = v;
+}
+
+template <typename T, bool COH = false>
 __device__ __forceinline__ void chol_diag_lds(DiagLds& S, T* __restrict__ A, long long n, int kb, int c0,
                                               int* __restrict__ info, int f) {
   double* __restrict__ sL = S.sL;
@@ -199,10 +211,7 @@ __device__ __forceinline__ void chol_diag_lds(DiagLds& S, T* __restrict__ A, lon
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
     const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
-    if (r < kb && c < kb) {
-      if (c <= r) A[r + (long long)c * n] = (T)sL[r * LDW + c];
-      else A[r + (long long)c * n] = (T)sW[c * LDW + r];
-    }
+    if (r < kb && c < kb) st_front<COH>(A + r + (long long)c * n, (T)(c <= r ? sL[r * LDW + c] : sW[c * LDW + r]));
   }
   if (tid == 0 && s_first && info[f] == 0) info[f] = c0 + s_first;
 }
@@ -772,11 +781,345 @@ int launch_gemm(const uhmk_front* fronts, const uhmk_item* items, int n_items, i
   return 0;
 }
 
+// ------------------------------------------------------------------------
+// Cholesky panel group as one persistent launch: the columns [g0, ge) of
+// every front of a launch, ge = min(g0 + gw, fs), factored left-looking
+// inside the group (the chol.cxx:60-92 steps -- diagonal block, panel
+// TRSM, update -- for panels of 64 columns, LINAL/src/dense/chol.cxx:67,
+// 80-84, 86-90; the updates from columns left of g0 came from earlier
+// trailing updates).  One workgroup per 64-row tile r of a front's rows
+// [g0, n) (tiles cut at fs; item {f, r, first row, flag slot}); tile r
+// runs the panels p = 0 .. (r < P ? r : P - 1) in order, P = panels of the
+// group:
+//   C = A(r, p) - sum_{q < p} L(r, q) L(p, q)^T    (MFMA, K = 64 p)
+//   r == p: C = L(p, p) L(p, p)^T and W_p = L(p, p)^-1 (chol_diag_lds), to
+//           the front; publish
+//   r != p: wait until tile p published W_p; L(r, p) = C W_p^T (FP64
+//           MFMA); tiles r < P publish.
+// Tile r < P publishes its progress (panels done) in flags[slot + r];
+// waiters spin on it (one thread, bounded: after SPIN_LIMIT polls flags[0]
+// is raised and the launch finishes with wrong values instead of hanging).
+// The factor entries written inside the launch (L and W) are stored and
+// read as agent-scope relaxed atomics (sc1: coherent across the XCDs'
+// L2 caches, without the whole-cache write-back / invalidation that
+// acquire / release fences at agent scope cost), and a tile's stores have
+// completed (s_waitcnt) before it publishes.  A tile waits only on tiles of
+// the same front with a smaller index, listed earlier in the launch, so the
+// in-order dispatch of workgroups always leaves the awaited tile running or
+// done.  The chain per panel is diagonal block -> TRSM of the next tile ->
+// its rank-64 diagonal update -> next diagonal block, without launches in
+// between.  TS storage, TM MFMA type of the updates (as k_gemm); the
+// diagonal block and the TRSM run in FP64.
+// ------------------------------------------------------------------------
+constexpr int GROUP_MAX_PANELS = 16;
+constexpr int SPIN_LIMIT = 1 << 22;
+
+__device__ __forceinline__ void group_wait(int* flag, int target, int* err) {
+  if (threadIdx.x == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+    int spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins > SPIN_LIMIT) {
+        atomicAdd(err, 1);
+        break;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// every thread's coherent stores of this step complete, then (flag
+// non-null) the progress value published by one thread
+__device__ __forceinline__ void group_publish(int* flag, int v) {
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  if (flag && threadIdx.x == 0) __hip_atomic_store(flag, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Coherent load (agent-scope relaxed atomic: sc1) of data that another
+// workgroup of the same launch may have stored, zero when !ok.
+template <typename TM, typename T>
+__device__ __forceinline__ TM ld_coh(const T* p, long long idx, bool ok) {
+  return ok ? (TM)__hip_atomic_load(const_cast<T*>(p) + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (TM)0;
+}
+
+template <typename TS, typename TM>
+__global__ __launch_bounds__(256, 2) void k_chol_group(const uhmk_front* __restrict__ F,
+                                                    const uhmk_item* __restrict__ items, int g0, int gw,
+                                                    TS* __restrict__ arena, int* __restrict__ flags,
+                                                    int* __restrict__ info) {
+  typedef typename Mfma<TM>::acc_t acc_t;
+  constexpr int T = TILE, LD = T + 16, NB = 2, KR = 256 / T;
+  constexpr int KC = sizeof(TM) == 4 ? 32 : 16;
+  constexpr int NL = KC / KR;
+  // the update staging stays below sWt, which lives through the tiles of a panel step
+  static_assert((size_t)2 * KC * LD * sizeof(TM) <= (size_t)TILE * LDW * sizeof(double), "staging below sWt");
+  static_assert(sizeof(DiagLds) >= T * LD * sizeof(TM), "the L(r, p) operand fits the diagonal LDS");
+  static_assert(sizeof(DiagLds) >= (size_t)(TILE * LDW + TILE * LDT) * sizeof(double), "TRSM fits");
+  __shared__ __align__(16) unsigned char smem[sizeof(DiagLds)];
+  TM* __restrict__ sA = reinterpret_cast<TM*>(smem);
+  TM* __restrict__ sB = sA + KC * LD;
+  double* __restrict__ sC = reinterpret_cast<double*>(smem);  // 64 x LDW, row major (= DiagLds::sL)
+  double* __restrict__ sWt = sC + TILE * LDW;                 // TRSM: sWt[k][j] = W(j, k)
+  DiagLds& S = *reinterpret_cast<DiagLds*>(smem);
+
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs;
+  const int ge = min(g0 + gw, fs);
+  const int P = (ge - g0 + T - 1) / T;
+  // row tiles of [g0, n) cut at fs: the item's tiles r .. r + ntile - 1
+  const int ratl = (fs - g0 + T - 1) / T;
+  auto tile_row = [&](int t) { return t < ratl ? g0 + T * t : fs + T * (t - ratl); };
+  const int r = it.a, ntile = it.b;
+  const bool diag_tile = r < P;
+  int* err = flags;
+  int* fl = flags + it.c;
+  TS* __restrict__ M = arena + fr.off;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int iw = (w & 1) * 32, jw = (w >> 1) * 32, cl = lane & 15, kq = lane >> 4;
+  const int li = tid % T, lk = tid / T;
+
+  // acc = L(rows [i0, i0 + h), [g0, pc0)) L(p, [g0, pc0))^T for panel p
+  auto update = [&](int i0, int h, int pc0, int kb, acc_t (&acc)[NB][NB]) {
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+    TM ra[NL], rb[NL];
+    auto load = [&](int k0) {
+#pragma unroll
+      for (int u = 0; u < NL; ++u) {
+        const int k = k0 + lk + KR * u;
+        ra[u] = ld_coh<TM>(M, (i0 + li) + (long long)k * n, li < h && k < pc0);
+        rb[u] = ld_coh<TM>(M, (pc0 + li) + (long long)k * n, li < kb && k < pc0);
+      }
+    };
+    if (g0 < pc0) load(g0);
+    for (int k0 = g0; k0 < pc0; k0 += KC) {
+#pragma unroll
+      for (int u = 0; u < NL; ++u) {
+        sA[(lk + KR * u) * LD + li] = ra[u];
+        sB[(lk + KR * u) * LD + li] = rb[u];
+      }
+      __syncthreads();
+      if (k0 + KC < pc0) load(k0 + KC);
+#pragma unroll
+      for (int kk = 0; kk < KC; kk += 4) {
+        const int kr = kk + kq;
+        TM bo[NB], ao[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) bo[b] = sB[kr * LD + jw + b * 16 + cl];
+#pragma unroll
+        for (int a = 0; a < NB; ++a) ao[a] = sA[kr * LD + iw + a * 16 + cl];
+#pragma unroll
+        for (int a = 0; a < NB; ++a)
+#pragma unroll
+          for (int b = 0; b < NB; ++b) acc[a][b] = Mfma<TM>::run(bo[b], ao[a], acc[a][b]);
+      }
+      __syncthreads();
+    }
+  };
+
+  // The panel steps of the item's tiles; DIAGT: a tile of the group's
+  // diagonal range (one tile, ends with its own diagonal block).  Two
+  // instances, so that the accumulator of the diagonal block is live only
+  // where it is used.
+  auto run = [&](auto diag_c) {
+    constexpr bool DIAGT = decltype(diag_c)::value;
+    // panels whose diagonal block belongs to another tile; a tile of the
+    // group's diagonal range also accumulates sum_p L(r, p) L(r, p)^T for its
+    // own diagonal block as each L(r, p) appears, so that block needs no
+    // update of its own once the last TRSM is done
+    const int nsolve = DIAGT ? r : P;
+    acc_t accd[NB][NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) accd[a][b] = acc_t{0, 0, 0, 0};
+    for (int p = 0; p < nsolve; ++p) {
+      const int pc0 = g0 + T * p, kb = min(T, ge - pc0);
+      // L(p, q < p) of tile p done (own rows: earlier steps of this workgroup)
+      if (p > 0) group_wait(fl + p, p, err);
+      for (int t = 0; t < ntile; ++t) {
+        const int i0 = tile_row(r + t);
+        const int h = min(i0 + T, i0 < fs ? fs : n) - i0;
+        acc_t acc[NB][NB];
+        update(i0, h, pc0, kb, acc);
+        // C (rounded to TS, as a store and reload would) into LDS, zero
+        // outside h x kb
+#pragma unroll
+        for (int a = 0; a < NB; ++a)
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            const int ii = iw + a * 16 + cl;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int jj = jw + b * 16 + Mfma<TM>::col(q, lane);
+              double v = 0.0;
+              if (ii < h && jj < kb) {
+                const double c = (double)M[(i0 + ii) + (long long)(pc0 + jj) * n];
+                v = (double)(TS)(c - (double)acc[a][b][q]);
+              }
+              sC[ii * LDW + jj] = v;
+            }
+          }
+        // W_p from the diagonal block of panel p (strict upper: W transposed;
+        // diagonal: L(k, k), inverted below), once per panel step
+        if (t > 0) {
+          __syncthreads();
+        } else {
+          group_wait(fl + p, p + 1, err);
+          const TS* __restrict__ D = M + pc0 + (long long)pc0 * n;
+#pragma unroll 4
+          for (int u = 0; u < 16; ++u) {
+            const int idx = tid + 256 * u, rr = idx & 63, cc = idx >> 6;
+            sWt[rr * LDT + cc] = ld_coh<double>(D, rr + (long long)cc * n, rr < kb && cc < kb && rr <= cc);
+          }
+          __syncthreads();
+          if (tid < kb) sWt[tid * LDT + tid] = 1.0 / sWt[tid * LDT + tid];
+          __syncthreads();
+        }
+        // L(r, p) = C W^T: one 64 x 64 x 64 FP64 MFMA product, wavefront w owns
+        // the 32 x 32 quarter (iw, jw) (as k_chol_trsm)
+        d4 acc2[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) acc2[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+        for (int ks = 0; ks < TILE / 4; ++ks) {
+          const int kr = 4 * ks + kq;
+          double wo[2], xo[2];
+#pragma unroll
+          for (int b = 0; b < 2; ++b) wo[b] = sWt[kr * LDT + jw + 16 * b + cl];
+#pragma unroll
+          for (int a = 0; a < 2; ++a) xo[a] = sC[(iw + 16 * a + cl) * LDW + kr];
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) acc2[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(wo[b], xo[a], acc2[a][b], 0, 0, 0);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            const int ii = iw + 16 * a + cl;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int jj = jw + 16 * b + 4 * q + kq;
+              if (ii < h && jj < kb) st_front<true>(M + (i0 + ii) + (long long)(pc0 + jj) * n, (TS)acc2[a][b][q]);
+            }
+          }
+        if (diag_tile) {
+          // accd += X X^T with X = L(r, p) as stored (TS), in TM products
+          __syncthreads();  // sC / sWt read by every wavefront
+          TM* __restrict__ sX = reinterpret_cast<TM*>(smem);  // sX[k][i] = X(i, k)
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+              const int ii = iw + 16 * a + cl;
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                const int jj = jw + 16 * b + 4 * q + kq;
+                sX[jj * LD + ii] = (ii < h && jj < kb) ? (TM)(TS)acc2[a][b][q] : (TM)0;
+              }
+            }
+          __syncthreads();
+#pragma unroll
+          for (int kk = 0; kk < T; kk += 4) {
+            const int kr = kk + kq;
+            TM bo[NB], ao[NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) bo[b] = sX[kr * LD + jw + b * 16 + cl];
+#pragma unroll
+            for (int a = 0; a < NB; ++a) ao[a] = sX[kr * LD + iw + a * 16 + cl];
+#pragma unroll
+            for (int a = 0; a < NB; ++a)
+#pragma unroll
+              for (int b = 0; b < NB; ++b) accd[a][b] = Mfma<TM>::run(bo[b], ao[a], accd[a][b]);
+          }
+        }
+        if (t + 1 < ntile) __syncthreads();  // sC read before the next tile's staging
+      }
+      // own rows visible to this workgroup's next steps; tiles of the
+      // group's diagonal range also tell the others
+      group_publish(DIAGT ? fl + r : nullptr, p + 1);
+    }
+    if constexpr (DIAGT) {
+
+    // the diagonal block of panel r (rows = panel columns) minus the
+    // accumulated products, its lower triangle (rounded to TS as a store and
+    // reload would, identity beyond kb) through chol_diag_lds
+    {
+      const int pc0 = g0 + T * r, kb = min(T, ge - pc0);
+      __syncthreads();  // sX read by every wavefront
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int idx = tid + 256 * u, rr = idx & 63, cc = idx >> 6;
+        S.sL[rr * LDW + cc] = (rr >= kb && rr == cc) ? 1.0 : 0.0;
+        S.sW[rr * LDW + cc] = 0.0;
+      }
+      if (tid == 0) S.s_first = 0;
+      __syncthreads();
+#pragma unroll
+      for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const int ii = iw + a * 16 + cl;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int jj = jw + b * 16 + Mfma<TM>::col(q, lane);
+            if (ii < kb && jj <= ii) {
+              const double c = (double)M[(pc0 + ii) + (long long)(pc0 + jj) * n];
+              S.sL[ii * LDW + jj] = (double)(TS)(c - (double)accd[a][b][q]);
+            }
+          }
+        }
+      __syncthreads();
+      chol_diag_lds<TS, true>(S, M + pc0 + (long long)pc0 * n, n, kb, pc0, info, it.f);
+      group_publish(fl + r, r + 1);
+    }
+    }
+  };
+  if (diag_tile) run(std::true_type{});
+  else run(std::false_type{});
+}
+
+template <typename TS, typename TM>
+int launch_chol_group(const uhmk_front* fronts, const uhmk_item* items, int n_items, int g0, int gw, TS* arena,
+                      int* flags, int* info, hipStream_t stream) {
+  hipLaunchKernelGGL((k_chol_group<TS, TM>), dim3(n_items), dim3(256), 0, stream, fronts, items, g0, gw, arena, flags,
+                     info);
+  return 0;
+}
+
 inline int ret(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
 
 }  // namespace
 
 extern "C" {
+
+int uhmk_chol_group(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int ncol, void* arena,
+                    int prec, int mma, int* flags, int* info, hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  if (ncol < TILE || ncol % TILE || ncol > TILE * GROUP_MAX_PANELS) return -1000;
+  int rc;
+  if (prec == 0 && mma == 0)
+    rc = launch_chol_group<double, double>(fronts, items, n_items, c0, ncol, static_cast<double*>(arena), flags, info,
+                                           stream);
+  else if (prec == 0 && mma == 1)
+    rc = launch_chol_group<double, float>(fronts, items, n_items, c0, ncol, static_cast<double*>(arena), flags, info,
+                                          stream);
+  else if (prec == 1 && mma == 1)
+    rc = launch_chol_group<float, float>(fronts, items, n_items, c0, ncol, static_cast<float*>(arena), flags, info,
+                                         stream);
+  else
+    return -1006;
+  if (rc) return rc;
+  return ret(hipGetLastError());
+}
 
 int uhmk_chol_diag(const uhmk_front* fronts, const int* flist, int n_fronts, int c0, int nb, void* arena, int prec,
                    int* info, hipStream_t stream) {

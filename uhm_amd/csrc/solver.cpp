@@ -398,6 +398,7 @@ void FrontSolver::build_schedule(int method) {
   items_.clear();
   flists_.clear();
   launches_.clear();
+  flag_slots_ = 1;
   const int nb = nb_;
   std::vector<int> R, C;
   auto tile_end = [](int s, int fs, int n, int T) { return std::min(s + T, s < fs ? fs : n); };
@@ -470,6 +471,102 @@ void FrontSolver::build_schedule(int method) {
     // so the trailing entries are read and written once per group instead
     // of once per panel.
     const int G = (chol && nb == 64) ? std::max(1, chol_group()) : 1;
+    // trailing update of the panel group [g0, g0 + GW): rows [ge, n),
+    // columns [ge, fs), K = [g0, ge), lower tiles
+    auto emit_trailing = [&](int g0, int GW) {
+      Launch lt;
+      lt.c0 = g0;
+      lt.nb = GW;
+      lt.level = L;
+      lt.kind = Launch::GEMM;
+      lt.extra = 1;
+      double sr = 0, sc = 0;
+      int cnt = 0;
+      for (int f : big) {
+        const int ge = std::min(g0 + GW, fr_[f].fs);
+        if (fr_[f].fs <= g0 || ge >= fr_[f].fs) continue;
+        sr += fr_[f].n - ge;
+        sc += fr_[f].fs - ge;
+        ++cnt;
+      }
+      lt.tile = cnt ? gemm_tile_for(sr / cnt, sc / cnt, tile_threshold_) : 64;
+      const int T = lt.tile;
+      lt.off = (long long)items_.size();
+      for (int f : big) {
+        const int fs = fr_[f].fs, n = fr_[f].n;
+        if (fs <= g0) continue;
+        const int ge = std::min(g0 + GW, fs);
+        if (ge >= fs) continue;
+        tile_starts(ge, n, fs, T, R);
+        tile_starts(ge, fs, fs, T, C);
+        for (int j0 : C)
+          for (int i0 : R) {
+            if (tile_end(i0, fs, n, T) - 1 < j0) continue;
+            items_.push_back(uhmk_item{f, i0, j0, 0});
+          }
+      }
+      lt.count = (int)(items_.size() - lt.off);
+      if (lt.count) push_main(lt);
+    };
+    // group launches where they measured faster than per-panel launches:
+    // levels with few fronts (the top of the tree, whose panel chains are
+    // long); below, the many small fronts of a level fill the chip better
+    // with one launch per panel step
+    const bool group_level = group_kernel_ == 1 || (group_kernel_ == 2 && (int)big.size() <= 16);
+    if (chol && nb == 64 && group_level) {
+      // each group of G panels in one persistent launch (uhmk_chol_group):
+      // per front one item per 64-row tile of rows [g0, n), the tiles of
+      // the group's diagonal range of every front first, then the trailing
+      // update of the group
+      const int GW = 64 * std::min(G, 16);
+      // workgroups resident at once (two per CU at the launch's LDS size):
+      // a launch that would exceed them gives each workgroup below a front's
+      // diagonal range several consecutive row tiles, since those workgroups
+      // hold their slot until the group's last panel
+      int cus = 256;
+      {
+        int dev = 0;
+        if (hipGetDevice(&dev) == hipSuccess)
+          (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      }
+      const int slots = 2 * std::max(1, cus);
+      for (int g0 = 0; g0 < maxfs; g0 += GW) {
+        Launch lk;
+        lk.kind = Launch::CHOL_GROUP;
+        lk.c0 = g0;
+        lk.nb = GW;
+        lk.level = L;
+        lk.off = (long long)items_.size();
+        int n_diag = 0, n_below = 0;
+        for (int f : big) {
+          const int fs = fr_[f].fs, n = fr_[f].n;
+          if (fs <= g0) continue;
+          const int P = (std::min(g0 + GW, fs) - g0 + 63) / 64;
+          const int rows = (fs - g0 + 63) / 64 + (n - fs + 63) / 64;
+          n_diag += P;
+          n_below += rows - P;
+        }
+        const int per = (n_diag < slots && n_diag + n_below > slots)
+                            ? std::min(16, (n_below + (slots - n_diag) - 1) / (slots - n_diag))
+                            : 1;
+        std::vector<uhmk_item> below;
+        for (int f : big) {
+          const int fs = fr_[f].fs, n = fr_[f].n;
+          if (fs <= g0) continue;
+          const int P = (std::min(g0 + GW, fs) - g0 + 63) / 64;
+          const int rows = (fs - g0 + 63) / 64 + (n - fs + 63) / 64;
+          const int slot = flag_slots_;
+          flag_slots_ += 16;
+          for (int r = 0; r < P; ++r) items_.push_back(uhmk_item{f, r, 1, slot});
+          for (int r = P; r < rows; r += per) below.push_back(uhmk_item{f, r, std::min(per, rows - r), slot});
+        }
+        items_.insert(items_.end(), below.begin(), below.end());
+        lk.count = (int)(items_.size() - lk.off);
+        if (lk.count) push_main(lk);
+        emit_trailing(g0, GW);
+      }
+      maxfs = 0;  // the panel loop below has nothing left to do
+    }
     for (int c0 = 0; c0 < maxfs; c0 += nb) {
       const int g0 = c0 - c0 % (G * nb);  // first column of this panel's group
       const bool group_last = (c0 + nb >= g0 + G * nb) || (c0 + nb >= maxfs);
@@ -753,6 +850,9 @@ void FrontSolver::build_schedule(int method) {
     // mode 2: the Schur complement products take BF16 operands; the panel
     // (trailing) updates, which form the factor columns themselves, stay FP32
     l.mma = prec_mode_ == 0 ? 0 : (prec_mode_ == 2 && l.kind == Launch::SCHUR ? 2 : 1);
+    // the group launches' small in-group products in the storage type (FP64
+    // products on FP64 fronts measured faster there than FP32 ones)
+    if (l.kind == Launch::CHOL_GROUP) l.mma = l.prec == 0 ? 0 : 1;
   }
   // Algorithmic work of every launch: the flops / bytes the operation needs,
   // not what a kernel happens to execute (diagonal SYRK tiles count their
@@ -801,6 +901,30 @@ void FrontSolver::build_schedule(int method) {
           }
         }
         break;
+      case Launch::CHOL_GROUP:
+        for (int q = 0; q < l.count; ++q) {
+          const uhmk_item& it = items_[l.off + q];
+          const int fs = fr_[it.f].fs, n = fr_[it.f].n;
+          const int ge = std::min(l.c0 + l.nb, fs), P = (ge - l.c0 + 63) / 64;
+          const int ratl = (fs - l.c0 + 63) / 64;
+          for (int t = it.a; t < it.a + it.b; ++t) {
+            const int i0 = t < ratl ? l.c0 + 64 * t : fs + 64 * (t - ratl);
+            const double h = std::min(i0 + 64, i0 < fs ? fs : n) - i0;
+            const int npan = t < P ? t + 1 : P;
+            for (int p = 0; p < npan; ++p) {
+              const int pc0 = l.c0 + 64 * p;
+              const double kb = std::min(64, ge - pc0), K = pc0 - l.c0;
+              if (t == p) {
+                fl += kb * (kb + 1.0) * K + flop_chol(kb);
+                by += 8.0 * K * kb + 16.0 * kb * kb;
+              } else {
+                fl += 2.0 * h * kb * K + kb * kb * h;
+                by += 8.0 * K * (h + kb) + 16.0 * h * kb;
+              }
+            }
+          }
+        }
+        break;
       case Launch::LU_FRONT:
         for (int q = 0; q < l.count; ++q) {
           const int f = flists_[l.off + q];
@@ -833,6 +957,7 @@ void FrontSolver::build_schedule(int method) {
   }
   d_items_.upload(items_, stream_);
   d_flists_.upload(flists_, stream_);
+  d_flags_.reserve((size_t)flag_slots_ * sizeof(int));
   check_hip(hipStreamSynchronize(stream_), "schedule upload");
   if (factor_graph_) (void)hipGraphExecDestroy(factor_graph_);
   factor_graph_ = nullptr;
@@ -850,6 +975,7 @@ void FrontSolver::enqueue_factor(int method, std::vector<hipEvent_t>* pev) {
   int* P = d_piv_.as<int>();
   int* info = d_info_.as<int>();
   check_k(uhmk_fill_int(info, (long long)(d_info_.bytes / sizeof(int)), 0, stream_), "clear info");
+  check_k(uhmk_fill_int(d_flags_.as<int>(), flag_slots_, 0, stream_), "clear group flags");
   for (size_t li = 0; li < launches_.size(); ++li) {
     const Launch& l = launches_[li];
     int rc = 0;
@@ -886,6 +1012,9 @@ void FrontSolver::enqueue_factor(int method, std::vector<hipEvent_t>* pev) {
         break;
       case Launch::SCHUR:
         rc = uhmk_gemm_update(F, IT + l.off, l.count, 0, 0, 1, l.extra, l.tile, A, l.prec, l.mma, st);
+        break;
+      case Launch::CHOL_GROUP:
+        rc = uhmk_chol_group(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, l.mma, d_flags_.as<int>(), info, st);
         break;
     }
     check_k(rc, "factor");
@@ -952,10 +1081,15 @@ void FrontSolver::set_tuning(int key, int value) {
     if (value != 0 && value != 1) throw std::runtime_error("set_tuning: diagonal fusion is 0 or 1");
     if (value != fuse_diag_) schedule_method_ = 0;
     fuse_diag_ = value;
+  } else if (key == 7) {
+    if (value < 0 || value > 2) throw std::runtime_error("set_tuning: panel-group launches are 0, 1 or 2");
+    if (value != group_kernel_) schedule_method_ = 0;
+    group_kernel_ = value;
   } else {
     throw std::runtime_error("set_tuning: unknown key (1: Cholesky panel group, 2: 128-tile threshold, "
                              "3: block-solve threshold, 4: whole-front LU limit, 5: Cholesky lookahead, "
-                             "6: Cholesky diagonal blocks fused into the panel updates)");
+                             "6: Cholesky diagonal blocks fused into the panel updates, 7: Cholesky panel groups "
+                             "in one persistent launch)");
   }
 }
 
@@ -966,6 +1100,7 @@ int FrontSolver::get_tuning(int key) const {
   if (key == 4) return front_small_;
   if (key == 5) return lookahead_;
   if (key == 6) return fuse_diag_;
+  if (key == 7) return group_kernel_;
   throw std::runtime_error("get_tuning: unknown key");
 }
 
@@ -1073,6 +1208,11 @@ int FrontSolver::factor(int method) {
   (void)hipEventDestroy(e1);
   std::vector<int> hinfo(fr_.size());
   check_hip(hipMemcpy(hinfo.data(), d_info_.p, hinfo.size() * sizeof(int), hipMemcpyDeviceToHost), "info");
+  int gave_up = 0;
+  check_hip(hipMemcpy(&gave_up, d_flags_.p, sizeof(int), hipMemcpyDeviceToHost), "group flags");
+  if (gave_up)
+    throw std::runtime_error("factor: " + std::to_string(gave_up) +
+                             " panel-group waits gave up (the factors are invalid)");
   factored_ = true;
   last_method_ = method;
   factor_nb_ = nb_;

@@ -46,7 +46,9 @@ struct DevBuf {
 struct Launch {
   // GEMM_DIAG: a Cholesky panel update whose diagonal tiles then factor the
   // next panel's diagonal block (items with c == 1)
-  enum Kind { ASSEMBLE, LU_PANEL, LU_ROWS, CHOL_DIAG, CHOL_TRSM, GEMM, SCHUR, LU_FRONT, GEMM_DIAG } kind;
+  // CHOL_GROUP: every panel of a Cholesky panel group (diagonal blocks,
+  // TRSM, in-group updates) in one persistent launch
+  enum Kind { ASSEMBLE, LU_PANEL, LU_ROWS, CHOL_DIAG, CHOL_TRSM, GEMM, SCHUR, LU_FRONT, GEMM_DIAG, CHOL_GROUP } kind;
   int c0 = 0, nb = 0;
   long long off = 0;  // offset into the item (or front-list) array
   int count = 0;
@@ -61,7 +63,7 @@ struct Launch {
   int wait_ev = -1;   // event the launch's stream waits for first (-1: none)
   int record_ev = -1; // event recorded on the launch's stream after it (-1: none)
 };
-constexpr int N_LAUNCH_KINDS = 9;
+constexpr int N_LAUNCH_KINDS = 10;
 
 // One launch of the level solves.
 struct SolveStep {
@@ -229,6 +231,9 @@ class FrontSolver {
   int front_small_ = 64;  // LU fronts with n <= this (at most 64) factor in one whole-front launch
   int lookahead_ = 0;     // Cholesky: next panel's diagonal block and TRSM on the lookahead stream
   int fuse_diag_ = 1;     // Cholesky: next panel's diagonal block factored by the update's diagonal tile
+  int group_kernel_ = 2;  // Cholesky: panel groups in one persistent launch (uhmk_chol_group): 0 never,
+                          // 1 every level, 2 levels with at most 16 fronts
+  int flag_slots_ = 1;    // ints of d_flags_ the schedule uses (progress flags of the group launches)
   hipStream_t side_stream_ = nullptr;
   std::vector<hipEvent_t> la_ev_;
   int final_wait_ev_ = -1;
@@ -275,6 +280,7 @@ class FrontSolver {
   DevBuf dF_, d_emap_, d_inv_, d_srcbeg_, d_srcs_, d_lmap_, d_goff_;
   DevBuf d_items_, d_flists_, d_gen_items_, d_lvl_, d_sitems_, d_sflist_, d_fsl_, d_perm_, d_pperm_;
   DevBuf d_arena_, d_arena32_, d_piv_, d_x_, d_b_, d_stage_, d_info_;
+  DevBuf d_flags_;  // progress flags of the Cholesky group launches, [0]: waits that gave up
   DevBuf d_rf_;  // refinement right-hand side (front layout)
   DevBuf d_z_;   // second solve vector (front layout) of the block-step Cholesky solves
   DevBuf d_part_;  // split-K partial tiles of the block-solve products
