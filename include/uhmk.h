@@ -204,6 +204,26 @@ int uhmk_solve_gemv(const uhmk_front* fronts, const uhmk_item* items, int n_item
 int uhmk_solve_chol_step(const uhmk_front* fronts, const uhmk_item* items, int n_items, int fwd, int k0,
                          const void* arena, int prec, double* x, double* z, int nrhs, hipStream_t stream);
 
+/* The Cholesky solves of one tree level in one launch per direction, for
+ * factors whose diagonal blocks carry the inverse from uhmk_chol_diag /
+ * uhmk_chol_group (solve_chol_{1,2}_hier,
+ * UHM/src/matrix/uhm/fla/chol/solve.cxx:152-202): the 64-row block steps of
+ * uhmk_solve_chol_step and the ABL products of uhmk_solve_gemv, with the
+ * dependencies between the blocks resolved inside the launch.  fwd = 1:
+ * items {f, 0, first row r0, 0} for every 64-row block of ATL (each front's
+ * in increasing r0), then {f, 1, first row, 0} for every 64-row chunk of
+ * ABL; z[0:fs) <- L^-1 x[0:fs) (z marked by uhmk_solve_mark beforehand),
+ * x[fs:n) -= ABL z[0:fs), and the ATL rows of x are marked for the
+ * backward launch.  fwd = 0: items {f, 0, r0, 0} for every block of ATL,
+ * each front's in decreasing r0; x[0:fs) <- L^-T (z[0:fs) - ABL^T x[fs:n)).
+ * x, z: the front layout of the right-hand sides; nrhs <= 8.  err: one int,
+ * zero before; raised if a wait gave up (the results are then invalid). */
+int uhmk_solve_chol_level(const uhmk_front* fronts, const uhmk_item* items, int n_items, int fwd,
+                          const void* arena, int prec, double* x, double* z, int nrhs, int* err, hipStream_t stream);
+/* p[0..n) = the mark of blocks not yet solved (a NaN bit pattern), as a
+ * kernel (usable inside a captured graph). */
+int uhmk_solve_mark(double* p, long long n, hipStream_t stream);
+
 /* Block solves with many right-hand sides (FP64 MFMA over tiles of 64
  * right-hand sides; items {f, first right-hand side of the tile, first
  * row, 0}).  One 64-row block step k0 of the ATL triangular solves of a

@@ -782,6 +782,232 @@ __global__ __launch_bounds__(256) void k_fill_int(int* __restrict__ p, long long
   if (i < n) p[i] = v;
 }
 
+// ------------------------------------------------------------------------
+// Cholesky level solves in one launch per tree level and direction, for
+// factors whose 64 x 64 diagonal blocks carry W = L_kk^-1 (the solves of
+// solve_chol_{1,2}_hier, UHM/src/matrix/uhm/fla/chol/solve.cxx:152-202,
+// for every front of a level at once).  One workgroup per 64-row block of
+// ATL, left-looking: forward, block r applies L(r, j) y_j of every earlier
+// block j as soon as y_j appears, then y_r = W_r (x_r - sum) goes to z;
+// workgroups for 64-row chunks of ABL apply the same terms to xb.
+// Backward, block r (listed in decreasing order) forms
+// z_r - ABL(:, r)^T xb - sum_{j > r} L(j, r)^T x_j and x_r = W_r^T of it.
+// A block's result is read by the others of the launch while they run:
+// the solution blocks start as SOLVE_MARK (a NaN bit pattern that no
+// arithmetic produces: z by uhmk_solve_mark before the forward pass, the
+// ATL rows of x by the forward pass itself once it has read them), the
+// result is stored with agent-scope coherent stores and the readers poll
+// their values with coherent loads until the mark is gone (one memory round
+// trip per dependency, no flags).  Polls are bounded: after SOLVE_SPIN
+// rounds *err is raised and the launch finishes with invalid values
+// instead of hanging.  Every wait is on a block of the same front listed
+// earlier in the launch, so the in-order dispatch of workgroups leaves the
+// awaited block running or done.
+// ------------------------------------------------------------------------
+constexpr unsigned long long SOLVE_MARK = 0xFFF4B17E5E4D1A00ULL;
+constexpr int SOLVE_SPIN = 1 << 22;
+constexpr int SOLVE_MAXR = 8;  // right-hand sides per level launch
+
+__device__ __forceinline__ bool is_mark(double v) {
+  return (unsigned long long)__double_as_longlong(v) == SOLVE_MARK;
+}
+
+__device__ __forceinline__ double poll_value(double* p, int* err) {
+  double v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int spins = 0;
+  while (is_mark(v)) {
+    if ((++spins & 1023) == 0 &&
+        (spins > SOLVE_SPIN || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+      if (spins > SOLVE_SPIN) atomicAdd(err, 1);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+    v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return v;
+}
+
+__device__ __forceinline__ void store_coh(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// item {f, 0 ATL block / 1 ABL chunk, first row, 0}; thread (w, lane): row
+// r0 + lane, columns 16 w .. 16 w + 15 of each block term
+template <typename T>
+__global__ __launch_bounds__(256) void k_sv_chol_fwd_level(const uhmk_front* __restrict__ F,
+                                                           const uhmk_item* __restrict__ items,
+                                                           const T* __restrict__ arena, double* __restrict__ x,
+                                                           double* __restrict__ z, int nrhs, int* __restrict__ err) {
+  __shared__ double sW[64 * 65];
+  __shared__ double sy[SOLVE_MAXR][64];
+  __shared__ double sp[SOLVE_MAXR][4][64];
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs, r0 = it.b;
+  const bool atl = it.a == 0;
+  const T* __restrict__ A = arena + fr.off;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int row = r0 + lane;
+  const int rend = atl ? min(r0 + 64, fs) : n;
+  const int nterm = atl ? r0 / 64 : (fs + 63) / 64;
+  double* __restrict__ xf = x + fr.xoff;
+  double* __restrict__ zf = z + fr.xoff;
+  double part[SOLVE_MAXR];
+#pragma unroll
+  for (int c = 0; c < SOLVE_MAXR; ++c) part[c] = 0.0;
+  for (int j = 0; j < nterm; ++j) {
+    const int c0 = 64 * j, kbj = min(64, fs - c0);
+    double e[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = 16 * w + q;
+      e[q] = uhmd::ldz(A, row + (long long)(c0 + k) * n, row < rend && k < kbj);
+    }
+    if (tid < 64)
+      for (int c = 0; c < nrhs; ++c) sy[c][tid] = tid < kbj ? poll_value(zf + (long long)c * n + c0 + tid, err) : 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < SOLVE_MAXR; ++c)
+      if (c < nrhs) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) part[c] = fma(e[q], sy[c][16 * w + q], part[c]);
+      }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < SOLVE_MAXR; ++c)
+    if (c < nrhs) sp[c][w][lane] = part[c];
+  __syncthreads();
+  if (!atl) {
+    if (tid < 64 && row < n)
+      for (int c = 0; c < nrhs; ++c)
+        xf[(long long)c * n + row] -= (sp[c][0][tid] + sp[c][1][tid]) + (sp[c][2][tid] + sp[c][3][tid]);
+    return;
+  }
+  const int kb = rend - r0;
+  stage_diag_block(sW, A + r0 + (long long)r0 * n, n, kb, tid);
+  if (tid < 64)
+    for (int c = 0; c < nrhs; ++c)
+      sy[c][tid] = tid < kb ? xf[(long long)c * n + row] -
+                                  ((sp[c][0][tid] + sp[c][1][tid]) + (sp[c][2][tid] + sp[c][3][tid]))
+                            : 0.0;
+  __syncthreads();
+  // y_j = acc_j / L(j,j) + sum_{k<j} W(j,k) acc_k, four threads per j
+  for (int c = 0; c < nrhs; ++c) {
+    const int jj = lane, q4 = w;
+    double acc = 0.0;
+    if (jj < kb) {
+      for (int k = 16 * q4; k < 16 * q4 + 16 && k < jj; ++k) acc = fma(sW[k * 65 + jj], sy[c][k], acc);
+      if (q4 == 0) acc += sy[c][jj] / sW[jj * 65 + jj];
+    }
+    sp[c][q4][jj] = acc;
+  }
+  __syncthreads();
+  if (tid < kb)
+    for (int c = 0; c < nrhs; ++c) {
+      store_coh(zf + (long long)c * n + row,
+                (sp[c][0][tid] + sp[c][1][tid]) + (sp[c][2][tid] + sp[c][3][tid]));
+      // the block's rows of x are read: they become the backward pass's mark
+      xf[(long long)c * n + row] = __longlong_as_double((long long)SOLVE_MARK);
+    }
+}
+
+// item {f, 0, first row of the ATL block, 0}, each front's blocks in
+// decreasing order.  Every 64 x 64 piece of the block's columns (ABL rows,
+// then the ATL blocks below) is staged through LDS with coalesced loads
+// (thread t: row t % 64, columns t / 64 + 4 u), the next one loaded while
+// the current one is applied; thread (w, lane) then sums column lane over
+// rows 16 w .. 16 w + 15 of the piece.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sv_chol_bwd_level(const uhmk_front* __restrict__ F,
+                                                           const uhmk_item* __restrict__ items,
+                                                           const T* __restrict__ arena, double* __restrict__ x,
+                                                           double* __restrict__ z, int nrhs, int* __restrict__ err) {
+  __shared__ double sW[64 * 65];  // the piece, then the diagonal block
+  __shared__ double sy[SOLVE_MAXR][64];
+  __shared__ double sp[SOLVE_MAXR][4][64];
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs, r0 = it.b;
+  const int kb = min(64, fs - r0);
+  const T* __restrict__ A = arena + fr.off;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int col = r0 + lane;
+  double* __restrict__ xf = x + fr.xoff;
+  double* __restrict__ zf = z + fr.xoff;
+  double part[SOLVE_MAXR];
+#pragma unroll
+  for (int c = 0; c < SOLVE_MAXR; ++c) part[c] = 0.0;
+  // pieces: ABL rows [fs, n) in 64-row chunks, then the ATL blocks below
+  // this one from the last (the first to be solved) upwards
+  const int nabl = (n - fs + 63) / 64, nblk = (fs + 63) / 64, rb = r0 / 64;
+  const int npiece = nabl + (nblk - 1 - rb);
+  auto piece_row = [&](int pc) { return pc < nabl ? fs + 64 * pc : 64 * (nblk - 1 - (pc - nabl)); };
+  double v[16];
+  auto load_piece = [&](int pc) {
+    const int p0 = piece_row(pc), pend = pc < nabl ? n : min(p0 + 64, fs);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int cc = w + 4 * u;
+      v[u] = uhmd::ldz(A, (p0 + lane) + (long long)(r0 + cc) * n, p0 + lane < pend && cc < kb);
+    }
+  };
+  if (npiece > 0) load_piece(0);
+  for (int pc = 0; pc < npiece; ++pc) {
+    const int p0 = piece_row(pc);
+    const bool abl = pc < nabl;
+    const int pend = abl ? n : min(p0 + 64, fs);
+    // the piece into LDS (sW[row][column]), the next piece's loads issued
+#pragma unroll
+    for (int u = 0; u < 16; ++u) sW[lane * 65 + w + 4 * u] = v[u];
+    if (pc + 1 < npiece) load_piece(pc + 1);
+    // the multiplying values: xb rows of the chunk, or the solved block
+    if (tid < 64)
+      for (int c = 0; c < nrhs; ++c)
+        sy[c][tid] = p0 + tid >= pend ? 0.0
+                     : abl         ? xf[(long long)c * n + p0 + tid]
+                                   : poll_value(xf + (long long)c * n + p0 + tid, err);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < SOLVE_MAXR; ++c)
+      if (c < nrhs) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) part[c] = fma(sW[(16 * w + q) * 65 + lane], sy[c][16 * w + q], part[c]);
+      }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < SOLVE_MAXR; ++c)
+    if (c < nrhs) sp[c][w][lane] = part[c];
+  stage_diag_block(sW, A + r0 + (long long)r0 * n, n, kb, tid);
+  __syncthreads();
+  if (tid < 64)
+    for (int c = 0; c < nrhs; ++c)
+      sy[c][tid] = tid < kb ? zf[(long long)c * n + col] -
+                                  ((sp[c][0][tid] + sp[c][1][tid]) + (sp[c][2][tid] + sp[c][3][tid]))
+                            : 0.0;
+  __syncthreads();
+  // x_j = acc_j / L(j,j) + sum_{k>j} W(k,j) acc_k; W(k,j) is block entry (j, k)
+  for (int c = 0; c < nrhs; ++c) {
+    const int jj = lane, q4 = w;
+    double acc = 0.0;
+    if (jj < kb) {
+      for (int k = max(jj + 1, 16 * q4); k < 16 * q4 + 16 && k < kb; ++k) acc = fma(sW[jj * 65 + k], sy[c][k], acc);
+      if (q4 == 0) acc += sy[c][jj] / sW[jj * 65 + jj];
+    }
+    sp[c][q4][jj] = acc;
+  }
+  __syncthreads();
+  if (tid < kb)
+    for (int c = 0; c < nrhs; ++c)
+      store_coh(xf + (long long)c * n + col, (sp[c][0][tid] + sp[c][1][tid]) + (sp[c][2][tid] + sp[c][3][tid]));
+}
+
+__global__ __launch_bounds__(256) void k_fill_mark(double* __restrict__ p, long long n) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) p[i] = __longlong_as_double((long long)SOLVE_MARK);
+}
+
 inline int ret(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
 
 template <typename T, bool CHOL, bool FWD, bool INV>
@@ -898,6 +1124,38 @@ int uhmk_solve_chol_step(const uhmk_front* fronts, const uhmk_item* items, int n
   } else {
     return -1006;
   }
+  return ret(hipGetLastError());
+}
+
+int uhmk_solve_chol_level(const uhmk_front* fronts, const uhmk_item* items, int n_items, int fwd,
+                          const void* arena, int prec, double* x, double* z, int nrhs, int* err, hipStream_t stream) {
+  if (n_items <= 0 || nrhs <= 0) return 0;
+  if (nrhs > SOLVE_MAXR) return -1014;
+  if (prec == 0) {
+    if (fwd)
+      hipLaunchKernelGGL(k_sv_chol_fwd_level<double>, dim3(n_items), dim3(256), 0, stream, fronts, items,
+                         static_cast<const double*>(arena), x, z, nrhs, err);
+    else
+      hipLaunchKernelGGL(k_sv_chol_bwd_level<double>, dim3(n_items), dim3(256), 0, stream, fronts, items,
+                         static_cast<const double*>(arena), x, z, nrhs, err);
+  } else if (prec == 1) {
+    if (fwd)
+      hipLaunchKernelGGL(k_sv_chol_fwd_level<float>, dim3(n_items), dim3(256), 0, stream, fronts, items,
+                         static_cast<const float*>(arena), x, z, nrhs, err);
+    else
+      hipLaunchKernelGGL(k_sv_chol_bwd_level<float>, dim3(n_items), dim3(256), 0, stream, fronts, items,
+                         static_cast<const float*>(arena), x, z, nrhs, err);
+  } else {
+    return -1006;
+  }
+  return ret(hipGetLastError());
+}
+
+int uhmk_solve_mark(double* p, long long n, hipStream_t stream) {
+  if (n <= 0) return 0;
+  const long long nb = (n + 255) / 256;
+  if (nb > 0x7fffffffLL) return -1011;
+  hipLaunchKernelGGL(k_fill_mark, dim3((unsigned)nb), dim3(256), 0, stream, p, n);
   return ret(hipGetLastError());
 }
 

@@ -272,6 +272,8 @@ void FrontSolver::create_buffers() {
   d_arena32_.alloc(std::max(1LL, arena32_elems_) * sizeof(float));
   d_rf_.alloc(std::max(1LL, x_elems_) * sizeof(double));
   d_z_.alloc(std::max(1LL, x_elems_) * sizeof(double));
+  d_serr_.alloc(sizeof(int));
+  check_hip(hipMemset(d_serr_.p, 0, sizeof(int)), "solve error word");
   check_hip(hipMemsetAsync(d_rf_.p, 0, d_rf_.bytes, stream_), "memset rf");
   d_piv_.alloc(std::max(1, piv_elems_) * sizeof(int));
   d_perm_.alloc(std::max(1, piv_elems_) * sizeof(int));
@@ -1081,6 +1083,10 @@ void FrontSolver::set_tuning(int key, int value) {
     if (value != 0 && value != 1) throw std::runtime_error("set_tuning: diagonal fusion is 0 or 1");
     if (value != fuse_diag_) schedule_method_ = 0;
     fuse_diag_ = value;
+  } else if (key == 8) {
+    if (value != 0 && value != 1) throw std::runtime_error("set_tuning: level solves are 0 or 1");
+    if (value != solve_level_) solve_method_ = 0;
+    solve_level_ = value;
   } else if (key == 7) {
     if (value < 0 || value > 2) throw std::runtime_error("set_tuning: panel-group launches are 0, 1 or 2");
     if (value != group_kernel_) schedule_method_ = 0;
@@ -1089,7 +1095,7 @@ void FrontSolver::set_tuning(int key, int value) {
     throw std::runtime_error("set_tuning: unknown key (1: Cholesky panel group, 2: 128-tile threshold, "
                              "3: block-solve threshold, 4: whole-front LU limit, 5: Cholesky lookahead, "
                              "6: Cholesky diagonal blocks fused into the panel updates, 7: Cholesky panel groups "
-                             "in one persistent launch)");
+                             "in one persistent launch, 8: Cholesky level solves in one launch per level)");
   }
 }
 
@@ -1101,6 +1107,7 @@ int FrontSolver::get_tuning(int key) const {
   if (key == 5) return lookahead_;
   if (key == 6) return fuse_diag_;
   if (key == 7) return group_kernel_;
+  if (key == 8) return solve_level_;
   throw std::runtime_error("get_tuning: unknown key");
 }
 
@@ -1394,6 +1401,44 @@ void FrontSolver::build_solve_schedule(int method) {
   };
   if (blk) {
     build_block_solve_schedule(method);
+  } else if (inv && solve_level_ && nrhs_ <= 8) {
+    // one launch per level and direction (at most 8 right-hand sides per
+    // launch); z marked first
+    SolveStep mk;
+    mk.kind = SolveStep::MARK;
+    sfwd_.push_back(mk);
+    for (int L = 0; L < nl; ++L) {
+      int o = (int)sitems_.size();
+      for (int f : levels_[L])
+        for (int r = 0; r < fr_[f].n; r += 256) sitems_.push_back(uhmk_item{f, 0, r, 0});
+      sfwd_.push_back(step(SolveStep::GATHER, o));
+      o = (int)sitems_.size();
+      for (int f : levels_[L])
+        for (int r = 0; r < fr_[f].fs; r += 64) sitems_.push_back(uhmk_item{f, 0, r, 0});
+      for (int f : levels_[L])
+        if (fr_[f].fs > 0)
+          for (int r = fr_[f].fs; r < fr_[f].n; r += 64) sitems_.push_back(uhmk_item{f, 1, r, 0});
+      SolveStep lv = step(SolveStep::LEVEL, o);
+      lv.fwd = 1;
+      lv.prec = level_prec_[L];
+      if (lv.cnt) sfwd_.push_back(lv);
+    }
+    for (int L = nl - 1; L >= 0; --L) {
+      int o = (int)sitems_.size();
+      for (int f : levels_[L])
+        if (fr_[f].parent >= 0)
+          for (int r = 0; r < fr_[f].n - fr_[f].fs; r += 256) sitems_.push_back(uhmk_item{f, 0, r, 0});
+      SolveStep br = step(SolveStep::BRANCH, o);
+      if (br.cnt) sbwd_.push_back(br);
+      o = (int)sitems_.size();
+      for (int f : levels_[L])
+        for (int r = ((fr_[f].fs - 1) / 64) * 64; r >= 0 && fr_[f].fs > 0; r -= 64)
+          sitems_.push_back(uhmk_item{f, 0, r, 0});
+      SolveStep lv = step(SolveStep::LEVEL, o);
+      lv.fwd = 0;
+      lv.prec = level_prec_[L];
+      if (lv.cnt) sbwd_.push_back(lv);
+    }
   } else if (inv) {
     // Cholesky factors with diagonal-block inverses: 64-row block steps
     for (int L = 0; L < nl; ++L) {
@@ -1547,6 +1592,14 @@ rc = (st.cols ? uhmk_solve_gather_cols : uhmk_solve_gather)(
         rc = uhmk_solve_block_step(F, SI + st.off, st.cnt, st.mode, st.fwd, st.k0, A, st.prec, X, d_z_.as<double>(),
                                    nrhs_, stream_);
         break;
+      case SolveStep::LEVEL:
+        rc = uhmk_solve_chol_level(F, SI + st.off, st.cnt, st.fwd, A, st.prec, X, d_z_.as<double>(), nrhs_,
+                                   d_serr_.as<int>(), stream_);
+        break;
+      case SolveStep::MARK:
+        rc = uhmk_solve_mark(d_z_.as<double>(), x_elems_, stream_);
+        if (!rc) rc = uhmk_fill_int(d_serr_.as<int>(), 1, 0, stream_);
+        break;
       case SolveStep::BGEMM:
         rc = uhmk_solve_block_gemm(F, SI + st.off, st.cnt, st.mode, A, st.prec, X, d_z_.as<double>(), nrhs_, st.ks,
                                    d_part_.as<double>(), stream_);
@@ -1590,6 +1643,13 @@ void FrontSolver::solve(int method) {
   t_solve_ms_ = ms;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  check_solve_err();
+}
+
+void FrontSolver::check_solve_err() {
+  int e = 0;
+  check_hip(hipMemcpy(&e, d_serr_.p, sizeof(int), hipMemcpyDeviceToHost), "solve error word");
+  if (e) throw std::runtime_error("solve: " + std::to_string(e) + " level-solve waits gave up (x is invalid)");
 }
 
 // ------------------------------------------------------------ assembled A
@@ -1989,6 +2049,7 @@ int FrontSolver::refine(int method, double tol, int max_iter) {
   }
   check_hip(hipStreamSynchronize(stream_), "refine");
   t_refine_ms_ = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  check_solve_err();
   return it;
 }
 

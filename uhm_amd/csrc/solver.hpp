@@ -67,7 +67,9 @@ constexpr int N_LAUNCH_KINDS = 10;
 
 // One launch of the level solves.
 struct SolveStep {
-  enum Kind { GATHER, TRSV, GEMV, BRANCH, CSTEP, BSTEP, BGEMM } kind = GATHER;
+  // LEVEL: the Cholesky solves of a level in one launch (fwd as for CSTEP);
+  // MARK: z marked as not yet solved and the error word cleared
+  enum Kind { GATHER, TRSV, GEMV, BRANCH, CSTEP, BSTEP, BGEMM, LEVEL, MARK } kind = GATHER;
   int off = 0, cnt = 0;            // items (gather / gemv / branch) or front list (trsv)
   int max_fs = 0, sb = 0, sb_size = 0;  // trsv
   int cols = 0;                    // gemv: row chunks (0) or column chunks (1); gather / branch:
@@ -231,6 +233,8 @@ class FrontSolver {
   int front_small_ = 64;  // LU fronts with n <= this (at most 64) factor in one whole-front launch
   int lookahead_ = 0;     // Cholesky: next panel's diagonal block and TRSM on the lookahead stream
   int fuse_diag_ = 1;     // Cholesky: next panel's diagonal block factored by the update's diagonal tile
+  int solve_level_ = 1;   // Cholesky solves with inverses: one launch per level and direction (1) or
+                          // one per 64-row block step (0)
   int group_kernel_ = 2;  // Cholesky: panel groups in one persistent launch (uhmk_chol_group): 0 never,
                           // 1 every level, 2 levels with at most 16 fronts
   int flag_slots_ = 1;    // ints of d_flags_ the schedule uses (progress flags of the group launches)
@@ -281,6 +285,8 @@ class FrontSolver {
   DevBuf d_items_, d_flists_, d_gen_items_, d_lvl_, d_sitems_, d_sflist_, d_fsl_, d_perm_, d_pperm_;
   DevBuf d_arena_, d_arena32_, d_piv_, d_x_, d_b_, d_stage_, d_info_;
   DevBuf d_flags_;  // progress flags of the Cholesky group launches, [0]: waits that gave up
+  DevBuf d_serr_;   // level solves: waits that gave up
+  void check_solve_err();
   DevBuf d_rf_;  // refinement right-hand side (front layout)
   DevBuf d_z_;   // second solve vector (front layout) of the block-step Cholesky solves
   DevBuf d_part_;  // split-K partial tiles of the block-solve products
