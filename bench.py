@@ -126,7 +126,7 @@ def run_path(U, m, w, mode, top, args):
     chol = w["method"] == 1
     factor = m.chol_with_free if chol else m.lu_piv_with_free
     solve = m.solve_chol if chol else m.solve_lu_piv
-    name = {U.PREC_FP64: "fp64", U.PREC_MIXED: "mixed", U.PREC_MIXED_BF16: "bf16"}[mode]
+    name = {U.PREC_FP64: "fp64", U.PREC_MIXED: "mixed", U.PREC_MIXED_BF16: "bf16", U.PREC_MIXED_FP16: "fp16"}[mode]
     m.set_precision(mode, top)
     m.set_refinement(REFINE_KINDS[args.refine or w.get("refine", "classical")](U), 20, 1e-6)
     tuned = None
@@ -360,6 +360,7 @@ def main():
         "mixed": _path(flops, rmx, r64, x_oracle),
         "fp64": _path(flops, r64, r64, x_oracle),
         "mixed_bf16": _path(flops, rbf, r64, x_oracle) if rbf else None,
+        "mixed_fp16": _path(flops, res["fp16"], r64, x_oracle) if res["fp16"] else None,
         "reseeds": res["reseeds"],
         "cpu_baseline": cpu,
     }
@@ -401,9 +402,10 @@ def measure(U, name, args, steps, warmup, bf16=False):
     r64 = run_path(U, m, w, U.PREC_FP64, top, a)
     rmx = run_path(U, m, w, U.PREC_MIXED, top, a)
     rbf = run_path(U, m, w, U.PREC_MIXED_BF16, top, a) if bf16 else None
+    rfp = run_path(U, m, w, U.PREC_MIXED_FP16, top, a) if bf16 else None
     n_reseeds = w.get("reseeds", 0) if args.reseeds is None else args.reseeds
     reseeds = run_reseeds(U, m, w, a, n_reseeds) if n_reseeds else None
-    return dict(name=name, w=w, m=m, flops=flops, fp64=r64, mixed=rmx, bf16=rbf, reseeds=reseeds)
+    return dict(name=name, w=w, m=m, flops=flops, fp64=r64, mixed=rmx, bf16=rbf, fp16=rfp, reseeds=reseeds)
 
 
 def _rel(a, b):
@@ -420,12 +422,12 @@ def _ttt(res):
     each path, and the FP64 path's time over each low-precision path's."""
     r64 = res["fp64"]
     out = {"fp64": round(r64["t_sol"], 3) if r64["eta"] <= 1e-12 else None}
-    for key, r in (("mixed", res["mixed"]), ("mixed_bf16", res["bf16"])):
+    for key, r in (("mixed", res["mixed"]), ("mixed_bf16", res["bf16"]), ("mixed_fp16", res["fp16"])):
         if r is None:
             continue
         ok = r["eta"] <= 1e-12
         out[key] = round(r["t_sol"], 3) if ok else None
-        out["speedup" if key == "mixed" else "speedup_bf16"] = round(r64["t_sol"] / r["t_sol"], 3) if ok else None
+        out["speedup" + key[5:]] = round(r64["t_sol"] / r["t_sol"], 3) if ok else None
     return out
 
 
@@ -451,7 +453,7 @@ def _compact(res):
     out = {"workload": w["name"], "ndof": res["m"].get_n_dof(), "fronts": res["m"].n_fronts(),
            "factor_gflop": round(flops / 1e9, 3), "n_rhs": w.get("nrhs", 1), "steps": EXTRA_STEPS,
            "time_to_1e-12_ms": _ttt(res)}
-    for key, r in (("fp64", r64), ("mixed", res["mixed"]), ("mixed_bf16", res["bf16"])):
+    for key, r in (("fp64", r64), ("mixed", res["mixed"]), ("mixed_bf16", res["bf16"]), ("mixed_fp16", res["fp16"])):
         if r is None:
             continue
         out[key] = {"factor_ms": round(r["ms_step"], 4), "factor_tflops": _tflops(flops, r),

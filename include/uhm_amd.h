@@ -150,8 +150,9 @@ void uhm_get_timing_(uhm_fort_p* mesh, uhm_fort_double* factor_ms, uhm_fort_doub
  * (final Schur complement), lu_front (whole-front LU of fronts with
  * n <= 64), gemm_diag (Cholesky panel update that also factors the next
  * panel's diagonal blocks), chol_group (a Cholesky panel group in one
- * persistent launch): milliseconds, algorithmic flops, algorithmic bytes
- * and launch counts (arrays of 10). */
+ * persistent launch), lp_pack (Schur operands packed to BF16 / FP16):
+ * milliseconds, algorithmic flops, algorithmic bytes and launch counts
+ * (arrays of 11). */
 void uhm_set_profile_(uhm_fort_p* mesh, uhm_fort_int* on);
 void uhm_get_kernel_stats_(uhm_fort_p* mesh, uhm_fort_double* ms, uhm_fort_double* flops, uhm_fort_double* bytes,
                            uhm_fort_int* launches);
@@ -162,9 +163,11 @@ void uhm_get_kernel_stats_(uhm_fort_p* mesh, uhm_fort_double* ms, uhm_fort_doubl
  * are stored in FP32, and every Schur / trailing update is an FP32 MFMA
  * product; the solution is then brought to FP64 accuracy with
  * uhm_improve_*_.  mode 2: as mode 1, but the Schur complement products
- * (ABR -= ABL ATR) take BF16 operands with FP32 accumulation; the panel
- * updates inside each front stay FP32.  Modes 1 and 2 need the refinement,
- * mode 2 usually more of it (GMRES-IR, uhm_set_refinement_). */
+ * (ABR -= ABL ATR) take BF16 operands with FP32 accumulation (packed once
+ * per front); the panel updates inside each front stay FP32.  mode 3: as
+ * mode 2 with FP16 operands, each front's operands scaled by a power of
+ * two into FP16's range.  Modes 1-3 need the refinement, modes 2 and 3
+ * usually more of it (GMRES-IR, uhm_set_refinement_). */
 void uhm_set_precision_(uhm_fort_p* mesh, uhm_fort_int* mode, uhm_fort_int* top_levels);
 /* Iterative refinement after uhm_solve_*_: x <- x + M^-1 (b - A x), the
  * residual in FP64 against the assembled matrix, M the last factorization,

@@ -148,6 +148,33 @@ int uhmk_chol_group(const uhmk_front* fronts, const uhmk_item* items, int n_item
 int uhmk_gemm_update(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, int schur,
                      int transb, int tile, void* arena, int prec, int mma, hipStream_t stream);
 
+/* Low-precision Schur complement products (the final ABR -= ABL ATR /
+ * ABL ABL^T of lu_piv_hier / chol_hier,
+ * UHM/src/matrix/uhm/fla/lu_piv/decompose.cxx:148-149,
+ * UHM/src/matrix/uhm/fla/chol/decompose.cxx:95-96) on packed BF16 / FP16
+ * operands with FP32 accumulation.  Packed layout of front f at element
+ * off[f] of lp: A = ABL as ss_pad rows of ldk entries (K contiguous), for LU
+ * B = ATR^T after it; ldk = fs rounded up to 64, ss_pad = ss rounded up to
+ * 128, zero padding; uhmk_schur_lp_size gives the entries a front needs.
+ * FP16 operands carry a power-of-two scale per front and operand, chosen
+ * from amax (2 unsigned per front: the float bits of max |entry| of A and
+ * B, zero before uhmk_schur_lp_pack) so that the largest entry lies in
+ * [2^14, 2^15); BF16 ones are unscaled.  Rounding: nearest even, once,
+ * from the storage value.
+ *   uhmk_schur_lp_pack  items {f, row tile, k tile, operand (0 A, 1 B)}
+ *                       over 64 x 64 tiles of the padded packed operands
+ *   uhmk_schur_lp       C -= A B^T (Cholesky: B = A) on tile x tile blocks
+ *                       of ABR (tile 64 or 128; 64 is the faster one, eight
+ *                       workgroups per CU), items {f, i0, j0, 0} (front
+ *                       indices, tiles cut at n); C updated in its storage
+ *                       type */
+long long uhmk_schur_lp_size(int n, int fs, int lu);
+int uhmk_schur_lp_pack(const uhmk_front* fronts, const uhmk_item* items, int n_items, int fp16, const void* arena,
+                       int prec, const long long* off, unsigned* amax, unsigned short* lp, hipStream_t stream);
+int uhmk_schur_lp(const uhmk_front* fronts, const uhmk_item* items, int n_items, int lu, int tile, int fp16,
+                  void* arena, int prec, const long long* off, const unsigned* amax, const unsigned short* lp,
+                  hipStream_t stream);
+
 /* ---- solve ---------------------------------------------------------------- */
 
 /* Composed row permutation of each LU front (perm[i] = row of the

@@ -14,8 +14,8 @@ cfg4  2D 256^2 quads, p_e ~ U{1..8}, K_e = M^T M + n I + 0.1 S, LU, FP32
       fronts, 64 right-hand sides: ten reseeds on the fixed pattern through
       the captured factorization / solve graphs, each refined to <= 1e-12.
 cfg5  3D 24^3 hex Q3 (389,017 dofs), SPD K_e = (M^T M + 64 I) lognormal(0,0.5),
-      Cholesky with BF16-operand Schur products, FP64 root front, 256
-      right-hand sides, GMRES-IR to <= 1e-12.
+      Cholesky with BF16-operand (and FP16-operand) Schur products, FP64 root
+      front, 256 right-hand sides, GMRES-IR to <= 1e-12.
 """
 import sys
 import time
@@ -150,7 +150,7 @@ def test_cfg4_varp_64rhs_graph_reseeds(lib):
     m.set_graph(False)
 
 
-@pytest.mark.parametrize("mode", [U.PREC_MIXED_BF16])
+@pytest.mark.parametrize("mode", [U.PREC_MIXED_BF16, U.PREC_MIXED_FP16])
 def test_cfg5_q3_24cube_low_precision_256rhs(lib, mode):
     say("case cfg5: 24^3 Q3 Cholesky, 256 RHS, precision mode", mode)
     m = _mesh(2, (24, 24, 24), 3, 256, seed=5, alpha=64.0, beta=0.0, sk=1, p0=0.5)

@@ -152,8 +152,8 @@ def _lu3d(nrhs, seed=10):
                          ids=lambda c: c[0])
 @pytest.mark.parametrize("top", [0, 1])
 def test_bf16_schur_refined_to_fp64_accuracy(lib, case, top):
-    """BF16-operand Schur complement products (FP32 accumulation), FP32
-    fronts below the top `top` levels: GMRES-IR brings every right-hand side
+    """BF16- and FP16-operand Schur complement products (FP32 accumulation),
+    FP32 fronts below the top `top` levels: GMRES-IR brings every right-hand side
     to backward error <= 1e-12 and to the FP64 path's solution within the
     condition-aware bar, and the unrefined BF16 solution is measurably less
     accurate than the FP32-Schur one (the BF16 products really ran)."""
@@ -166,7 +166,7 @@ def test_bf16_schur_refined_to_fp64_accuracy(lib, case, top):
     x64 = m.fetch_x().copy()
     kappa = kappa_1(m)
     eta0 = {}
-    for mode in (U.PREC_MIXED, U.PREC_MIXED_BF16):
+    for mode in (U.PREC_MIXED, U.PREC_MIXED_BF16, U.PREC_MIXED_FP16):
         m.set_precision(mode, top)
         assert _factor(m, method) == 0
         _solve(m, method)
@@ -183,3 +183,5 @@ def test_bf16_schur_refined_to_fp64_accuracy(lib, case, top):
         say("  kappa_1 ~ %.2e, forward error vs FP64 path %.3e (bar %.2e)" % (kappa, err, tol))
         assert err <= tol
     assert eta0[U.PREC_MIXED_BF16] > 10.0 * eta0[U.PREC_MIXED]
+    # FP16 keeps three more bits than BF16: its unrefined solution sits between
+    assert eta0[U.PREC_MIXED_FP16] > eta0[U.PREC_MIXED] and eta0[U.PREC_MIXED_FP16] < eta0[U.PREC_MIXED_BF16]

@@ -92,11 +92,11 @@ _SIGS = {
 }
 
 KERNEL_KINDS = ("assemble", "lu_panel", "lu_rows", "chol_diag", "chol_trsm", "gemm", "schur", "lu_front",
-                "gemm_diag", "chol_group")
+                "gemm_diag", "chol_group", "lp_pack")
 _RET = {"uhm_get_last_error_": C.c_int}
 
 
-PREC_FP64, PREC_MIXED, PREC_MIXED_BF16 = 0, 1, 2
+PREC_FP64, PREC_MIXED, PREC_MIXED_BF16, PREC_MIXED_FP16 = 0, 1, 2, 3
 REFINE_CLASSICAL, REFINE_GMRES, REFINE_AUTO = 0, 1, 2
 
 
@@ -304,8 +304,9 @@ class Mesh:
 
     def set_precision(self, mode, top_levels=2):
         """mode PREC_FP64 (0), PREC_MIXED (1: FP32 fronts below the top
-        `top_levels` levels, FP32 MFMA Schur updates) or PREC_MIXED_BF16 (2:
-        as 1 with BF16-operand, FP32-accumulate Schur complement products)."""
+        `top_levels` levels, FP32 MFMA Schur updates), PREC_MIXED_BF16 (2:
+        as 1 with BF16-operand, FP32-accumulate Schur complement products) or
+        PREC_MIXED_FP16 (3: as 2 with FP16 operands, scaled per front)."""
         _call("uhm_set_precision_", C.byref(self.handle), _i(mode), _i(top_levels))
 
     def improve(self, method, tol=1e-12, max_iter=20):

@@ -18,7 +18,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("UHM_AMD_ARCH", "gfx950")
 
 HOST_SRC = ["mesh.cpp", "synth.cpp", "solver.cpp", "capi.cpp"]
-HIP_SRC = ["kernels_factor.hip", "kernels_dense.hip", "kernels_solve.hip", "kernels_solve_block.hip", "kernels_krylov.hip"]
+HIP_SRC = ["kernels_factor.hip", "kernels_dense.hip", "kernels_solve.hip", "kernels_solve_block.hip", "kernels_krylov.hip",
+           "kernels_lowprec.hip"]
 
 INC = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "-I" + os.path.join(ROCM, "include")]
 CXXFLAGS = ["-O2", "-g0", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-D__HIP_PLATFORM_AMD__"]

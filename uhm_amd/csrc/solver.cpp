@@ -107,8 +107,9 @@ void FrontSolver::layout_arenas() {
 }
 
 void FrontSolver::set_precision(int mode, int top_levels) {
-  if (mode < 0 || mode > 2)
-    throw std::runtime_error("set_precision: mode must be 0 (FP64), 1 (mixed FP32) or 2 (mixed, BF16 Schur)");
+  if (mode < 0 || mode > 3)
+    throw std::runtime_error("set_precision: mode must be 0 (FP64), 1 (mixed FP32), 2 (mixed, BF16 Schur) or 3 "
+                             "(mixed, FP16 Schur)");
   if (top_levels < 0) throw std::runtime_error("set_precision: top_levels must be >= 0");
   if (mode == prec_mode_ && top_levels == top_fp64_) return;
   prec_mode_ = mode;
@@ -817,6 +818,7 @@ void FrontSolver::build_schedule(int method) {
     ls.kind = Launch::SCHUR;
     ls.level = L;
     ls.extra = chol ? 1 : 0;
+    const bool lowp = prec_mode_ >= 2;
     {
       double sss = 0;
       int cnt = 0;
@@ -825,7 +827,27 @@ void FrontSolver::build_schedule(int method) {
           sss += fr_[f].n - fr_[f].fs;
           ++cnt;
         }
-      ls.tile = cnt ? gemm_tile_for(sss / cnt, sss / cnt, tile_threshold_) : 64;
+      // the packed low-precision products run 64-wide tiles at every size
+      // (eight workgroups per CU hide their load latency)
+      ls.tile = cnt && !lowp ? gemm_tile_for(sss / cnt, sss / cnt, tile_threshold_) : 64;
+    }
+    if (lowp) {
+      // pack the operands of every front of the level (64 x 64 tiles over
+      // the padded packed rows and K)
+      Launch lp;
+      lp.kind = Launch::LP_PACK;
+      lp.level = L;
+      lp.off = (long long)items_.size();
+      for (int f : big) {
+        const int fs = fr_[f].fs, ss = fr_[f].n - fs;
+        if (!fs || !ss) continue;
+        const int rt = ((ss + 127) / 128) * 2, kt = (fs + 63) / 64;
+        for (int op = 0; op < (chol ? 1 : 2); ++op)
+          for (int a = 0; a < rt; ++a)
+            for (int b = 0; b < kt; ++b) items_.push_back(uhmk_item{f, a, b, op});
+      }
+      lp.count = (int)(items_.size() - lp.off);
+      if (lp.count) push_main(lp);
     }
     const int T = ls.tile;
     ls.off = (long long)items_.size();
@@ -851,7 +873,8 @@ void FrontSolver::build_schedule(int method) {
     l.prec = level_prec_[l.level];
     // mode 2: the Schur complement products take BF16 operands; the panel
     // (trailing) updates, which form the factor columns themselves, stay FP32
-    l.mma = prec_mode_ == 0 ? 0 : (prec_mode_ == 2 && l.kind == Launch::SCHUR ? 2 : 1);
+    // mma 2 / 3: packed BF16 / FP16 Schur products
+    l.mma = prec_mode_ == 0 ? 0 : (prec_mode_ >= 2 && l.kind == Launch::SCHUR ? prec_mode_ : 1);
     // the group launches' small in-group products in the storage type (FP64
     // products on FP64 fronts measured faster there than FP32 ones)
     if (l.kind == Launch::CHOL_GROUP) l.mma = l.prec == 0 ? 0 : 1;
@@ -901,6 +924,14 @@ void FrontSolver::build_schedule(int method) {
             if (it.a == 2) fl += kb * kb * w;
             by += 16.0 * 2.0 * kb * w;
           }
+        }
+        break;
+      case Launch::LP_PACK:
+        for (int q = 0; q < l.count; ++q) {
+          const uhmk_item& it = items_[l.off + q];
+          const int fs = fr_[it.f].fs, ss = fr_[it.f].n - fs;
+          const double rows = std::max(0, std::min(64, ss - 64 * it.a)), ks = std::max(0, std::min(64, fs - 64 * it.b));
+          by += 8.0 * rows * ks + 2.0 * 64 * 64;  // read (FP64 units, halved below for FP32), write packed
         }
         break;
       case Launch::CHOL_GROUP:
@@ -960,6 +991,19 @@ void FrontSolver::build_schedule(int method) {
   d_items_.upload(items_, stream_);
   d_flists_.upload(flists_, stream_);
   d_flags_.reserve((size_t)flag_slots_ * sizeof(int));
+  if (prec_mode_ >= 2) {
+    // packed operands of every front with a Schur complement
+    std::vector<long long> lpoff(fr_.size(), 0);
+    long long tot = 0;
+    for (size_t f = 0; f < fr_.size(); ++f) {
+      lpoff[f] = tot;
+      tot += (uhmk_schur_lp_size(fr_[f].n, fr_[f].fs, chol ? 0 : 1) + 63) / 64 * 64;
+    }
+    lp_elems_ = tot;
+    d_lp_.reserve(std::max(1LL, tot) * sizeof(unsigned short));
+    d_lpoff_.upload(lpoff, stream_);
+    d_lpamax_.reserve(std::max<size_t>(1, 2 * fr_.size()) * sizeof(unsigned));
+  }
   check_hip(hipStreamSynchronize(stream_), "schedule upload");
   if (factor_graph_) (void)hipGraphExecDestroy(factor_graph_);
   factor_graph_ = nullptr;
@@ -978,6 +1022,9 @@ void FrontSolver::enqueue_factor(int method, std::vector<hipEvent_t>* pev) {
   int* info = d_info_.as<int>();
   check_k(uhmk_fill_int(info, (long long)(d_info_.bytes / sizeof(int)), 0, stream_), "clear info");
   check_k(uhmk_fill_int(d_flags_.as<int>(), flag_slots_, 0, stream_), "clear group flags");
+  if (prec_mode_ == 3)
+    check_k(uhmk_fill_int(d_lpamax_.as<int>(), (long long)(d_lpamax_.bytes / sizeof(int)), 0, stream_),
+            "clear operand maxima");
   for (size_t li = 0; li < launches_.size(); ++li) {
     const Launch& l = launches_[li];
     int rc = 0;
@@ -1013,7 +1060,15 @@ void FrontSolver::enqueue_factor(int method, std::vector<hipEvent_t>* pev) {
         rc = uhmk_chol_gemm_diag(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, l.mma, info, st);
         break;
       case Launch::SCHUR:
-        rc = uhmk_gemm_update(F, IT + l.off, l.count, 0, 0, 1, l.extra, l.tile, A, l.prec, l.mma, st);
+        if (l.mma >= 2)
+          rc = uhmk_schur_lp(F, IT + l.off, l.count, l.extra ? 0 : 1, l.tile, l.mma == 3, A, l.prec,
+                             d_lpoff_.as<long long>(), d_lpamax_.as<unsigned>(), d_lp_.as<unsigned short>(), st);
+        else
+          rc = uhmk_gemm_update(F, IT + l.off, l.count, 0, 0, 1, l.extra, l.tile, A, l.prec, l.mma, st);
+        break;
+      case Launch::LP_PACK:
+        rc = uhmk_schur_lp_pack(F, IT + l.off, l.count, prec_mode_ == 3, A, l.prec, d_lpoff_.as<long long>(),
+                                d_lpamax_.as<unsigned>(), d_lp_.as<unsigned short>(), st);
         break;
       case Launch::CHOL_GROUP:
         rc = uhmk_chol_group(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, l.mma, d_flags_.as<int>(), info, st);

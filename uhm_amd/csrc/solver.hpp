@@ -48,7 +48,11 @@ struct Launch {
   // next panel's diagonal block (items with c == 1)
   // CHOL_GROUP: every panel of a Cholesky panel group (diagonal blocks,
   // TRSM, in-group updates) in one persistent launch
-  enum Kind { ASSEMBLE, LU_PANEL, LU_ROWS, CHOL_DIAG, CHOL_TRSM, GEMM, SCHUR, LU_FRONT, GEMM_DIAG, CHOL_GROUP } kind;
+  // LP_PACK: the Schur operands of a level packed into BF16 / FP16 for the
+  // low-precision SCHUR launch that follows (precision modes 2 and 3)
+  enum Kind {
+    ASSEMBLE, LU_PANEL, LU_ROWS, CHOL_DIAG, CHOL_TRSM, GEMM, SCHUR, LU_FRONT, GEMM_DIAG, CHOL_GROUP, LP_PACK
+  } kind;
   int c0 = 0, nb = 0;
   long long off = 0;  // offset into the item (or front-list) array
   int count = 0;
@@ -63,7 +67,7 @@ struct Launch {
   int wait_ev = -1;   // event the launch's stream waits for first (-1: none)
   int record_ev = -1; // event recorded on the launch's stream after it (-1: none)
 };
-constexpr int N_LAUNCH_KINDS = 10;
+constexpr int N_LAUNCH_KINDS = 11;
 
 // One launch of the level solves.
 struct SolveStep {
@@ -142,6 +146,8 @@ class FrontSolver {
   // mode 1 (mixed): fronts of the top `top_levels` tree levels stay FP64
   // (their pivot blocks factorize in FP64), all other fronts are stored in
   // FP32, and every Schur / trailing update runs as FP32 MFMA products.
+  // mode 2 / 3: as mode 1 with the final Schur complement products on BF16 /
+  // FP16 operands (FP32 accumulation), the operands packed once per front.
   // Changing it re-lays out the arenas and needs a new factorization.
   void set_precision(int mode, int top_levels);
   int precision_mode() const { return prec_mode_; }
@@ -286,6 +292,10 @@ class FrontSolver {
   DevBuf d_arena_, d_arena32_, d_piv_, d_x_, d_b_, d_stage_, d_info_;
   DevBuf d_flags_;  // progress flags of the Cholesky group launches, [0]: waits that gave up
   DevBuf d_serr_;   // level solves: waits that gave up
+  // low-precision Schur operands (modes 2, 3): packed buffer, per-front
+  // element offsets, per-front operand maxima (FP16 scales)
+  DevBuf d_lp_, d_lpoff_, d_lpamax_;
+  long long lp_elems_ = 0;
   void check_solve_err();
   DevBuf d_rf_;  // refinement right-hand side (front layout)
   DevBuf d_z_;   // second solve vector (front layout) of the block-step Cholesky solves
