@@ -192,7 +192,7 @@ void uhm_set_refinement_(uhm_fort_p* mesh, uhm_fort_int* kind, uhm_fort_int* res
 /* Applications of M^-1 (level solves) in the last refinement. */
 void uhm_get_refine_solves_(uhm_fort_p* mesh, uhm_fort_int* n);
 /* Schedule knobs: key 1 = Cholesky panels per trailing update (default 0:
- * 8), key 2 = mean tile extent from
+ * 6), key 2 = mean tile extent from
  * which a GEMM launch uses 128 x 128 tiles (default 8192: 64 x 64 tiles on
  * every workload measured, whose 128-tile launches ran slower), key 3 = number
  * of right-hand sides from which the solves run as block solves, FP64 MFMA

@@ -82,12 +82,14 @@ struct Chol16Step<16> {
 
 
 // Cholesky of the kb x kb diagonal panel block and its inverse, one
-// workgroup of four wavefronts per front, by 16 x 16 blocks.  Block step B:
-// lanes 0..15 of wavefront 0 factor the diagonal block (v_readlane
-// broadcast, 1 / sqrt from v_rsq_f64 and two Newton steps) and invert it;
-// the blocks below are multiplied by that inverse and the trailing blocks
-// updated, as 16 x 16 x 16 FP64 MFMA products spread over the wavefronts.
-// The blocks of W = L11^-1 below the diagonal then follow diagonal by
+// workgroup of four wavefronts per front, by 16-column panels.  Panel step
+// B: wavefront 0 factors the panel rows [16 B, 64) x 16 columns, a row per
+// lane (v_readlane broadcast of the column entries of the diagonal block,
+// 1 / sqrt from v_rsq_f64 and two Newton steps), which gives the diagonal
+// block and the blocks below it in one pass; the trailing blocks are then
+// updated as 16 x 16 x 16 FP64 MFMA products spread over the wavefronts.
+// Afterwards the four wavefronts invert the four diagonal blocks at once,
+// and the blocks of W = L11^-1 below the diagonal follow diagonal by
 // diagonal, W(I,J) = -W(I,I) sum_{K=J}^{I-1} L(I,K) W(K,J).  W's strict
 // lower part is stored transposed in the strict upper triangle of the
 // diagonal block, which the Cholesky path never reads as matrix data (only
@@ -127,7 +129,10 @@ __device__ __forceinline__ void chol_diag_lds(DiagLds& S, T* __restrict__ A, lon
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   for (int B = 0; B < 4; ++B) {
     const int b0 = 16 * B;
-    if (w == 0 && lane < 16) {
+    // the 16-column panel [b0, 64) x [b0, b0 + 16): wavefront 0, lane l
+    // holds row b0 + l, so the pivots and the multipliers of the rows below
+    // the diagonal block come out of the same column steps
+    if (w == 0 && lane < 64 - b0) {
       double a[16];
 #pragma unroll
       for (int c = 0; c < 16; ++c) a[c] = (c <= lane) ? sL[(b0 + lane) * LDW + b0 + c] : 0.0;
@@ -136,34 +141,6 @@ __device__ __forceinline__ void chol_diag_lds(DiagLds& S, T* __restrict__ A, lon
       if (lane == 0 && first && b0 + first <= kb && s_first == 0) s_first = b0 + first;
 #pragma unroll
       for (int c = 0; c < 16; ++c) sL[(b0 + lane) * LDW + b0 + c] = a[c];
-    }
-    __builtin_amdgcn_s_barrier();  // wavefront 0's block rows visible to its own column pass below
-    if (w == 0 && lane < 16) {
-      // inverse of the diagonal block, column lane; the 16 reciprocals of
-      // the diagonal come first (one division per lane, in parallel), so the
-      // substitution chain holds multiplications only
-      const double rdl = 1.0 / sL[(b0 + lane) * LDW + b0 + lane];
-      double wv[16];
-#pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        double acc = (k == lane) ? 1.0 : 0.0;
-#pragma unroll
-        for (int m = 0; m < k; ++m) acc = fma(-sL[(b0 + k) * LDW + b0 + m], wv[m], acc);
-        wv[k] = acc * readlane_d16_rt(rdl, k);
-      }
-#pragma unroll
-      for (int k = 0; k < 16; ++k) sW[(b0 + k) * LDW + b0 + lane] = wv[k];
-    }
-    __syncthreads();
-    // L(I,B) = A(I,B) W(B,B)^T for the blocks below, one wavefront each
-    const int I = B + 1 + w;
-    if (I < 4) {
-      d4 c = d4{0.0, 0.0, 0.0, 0.0};
-      c = mma16_op<true>(sL + (16 * I) * LDW + b0, LDW, sW + b0 * LDW + b0, LDW, c, lane);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) sS[w][(4 * r + (lane >> 4)) * 17 + (lane & 15)] = c[r];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) sL[(16 * I + 4 * r + (lane >> 4)) * LDW + b0 + (lane & 15)] = c[r];
     }
     __syncthreads();
     // trailing blocks A(I,J) -= L(I,B) L(J,B)^T, B < J <= I
@@ -189,6 +166,25 @@ __device__ __forceinline__ void chol_diag_lds(DiagLds& S, T* __restrict__ A, lon
     }
     __syncthreads();
   }
+  // inverses of the four diagonal blocks, wavefront w block w, lane c
+  // forming column c; the 16 reciprocals of the diagonal come first (one
+  // division per lane, in parallel), so the substitution chain holds
+  // multiplications only
+  if (lane < 16) {
+    const int b0 = 16 * w;
+    const double rdl = 1.0 / sL[(b0 + lane) * LDW + b0 + lane];
+    double wv[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      double acc = (k == lane) ? 1.0 : 0.0;
+#pragma unroll
+      for (int m = 0; m < k; ++m) acc = fma(-sL[(b0 + k) * LDW + b0 + m], wv[m], acc);
+      wv[k] = acc * readlane_d16_rt(rdl, k);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) sW[(b0 + k) * LDW + b0 + lane] = wv[k];
+  }
+  __syncthreads();
   for (int d = 1; d < 4; ++d) {
     // wavefront w forms block (I, J) = (w + d, w) of W when it exists
     const int I = w + d, J = w;
@@ -971,7 +967,7 @@ __global__ __launch_bounds__(256, 2) void k_chol_group(const uhmk_front* __restr
         } else {
           group_wait(fl + p, p + 1, err);
           const TS* __restrict__ D = M + pc0 + (long long)pc0 * n;
-#pragma unroll 4
+#pragma unroll 8
           for (int u = 0; u < 16; ++u) {
             const int idx = tid + 256 * u, rr = idx & 63, cc = idx >> 6;
             sWt[rr * LDT + cc] = ld_coh<double>(D, rr + (long long)cc * n, rr < kb && cc < kb && rr <= cc);
@@ -1054,6 +1050,20 @@ __global__ __launch_bounds__(256, 2) void k_chol_group(const uhmk_front* __restr
     // reload would, identity beyond kb) through chol_diag_lds
     {
       const int pc0 = g0 + T * r, kb = min(T, ge - pc0);
+      // the block's entries as the earlier launches left them, all loads
+      // in flight at once
+      TS mv[NB][NB][4];
+#pragma unroll
+      for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const int ii = iw + a * 16 + cl;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int jj = jw + b * 16 + Mfma<TM>::col(q, lane);
+            mv[a][b][q] = M[ii < kb && jj <= ii ? (pc0 + ii) + (long long)(pc0 + jj) * n : pc0 + (long long)pc0 * n];
+          }
+        }
       __syncthreads();  // sX read by every wavefront
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
@@ -1071,10 +1081,7 @@ __global__ __launch_bounds__(256, 2) void k_chol_group(const uhmk_front* __restr
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int jj = jw + b * 16 + Mfma<TM>::col(q, lane);
-            if (ii < kb && jj <= ii) {
-              const double c = (double)M[(pc0 + ii) + (long long)(pc0 + jj) * n];
-              S.sL[ii * LDW + jj] = (double)(TS)(c - (double)accd[a][b][q]);
-            }
+            if (ii < kb && jj <= ii) S.sL[ii * LDW + jj] = (double)(TS)((double)mv[a][b][q] - (double)accd[a][b][q]);
           }
         }
       __syncthreads();

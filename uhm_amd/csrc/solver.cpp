@@ -1186,7 +1186,7 @@ double FrontSolver::autotune(int method) {
   double best = time_it();
   if (method == UHM_CHOL) {
     int bg = chol_group();
-    for (int g : {1, 2, 4, 8}) {
+    for (int g : {1, 2, 4, 6, 8, 12}) {
       if (g == bg) continue;
       set_chol_group(g);
       const double t = time_it();

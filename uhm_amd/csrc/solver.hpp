@@ -191,7 +191,7 @@ class FrontSolver {
     if (g != chol_group_) schedule_method_ = 0;
     chol_group_ = g;
   }
-  int chol_group() const { return chol_group_ ? chol_group_ : 8; }
+  int chol_group() const { return chol_group_ ? chol_group_ : 6; }
   void set_profile(bool on) { profile_ = on; }
   // Graph mode: the factorization's launch sequence (and each solve's) is
   // captured once per pattern / method / precision and replayed, for
@@ -233,7 +233,7 @@ class FrontSolver {
   Mesh_* mesh_;
   int nrhs_;
   int nb_ = 64;
-  int chol_group_ = 0;  // 0: 8 panels (measured best or tied on both paths with fused diagonal blocks)
+  int chol_group_ = 0;  // 0: 6 panels (measured best on the FP64, FP32 and low-precision paths)
   int tile_threshold_ = 8192;  // mean tile extent from which GEMM launches use 128 x 128 tiles
   int block_rhs_ = 8;
   int front_small_ = 64;  // LU fronts with n <= this (at most 64) factor in one whole-front launch
