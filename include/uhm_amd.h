@@ -150,9 +150,10 @@ void uhm_get_timing_(uhm_fort_p* mesh, uhm_fort_double* factor_ms, uhm_fort_doub
  * (final Schur complement), lu_front (whole-front LU of fronts with
  * n <= 64), gemm_diag (Cholesky panel update that also factors the next
  * panel's diagonal blocks), chol_group (a Cholesky panel group in one
- * persistent launch), lp_pack (Schur operands packed to BF16 / FP16):
- * milliseconds, algorithmic flops, algorithmic bytes and launch counts
- * (arrays of 11). */
+ * persistent launch), lp_pack (Schur operands packed to BF16 / FP16),
+ * chol_small (assembly and whole Cholesky factorization of the small
+ * fronts of a level in one launch): milliseconds, algorithmic flops,
+ * algorithmic bytes and launch counts (arrays of 12). */
 void uhm_set_profile_(uhm_fort_p* mesh, uhm_fort_int* on);
 void uhm_get_kernel_stats_(uhm_fort_p* mesh, uhm_fort_double* ms, uhm_fort_double* flops, uhm_fort_double* bytes,
                            uhm_fort_int* launches);
@@ -210,7 +211,19 @@ void uhm_get_refine_solves_(uhm_fort_p* mesh, uhm_fort_int* n);
  * launch whose workgroups resolve the panel-to-panel dependencies among
  * themselves, left-looking inside the group: 0 never (the per-panel
  * launches that keys 5 and 6 shape), 1 on every tree level, 2 (default) on
- * the levels with at most 16 fronts, where they measured faster. */
+ * the levels with at most 16 fronts, where they measured faster; key 8 =
+ * Cholesky level solves (1, default: one launch per tree level and
+ * direction for up to 8 right-hand sides; 0: per-block launches); key 9 =
+ * Cholesky tree levels of at least this many fronts, every one with
+ * fs <= 64, assembled and factored in one launch, a workgroup per front,
+ * with the same factor bits as the four level launches (default 0: never;
+ * on the 16^3 Q4 mesh the fused launch ran its levels 1.2-1.4x slower than
+ * the four launches, whose separate kernels keep 4-8 workgroups per CU
+ * where the fused one holds 2); key 10 = precision modes 2 and 3: the
+ * smallest mean Schur block (rows) of a tree level whose Schur products
+ * take packed BF16 / FP16 operands, the levels below it FP32 products
+ * (default 768: on small blocks the packing saves next to nothing and the
+ * rounding costs refinement steps; 0: every level). */
 void uhm_set_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value);
 void uhm_get_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value);
 /* Autotuner: times factorizations of the current values over the knobs

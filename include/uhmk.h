@@ -148,6 +148,21 @@ int uhmk_chol_group(const uhmk_front* fronts, const uhmk_item* items, int n_item
 int uhmk_gemm_update(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, int schur,
                      int transb, int tile, void* arena, int prec, int mma, hipStream_t stream);
 
+/* Whole Cholesky factorization of small fronts, one workgroup per front of
+ * flist (every front needs fs <= 64; one with fs > 64 is left untouched and
+ * reports info[f] = -1): the front is assembled as uhmk_assemble does, its
+ * diagonal block factored and inverted as uhmk_chol_diag, the rows below
+ * solved as uhmk_chol_trsm and ABR updated as uhmk_gemm_update(schur = 1,
+ * transb = 1, tile = 64, mma) -- the same operations in the same order, so
+ * the same bits as those four launches, in one launch per tree level.
+ * Replaces op_merge + chol_hier per element, leaves first
+ * (UHM/src/operation/element.cxx:200-241,
+ * UHM/src/matrix/uhm/fla/chol/decompose.cxx:85-98).  prec: storage of the
+ * fronts; mma: 0 FP64 products (FP64 fronts), 1 FP32 products. */
+int uhmk_chol_front_small(const uhmk_front* fronts, const int* flist, int n_fronts, const int* src_beg,
+                          const uhmk_source* srcs, const int* inv, double* arena64, float* arena32, int prec, int mma,
+                          const double* stage, int* info, hipStream_t stream);
+
 /* Low-precision Schur complement products (the final ABR -= ABL ATR /
  * ABL ABL^T of lu_piv_hier / chol_hier,
  * UHM/src/matrix/uhm/fla/lu_piv/decompose.cxx:148-149,

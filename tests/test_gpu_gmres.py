@@ -166,6 +166,9 @@ def test_bf16_schur_refined_to_fp64_accuracy(lib, case, top):
     x64 = m.fetch_x().copy()
     kappa = kappa_1(m)
     eta0 = {}
+    # low-precision products on every level (the default keeps them to
+    # levels with large Schur blocks, which meshes this small do not have)
+    m.set_tuning(10, 0)
     for mode in (U.PREC_MIXED, U.PREC_MIXED_BF16, U.PREC_MIXED_FP16):
         m.set_precision(mode, top)
         assert _factor(m, method) == 0
@@ -185,3 +188,25 @@ def test_bf16_schur_refined_to_fp64_accuracy(lib, case, top):
     assert eta0[U.PREC_MIXED_BF16] > 10.0 * eta0[U.PREC_MIXED]
     # FP16 keeps three more bits than BF16: its unrefined solution sits between
     assert eta0[U.PREC_MIXED_FP16] > eta0[U.PREC_MIXED] and eta0[U.PREC_MIXED_FP16] < eta0[U.PREC_MIXED_BF16]
+
+
+def test_low_precision_level_threshold(lib):
+    """Tuning key 10: packed low-precision Schur products only on the levels
+    whose Schur blocks average at least that many rows (default 768); the
+    other levels take FP32 products, so with a threshold above every level
+    mode 2 gives mode 1's factors bit for bit, and with 0 every level packs."""
+    m = _chol3d(1)
+    assert m.get_tuning(10) == 768
+    m.set_profile(True)
+    m.set_precision(U.PREC_MIXED, 0)
+    assert m.chol_with_free() == 0
+    ref = [np.tril(m.fetch_front(f)) for f in range(m.n_fronts())]
+    m.set_precision(U.PREC_MIXED_BF16, 0)
+    m.set_tuning(10, 1 << 30)
+    assert m.chol_with_free() == 0
+    assert m.kernel_stats()["lp_pack"][3] == 0
+    assert all(np.array_equal(np.tril(m.fetch_front(f)), ref[f]) for f in range(m.n_fronts()))
+    m.set_tuning(10, 0)
+    assert m.chol_with_free() == 0
+    assert m.kernel_stats()["lp_pack"][3] > 0
+    assert any(not np.array_equal(np.tril(m.fetch_front(f)), ref[f]) for f in range(m.n_fronts()))

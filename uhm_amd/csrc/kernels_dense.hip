@@ -11,6 +11,7 @@
 
 #include <type_traits>
 
+#include "assemble_device.hpp"
 #include "dense_device.hpp"
 #include "uhmk.h"
 
@@ -127,12 +128,15 @@ __device__ __forceinline__ void chol_diag_lds(DiagLds& S, T* __restrict__ A, lon
   auto& sS = S.sS;
   int& s_first = S.s_first;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  for (int B = 0; B < 4; ++B) {
+  // 16-column blocks that hold the kb columns (the rest is identity
+  // padding, which no real entry depends on)
+  const int nblk = (kb + 15) >> 4, kbp = 16 * nblk;
+  for (int B = 0; B < nblk; ++B) {
     const int b0 = 16 * B;
     // the 16-column panel [b0, 64) x [b0, b0 + 16): wavefront 0, lane l
     // holds row b0 + l, so the pivots and the multipliers of the rows below
     // the diagonal block come out of the same column steps
-    if (w == 0 && lane < 64 - b0) {
+    if (w == 0 && lane < kbp - b0) {
       double a[16];
 #pragma unroll
       for (int c = 0; c < 16; ++c) a[c] = (c <= lane) ? sL[(b0 + lane) * LDW + b0 + c] : 0.0;
@@ -147,8 +151,8 @@ __device__ __forceinline__ void chol_diag_lds(DiagLds& S, T* __restrict__ A, lon
     for (int t = w; t < 6; t += 4) {
       // t -> (I, J) over the lower blocks right of column block B
       int Ib = -1, Jb = -1, cnt = 0;
-      for (int J = B + 1; J < 4; ++J)
-        for (int I2 = J; I2 < 4; ++I2) {
+      for (int J = B + 1; J < nblk; ++J)
+        for (int I2 = J; I2 < nblk; ++I2) {
           if (cnt == t) {
             Ib = I2;
             Jb = J;
@@ -170,7 +174,7 @@ __device__ __forceinline__ void chol_diag_lds(DiagLds& S, T* __restrict__ A, lon
   // forming column c; the 16 reciprocals of the diagonal come first (one
   // division per lane, in parallel), so the substitution chain holds
   // multiplications only
-  if (lane < 16) {
+  if (lane < 16 && w < nblk) {
     const int b0 = 16 * w;
     const double rdl = 1.0 / sL[(b0 + lane) * LDW + b0 + lane];
     double wv[16];
@@ -185,17 +189,17 @@ __device__ __forceinline__ void chol_diag_lds(DiagLds& S, T* __restrict__ A, lon
     for (int k = 0; k < 16; ++k) sW[(b0 + k) * LDW + b0 + lane] = wv[k];
   }
   __syncthreads();
-  for (int d = 1; d < 4; ++d) {
+  for (int d = 1; d < nblk; ++d) {
     // wavefront w forms block (I, J) = (w + d, w) of W when it exists
     const int I = w + d, J = w;
     d4 S = d4{0.0, 0.0, 0.0, 0.0};
-    if (I < 4) {
+    if (I < nblk) {
       for (int K = J; K < I; ++K) S = mma16(sL + (16 * I) * LDW + 16 * K, sW + (16 * K) * LDW + 16 * J, LDW, S, lane);
 #pragma unroll
       for (int r = 0; r < 4; ++r) sS[w][(4 * r + (lane >> 4)) * 17 + (lane & 15)] = S[r];
     }
     __syncthreads();
-    if (I < 4) {
+    if (I < nblk) {
       d4 Wb = d4{0.0, 0.0, 0.0, 0.0};
       Wb = mma16_ld(sW + (16 * I) * LDW + 16 * I, LDW, sS[w], 17, Wb, lane);
 #pragma unroll
@@ -212,13 +216,12 @@ __device__ __forceinline__ void chol_diag_lds(DiagLds& S, T* __restrict__ A, lon
   if (tid == 0 && s_first && info[f] == 0) info[f] = c0 + s_first;
 }
 
+// The kb x kb diagonal block at column c0 of front fr (kb = min(nb, fs -
+// c0)) from the front into S, factored there (chol_diag_lds) and written
+// back with its inverse.
 template <typename T>
-__global__ __launch_bounds__(256) void k_chol_diag(const uhmk_front* __restrict__ F,
-                                                   const int* __restrict__ flist, int c0, int nb,
-                                                   T* __restrict__ arena, int* __restrict__ info) {
-  __shared__ DiagLds S;
-  const int f = flist[blockIdx.x];
-  const uhmk_front fr = F[f];
+__device__ __forceinline__ void chol_diag_front(DiagLds& S, const uhmk_front& fr, int f, int c0, int nb,
+                                                T* __restrict__ arena, int* __restrict__ info) {
   const int n = fr.n, fs = fr.fs;
   const int kb = min(nb, fs - c0);
   T* __restrict__ A = arena + fr.off + c0 + (long long)c0 * n;
@@ -242,22 +245,29 @@ __global__ __launch_bounds__(256) void k_chol_diag(const uhmk_front* __restrict_
   chol_diag_lds<T>(S, A, n, kb, c0, info, f);
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_chol_diag(const uhmk_front* __restrict__ F,
+                                                   const int* __restrict__ flist, int c0, int nb,
+                                                   T* __restrict__ arena, int* __restrict__ info) {
+  __shared__ DiagLds S;
+  const int f = flist[blockIdx.x];
+  chol_diag_front<T>(S, F[f], f, c0, nb, arena, info);
+}
+
 // Cholesky panel rows [b, b+64) below the diagonal block: X <- X L11^-T =
 // X W^T with W = L11^-1 from the diagonal block (strict upper triangle:
 // W transposed; diagonal: 1 / L(k,k)).  One 64 x 64 x kb FP64 MFMA product
 // per workgroup; each wavefront owns a 32 x 32 quarter of the result.
 constexpr int LDT = TILE + 4;
 
+// Rows [rb, rb + 64) of front fr; sWt: LDS of TILE x LDT doubles.
 template <typename T>
-__global__ __launch_bounds__(256) void k_chol_trsm(const uhmk_front* __restrict__ F,
-                                                   const uhmk_item* __restrict__ items, int c0, int nb,
-                                                   T* __restrict__ arena) {
-  const uhmk_item it = items[blockIdx.x];
-  const uhmk_front fr = F[it.f];
+__device__ __forceinline__ void chol_trsm_rows(double* __restrict__ sWt, const uhmk_front& fr, int rb, int c0,
+                                               int nb, T* __restrict__ arena) {
   const int n = fr.n, fs = fr.fs;
   const int kb = min(nb, fs - c0);
   T* __restrict__ A = arena + fr.off;
-  __shared__ double sWt[TILE * LDT];  // sWt[k][j] = W(j, k)
+  // sWt[k][j] = W(j, k)
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int iw = (w & 1) * 32, jw = (w >> 1) * 32, cl = lane & 15, kq = lane >> 4;
   const T* __restrict__ D = A + c0 + (long long)c0 * n;
@@ -273,7 +283,7 @@ __global__ __launch_bounds__(256) void k_chol_trsm(const uhmk_front* __restrict_
   double xo[2][TILE / 4];
 #pragma unroll
   for (int a = 0; a < 2; ++a) {
-    const int row = it.b + iw + 16 * a + cl;
+    const int row = rb + iw + 16 * a + cl;
 #pragma unroll
     for (int ks = 0; ks < TILE / 4; ++ks) {
       const int k = 4 * ks + kq;
@@ -311,13 +321,22 @@ __global__ __launch_bounds__(256) void k_chol_trsm(const uhmk_front* __restrict_
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-      const int row = it.b + iw + 16 * a + cl;
+      const int row = rb + iw + 16 * a + cl;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int j = jw + 16 * b + 4 * r + (lane >> 4);
         if (row < n && j < kb) A[row + (long long)(c0 + j) * n] = (T)acc[a][b][r];
       }
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_chol_trsm(const uhmk_front* __restrict__ F,
+                                                   const uhmk_item* __restrict__ items, int c0, int nb,
+                                                   T* __restrict__ arena) {
+  __shared__ double sWt[TILE * LDT];
+  const uhmk_item it = items[blockIdx.x];
+  chol_trsm_rows<T>(sWt, F[it.f], it.b, c0, nb, arena);
 }
 
 // ------------------------------------------------------------------------
@@ -390,10 +409,21 @@ struct Mfma<float> {
 // LDS and factors it there (chol_diag_lds, the k_chol_diag computation), so
 // the next panel needs no diagonal launch.  Its LDS is shared with the
 // product's panels, which are dead by then.
-template <bool TRANSB, int T, typename TS, typename TM, bool DIAG = false>
-__global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ F,
-                                              const uhmk_item* __restrict__ items, int c0, int nb, int schur,
-                                              TS* __restrict__ arena, int* __restrict__ info = nullptr) {
+// LDS bytes of one gemm_tile: the two K-chunk panels, or the diagonal
+// factorization's when that is larger (DIAG).
+template <int T, typename TM, bool DIAG>
+constexpr size_t gemm_lds_bytes() {
+  constexpr int LD = T + 16, KC = sizeof(TM) == 4 ? 32 : 16;
+  constexpr size_t panel = 2 * KC * LD * sizeof(TM);
+  return (DIAG && sizeof(DiagLds) > panel) ? sizeof(DiagLds) : panel;
+}
+
+// One tile item of the update (the k_gemm computation), smem: LDS of
+// gemm_lds_bytes<T, TM, DIAG>() bytes, 16-byte aligned.
+template <bool TRANSB, int T, typename TS, typename TM, bool DIAG>
+__device__ __forceinline__ void gemm_tile(const uhmk_front& fr, const uhmk_item it, int c0, int nb, int schur,
+                                          TS* __restrict__ arena, int* __restrict__ info,
+                                          unsigned char* __restrict__ smem) {
   typedef typename Mfma<TM>::acc_t acc_t;
   constexpr int LD = T + 16;          // row stride of the LDS panels (conflict-free half-wave reads)
   constexpr int NB = T / 32;          // 16x16 blocks per wave and dimension
@@ -403,8 +433,6 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ 
   constexpr int KC = sizeof(TM) == 4 ? 32 : 16;
   constexpr int NL = KC / KR;         // loads per thread per operand
   constexpr int BJ = 256 / KC;        // NN B: columns per load pass
-  const uhmk_item it = items[uhmd::xcd_item(blockIdx.x, gridDim.x)];
-  const uhmk_front fr = F[it.f];
   const int n = fr.n, fs = fr.fs;
   const int kbeg = schur ? 0 : c0;
   const int kend = schur ? fs : min(c0 + nb, fs);
@@ -413,9 +441,6 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ 
   const int j1 = min(j0 + T, j0 < fs ? fs : n);
   TS* __restrict__ M = arena + fr.off;
   static_assert(!DIAG || (TRANSB && T == TILE), "fused diagonal blocks: Cholesky updates on 64-wide tiles");
-  constexpr size_t PANEL_BYTES = 2 * KC * LD * sizeof(TM);
-  constexpr size_t LDS_BYTES = (DIAG && sizeof(DiagLds) > PANEL_BYTES) ? sizeof(DiagLds) : PANEL_BYTES;
-  __shared__ __align__(16) unsigned char smem[LDS_BYTES];
   TM* __restrict__ sA = reinterpret_cast<TM*>(smem);
   TM* __restrict__ sB = sA + KC * LD;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -591,6 +616,15 @@ __global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ 
         }
       }
     }
+}
+
+template <bool TRANSB, int T, typename TS, typename TM, bool DIAG = false>
+__global__ __launch_bounds__(256, 2) void k_gemm(const uhmk_front* __restrict__ F,
+                                              const uhmk_item* __restrict__ items, int c0, int nb, int schur,
+                                              TS* __restrict__ arena, int* __restrict__ info = nullptr) {
+  __shared__ __align__(16) unsigned char smem[gemm_lds_bytes<T, TM, DIAG>()];
+  const uhmk_item it = items[uhmd::xcd_item(blockIdx.x, gridDim.x)];
+  gemm_tile<TRANSB, T, TS, TM, DIAG>(F[it.f], it, c0, nb, schur, arena, info, smem);
 }
 
 template <typename TS, typename TM>
@@ -1094,6 +1128,64 @@ __global__ __launch_bounds__(256, 2) void k_chol_group(const uhmk_front* __restr
   else run(std::false_type{});
 }
 
+// ------------------------------------------------------------------------
+// Whole small fronts in one launch (Cholesky, fs <= 64): one workgroup per
+// front runs, in order, the computations of the assembly, diagonal, panel
+// TRSM and Schur complement launches of a tree level on its own front --
+// extend-add of the lower 64 x 64 tiles (uhmd::asm_tile), the fs x fs
+// diagonal block and its inverse (chol_diag_front), the rows below it
+// (chol_trsm_rows) and ABR -= ABL ABL^T on 64-wide tiles (gemm_tile,
+// K = [0, fs), TM products) -- so a level of many small fronts takes one
+// launch instead of four, the front stays in the L2 between the steps, and
+// the factors are the same bits as the four launches give (the same
+// operations in the same order).  The reference steps: op_merge then
+// chol_hier on each element, leaves first (UHM/src/operation/element.cxx:
+// 200-241, UHM/src/matrix/uhm/fla/chol/decompose.cxx:85-98).  A front with
+// fs > 64 is not touched and reports info[f] = -1.
+// ------------------------------------------------------------------------
+template <typename TS, typename TM>
+__global__ __launch_bounds__(256, 2) void k_chol_front_small(const uhmk_front* __restrict__ F,
+                                                          const int* __restrict__ flist,
+                                                          const int* __restrict__ src_beg,
+                                                          const uhmk_source* __restrict__ srcs,
+                                                          const int* __restrict__ inv, double* __restrict__ arena64,
+                                                          float* __restrict__ arena32,
+                                                          const double* __restrict__ stage,
+                                                          int* __restrict__ info) {
+  __shared__ __align__(16) unsigned char smem[sizeof(DiagLds)];
+  static_assert(sizeof(DiagLds) >= TILE * LDT * sizeof(double), "TRSM staging fits");
+  static_assert(sizeof(DiagLds) >= gemm_lds_bytes<64, TM, false>(), "product panels fit");
+  static_assert(sizeof(DiagLds) >= sizeof(uhmd::AsmSources), "source table fits");
+  const int f = flist[blockIdx.x];
+  const uhmk_front fr = F[f];
+  const int n = fr.n, fs = fr.fs;
+  if (fs > TILE) {
+    if (threadIdx.x == 0) info[f] = -1;
+    return;
+  }
+  TS* __restrict__ arena;
+  if constexpr (sizeof(TS) == 8) arena = reinterpret_cast<TS*>(arena64);
+  else arena = reinterpret_cast<TS*>(arena32);
+  {
+    uhmd::AsmSources& S = *reinterpret_cast<uhmd::AsmSources*>(smem);
+    uhmd::asm_sources(S, F, f, src_beg, srcs);
+    __syncthreads();
+    for (int j0 = 0; j0 < n; j0 += TILE)
+      for (int i0 = j0; i0 < n; i0 += TILE) uhmd::asm_tile<TS>(S, n, fr.off, i0, j0, inv, arena64, arena32, stage);
+    __syncthreads();
+  }
+  if (fs == 0) return;
+  chol_diag_front<TS>(*reinterpret_cast<DiagLds*>(smem), fr, f, 0, TILE, arena, info);
+  __syncthreads();
+  for (int rb = fs; rb < n; rb += TILE) {
+    chol_trsm_rows<TS>(reinterpret_cast<double*>(smem), fr, rb, 0, TILE, arena);
+    __syncthreads();
+  }
+  for (int j0 = fs; j0 < n; j0 += TILE)
+    for (int i0 = j0; i0 < n; i0 += TILE)
+      gemm_tile<true, TILE, TS, TM, false>(fr, uhmk_item{f, i0, j0, 0}, 0, TILE, 1, arena, nullptr, smem);
+}
+
 template <typename TS, typename TM>
 int launch_chol_group(const uhmk_front* fronts, const uhmk_item* items, int n_items, int g0, int gw, TS* arena,
                       int* flags, int* info, hipStream_t stream) {
@@ -1125,6 +1217,25 @@ int uhmk_chol_group(const uhmk_front* fronts, const uhmk_item* items, int n_item
   else
     return -1006;
   if (rc) return rc;
+  return ret(hipGetLastError());
+}
+
+int uhmk_chol_front_small(const uhmk_front* fronts, const int* flist, int n_fronts, const int* src_beg,
+                          const uhmk_source* srcs, const int* inv, double* arena64, float* arena32, int prec, int mma,
+                          const double* stage, int* info, hipStream_t stream) {
+  if (n_fronts <= 0) return 0;
+  const dim3 g(n_fronts), b(256);
+  if (prec == 0 && mma == 0)
+    hipLaunchKernelGGL((k_chol_front_small<double, double>), g, b, 0, stream, fronts, flist, src_beg, srcs, inv,
+                       arena64, arena32, stage, info);
+  else if (prec == 0 && mma == 1)
+    hipLaunchKernelGGL((k_chol_front_small<double, float>), g, b, 0, stream, fronts, flist, src_beg, srcs, inv,
+                       arena64, arena32, stage, info);
+  else if (prec == 1 && mma == 1)
+    hipLaunchKernelGGL((k_chol_front_small<float, float>), g, b, 0, stream, fronts, flist, src_beg, srcs, inv,
+                       arena64, arena32, stage, info);
+  else
+    return -1006;
   return ret(hipGetLastError());
 }
 

@@ -16,6 +16,7 @@
 //   trsm / gemm / syrk  LINAL/src/dense/{trsm,gemm,syrk}.cxx
 #include <hip/hip_runtime.h>
 
+#include "assemble_device.hpp"
 #include "dense_device.hpp"
 #include "uhmk.h"
 
@@ -31,10 +32,8 @@ constexpr int TILE = 64;
 // operation sequence of op_merge on a freshly zeroed front
 // (UHM/src/operation/element.cxx:135-185).  Item: f, a = row tile start,
 // b = column tile start.  The sum is formed in FP64 whatever the storage
-// types: T of the assembled front, FP64 or FP32 per child.
+// types: T of the assembled front, FP64 or FP32 per child (uhmd::asm_tile).
 // ------------------------------------------------------------------------
-constexpr int MAX_SRC = 16;
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_assemble(const uhmk_front* __restrict__ F,
                                                   const uhmk_item* __restrict__ items,
@@ -45,67 +44,10 @@ __global__ __launch_bounds__(256) void k_assemble(const uhmk_front* __restrict__
                                                   float* __restrict__ arena32,
                                                   const double* __restrict__ stage) {
   const uhmk_item it = items[blockIdx.x];
-  const int f = it.f;
-  __shared__ long long s_off[MAX_SRC];
-  __shared__ int s_ld[MAX_SRC], s_inv[MAX_SRC], s_kind[MAX_SRC];
-  const int s0 = src_beg[f], ns = min(src_beg[f + 1] - s0, MAX_SRC);
-  const long long off = F[f].off;
-  const int n = F[f].n;
-  if (threadIdx.x < ns) {
-    const uhmk_source sc = srcs[s0 + threadIdx.x];
-    if (sc.src < 0) {
-      s_kind[threadIdx.x] = 0;
-      s_off[threadIdx.x] = F[f].soff;
-      s_ld[threadIdx.x] = n;
-      s_inv[threadIdx.x] = -1;
-    } else {
-      const uhmk_front ch = F[sc.src];
-      s_kind[threadIdx.x] = 1 + ch.prec;
-      s_off[threadIdx.x] = ch.off + ch.fs + (long long)ch.fs * ch.n;
-      s_ld[threadIdx.x] = ch.n;
-      s_inv[threadIdx.x] = sc.inv;
-    }
-  }
+  __shared__ uhmd::AsmSources S;
+  uhmd::asm_sources(S, F, it.f, src_beg, srcs);
   __syncthreads();
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int row = it.a + tx;
-  if (row >= n) return;
-  // thread (tx, ty): row it.a + tx, columns it.b + ty + 4 u (u < 16); the
-  // sources are summed in order into per-column registers, each source's
-  // 16 index and 16 value loads issued together
-  double acc[TILE / 4];
-#pragma unroll
-  for (int u = 0; u < TILE / 4; ++u) acc[u] = 0.0;
-  for (int s = 0; s < ns; ++s) {
-    const int k = s_kind[s];
-    const int ri = k ? inv[s_inv[s] + row] : row;
-    if (ri < 0) continue;
-    const long long base = s_off[s] + ri;
-    const int ld = s_ld[s];
-    int cj[TILE / 4];
-#pragma unroll
-    for (int u = 0; u < TILE / 4; ++u) {
-      const int col = it.b + ty + 4 * u;
-      const bool ok = col < n;
-      cj[u] = k ? (ok ? inv[s_inv[s] + col] : -1) : (ok ? col : -1);
-    }
-    if (k == 0) {
-#pragma unroll
-      for (int u = 0; u < TILE / 4; ++u) acc[u] += uhmd::ldz(stage, base + (long long)cj[u] * ld, cj[u] >= 0);
-    } else if (k == 1) {
-#pragma unroll
-      for (int u = 0; u < TILE / 4; ++u) acc[u] += uhmd::ldz(arena64, base + (long long)cj[u] * ld, cj[u] >= 0);
-    } else {
-#pragma unroll
-      for (int u = 0; u < TILE / 4; ++u) acc[u] += uhmd::ldz(arena32, base + (long long)cj[u] * ld, cj[u] >= 0);
-    }
-  }
-  T* A = (sizeof(T) == 8 ? (T*)arena64 : (T*)arena32) + off;
-#pragma unroll
-  for (int u = 0; u < TILE / 4; ++u) {
-    const int col = it.b + ty + 4 * u;
-    if (col < n) A[row + (long long)col * n] = (T)acc[u];
-  }
+  uhmd::asm_tile<T>(S, F[it.f].n, F[it.f].off, it.a, it.b, inv, arena64, arena32, stage);
 }
 
 // ------------------------------------------------------------------------

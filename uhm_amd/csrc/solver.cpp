@@ -431,8 +431,25 @@ void FrontSolver::build_schedule(int method) {
         items_.push_back(uhmk_item{f, i0, j0, 0});
       }
   };
+  std::vector<char> lp_level(levels_.size(), 0);  // levels with low-precision Schur products
   for (int L = 0; L < (int)levels_.size(); ++L) {
     const auto& lv = levels_[L];
+    if (chol && nb == 64 && small_fused_ > 0 && (int)lv.size() >= small_fused_) {
+      // a level of many small fronts (every fs <= 64): one launch runs the
+      // assembly, diagonal block, TRSM and Schur complement of each front
+      bool small = true;
+      for (int f : lv) small = small && fr_[f].fs <= 64;
+      if (small) {
+        Launch lf;
+        lf.kind = Launch::CHOL_SMALL;
+        lf.level = L;
+        lf.off = (long long)flists_.size();
+        lf.count = (int)lv.size();
+        flists_.insert(flists_.end(), lv.begin(), lv.end());
+        push_main(lf);
+        continue;
+      }
+    }
     Launch la;
     la.kind = Launch::ASSEMBLE;
     la.level = L;
@@ -818,7 +835,7 @@ void FrontSolver::build_schedule(int method) {
     ls.kind = Launch::SCHUR;
     ls.level = L;
     ls.extra = chol ? 1 : 0;
-    const bool lowp = prec_mode_ >= 2;
+    bool lowp = prec_mode_ >= 2;
     {
       double sss = 0;
       int cnt = 0;
@@ -827,6 +844,12 @@ void FrontSolver::build_schedule(int method) {
           sss += fr_[f].n - fr_[f].fs;
           ++cnt;
         }
+      // low-precision operands only where the Schur blocks are large: on
+      // small ones the packing saves next to nothing and the rounding costs
+      // refinement steps
+      if (lowp && (!cnt || sss / cnt < lp_min_ss_)) lowp = false;
+      ls.mma = lowp ? prec_mode_ : -1;
+      lp_level[L] = lowp;
       // the packed low-precision products run 64-wide tiles at every size
       // (eight workgroups per CU hide their load latency)
       ls.tile = cnt && !lowp ? gemm_tile_for(sss / cnt, sss / cnt, tile_threshold_) : 64;
@@ -874,10 +897,13 @@ void FrontSolver::build_schedule(int method) {
     // mode 2: the Schur complement products take BF16 operands; the panel
     // (trailing) updates, which form the factor columns themselves, stay FP32
     // mma 2 / 3: packed BF16 / FP16 Schur products
-    l.mma = prec_mode_ == 0 ? 0 : (prec_mode_ >= 2 && l.kind == Launch::SCHUR ? prec_mode_ : 1);
+    l.mma = prec_mode_ == 0 ? 0 : (prec_mode_ >= 2 && l.kind == Launch::SCHUR && l.mma >= 2 ? prec_mode_ : 1);
     // the group launches' small in-group products in the storage type (FP64
     // products on FP64 fronts measured faster there than FP32 ones)
     if (l.kind == Launch::CHOL_GROUP) l.mma = l.prec == 0 ? 0 : 1;
+    // whole small fronts: the Schur products of the mode (FP32 in the
+    // low-precision modes, whose packing does not pay on fronts this small)
+    if (l.kind == Launch::CHOL_SMALL) l.mma = prec_mode_ == 0 ? 0 : 1;
   }
   // Algorithmic work of every launch: the flops / bytes the operation needs,
   // not what a kernel happens to execute (diagonal SYRK tiles count their
@@ -958,6 +984,16 @@ void FrontSolver::build_schedule(int method) {
           }
         }
         break;
+      case Launch::CHOL_SMALL:
+        for (int q = 0; q < l.count; ++q) {
+          const int f = flists_[l.off + q];
+          const double n = fr_[f].n, fs = fr_[f].fs, ss = n - fs;
+          double rd = fr_[f].soff >= 0 ? tri(n) : 0.0;
+          for (Element_* c : order_[f]->children) rd += tri(c->ss);
+          fl += flop_chol(fs) + fs * fs * ss + fs * ss * (ss + 1.0);
+          by += 8.0 * (tri(n) + rd);
+        }
+        break;
       case Launch::LU_FRONT:
         for (int q = 0; q < l.count; ++q) {
           const int f = flists_[l.off + q];
@@ -992,13 +1028,14 @@ void FrontSolver::build_schedule(int method) {
   d_flists_.upload(flists_, stream_);
   d_flags_.reserve((size_t)flag_slots_ * sizeof(int));
   if (prec_mode_ >= 2) {
-    // packed operands of every front with a Schur complement
+    // packed operands of the fronts of the levels with low-precision products
     std::vector<long long> lpoff(fr_.size(), 0);
     long long tot = 0;
-    for (size_t f = 0; f < fr_.size(); ++f) {
-      lpoff[f] = tot;
-      tot += (uhmk_schur_lp_size(fr_[f].n, fr_[f].fs, chol ? 0 : 1) + 63) / 64 * 64;
-    }
+    for (size_t L = 0; L < levels_.size(); ++L)
+      for (int f : levels_[L]) {
+        lpoff[f] = tot;
+        if (lp_level[L]) tot += (uhmk_schur_lp_size(fr_[f].n, fr_[f].fs, chol ? 0 : 1) + 63) / 64 * 64;
+      }
     lp_elems_ = tot;
     d_lp_.reserve(std::max(1LL, tot) * sizeof(unsigned short));
     d_lpoff_.upload(lpoff, stream_);
@@ -1033,6 +1070,11 @@ void FrontSolver::enqueue_factor(int method, std::vector<hipEvent_t>* pev) {
     if (pev) check_hip(hipEventRecord((*pev)[2 * li], st), "event");
     void* A = arena_ptr(l.prec);
     switch (l.kind) {
+      case Launch::CHOL_SMALL:
+        rc = uhmk_chol_front_small(F, FL + l.off, l.count, d_srcbeg_.as<int>(), d_srcs_.as<uhmk_source>(),
+                                   d_inv_.as<int>(), d_arena_.as<double>(), d_arena32_.as<float>(), l.prec, l.mma,
+                                   d_stage_.as<double>(), info, st);
+        break;
       case Launch::ASSEMBLE:
         rc = uhmk_assemble(F, IT + l.off, l.count, d_srcbeg_.as<int>(), d_srcs_.as<uhmk_source>(), d_inv_.as<int>(),
                            d_arena_.as<double>(), d_arena32_.as<float>(), l.prec, d_stage_.as<double>(), st);
@@ -1138,6 +1180,14 @@ void FrontSolver::set_tuning(int key, int value) {
     if (value != 0 && value != 1) throw std::runtime_error("set_tuning: diagonal fusion is 0 or 1");
     if (value != fuse_diag_) schedule_method_ = 0;
     fuse_diag_ = value;
+  } else if (key == 10) {
+    if (value < 0) throw std::runtime_error("set_tuning: key 10 (low-precision Schur rows) takes a row count");
+    if (value != lp_min_ss_) schedule_method_ = 0;
+    lp_min_ss_ = value;
+  } else if (key == 9) {
+    if (value < 0) throw std::runtime_error("set_tuning: key 9 (fused small-front levels) takes 0 or a front count");
+    if (value != small_fused_) schedule_method_ = 0;
+    small_fused_ = value;
   } else if (key == 8) {
     if (value != 0 && value != 1) throw std::runtime_error("set_tuning: level solves are 0 or 1");
     if (value != solve_level_) solve_method_ = 0;
@@ -1150,7 +1200,9 @@ void FrontSolver::set_tuning(int key, int value) {
     throw std::runtime_error("set_tuning: unknown key (1: Cholesky panel group, 2: 128-tile threshold, "
                              "3: block-solve threshold, 4: whole-front LU limit, 5: Cholesky lookahead, "
                              "6: Cholesky diagonal blocks fused into the panel updates, 7: Cholesky panel groups "
-                             "in one persistent launch, 8: Cholesky level solves in one launch per level)");
+                             "in one persistent launch, 8: Cholesky level solves in one launch per level, "
+                             "9: fused small-front Cholesky levels, 10: smallest mean Schur block of a level "
+                             "with low-precision products)");
   }
 }
 
@@ -1163,6 +1215,8 @@ int FrontSolver::get_tuning(int key) const {
   if (key == 6) return fuse_diag_;
   if (key == 7) return group_kernel_;
   if (key == 8) return solve_level_;
+  if (key == 9) return small_fused_;
+  if (key == 10) return lp_min_ss_;
   throw std::runtime_error("get_tuning: unknown key");
 }
 

@@ -50,8 +50,12 @@ struct Launch {
   // TRSM, in-group updates) in one persistent launch
   // LP_PACK: the Schur operands of a level packed into BF16 / FP16 for the
   // low-precision SCHUR launch that follows (precision modes 2 and 3)
+  // CHOL_SMALL: assembly and whole Cholesky factorization (diagonal block,
+  // TRSM, Schur complement) of every front of a level with fs <= 64, a
+  // workgroup per front (front list)
   enum Kind {
-    ASSEMBLE, LU_PANEL, LU_ROWS, CHOL_DIAG, CHOL_TRSM, GEMM, SCHUR, LU_FRONT, GEMM_DIAG, CHOL_GROUP, LP_PACK
+    ASSEMBLE, LU_PANEL, LU_ROWS, CHOL_DIAG, CHOL_TRSM, GEMM, SCHUR, LU_FRONT, GEMM_DIAG, CHOL_GROUP, LP_PACK,
+    CHOL_SMALL
   } kind;
   int c0 = 0, nb = 0;
   long long off = 0;  // offset into the item (or front-list) array
@@ -67,7 +71,7 @@ struct Launch {
   int wait_ev = -1;   // event the launch's stream waits for first (-1: none)
   int record_ev = -1; // event recorded on the launch's stream after it (-1: none)
 };
-constexpr int N_LAUNCH_KINDS = 11;
+constexpr int N_LAUNCH_KINDS = 12;
 
 // One launch of the level solves.
 struct SolveStep {
@@ -239,6 +243,11 @@ class FrontSolver {
   int front_small_ = 64;  // LU fronts with n <= this (at most 64) factor in one whole-front launch
   int lookahead_ = 0;     // Cholesky: next panel's diagonal block and TRSM on the lookahead stream
   int fuse_diag_ = 1;     // Cholesky: next panel's diagonal block factored by the update's diagonal tile
+  int small_fused_ = 0;  // Cholesky levels of at least this many fronts, all with fs <= 64, run
+                         // as one whole-front launch (uhmk_chol_front_small); 0 never (default:
+                         // the four level launches measured faster, see set_tuning key 9)
+  int lp_min_ss_ = 768;   // precision modes 2 / 3: levels whose Schur blocks average fewer rows take
+                          // FP32 products instead of packed BF16 / FP16 ones
   int solve_level_ = 1;   // Cholesky solves with inverses: one launch per level and direction (1) or
                           // one per 64-row block step (0)
   int group_kernel_ = 2;  // Cholesky: panel groups in one persistent launch (uhmk_chol_group): 0 never,
