@@ -494,20 +494,31 @@ __global__ __launch_bounds__(256) void k_sv_gemv_cols(const uhmk_front* __restri
 // block to z and updates x below it; backward reads z, writes the solved
 // block to x and updates z above it.
 // ------------------------------------------------------------------------
+// The kb x kb diagonal block at D (column stride n) into registers (thread
+// tid: entries tid + 256 u, row-fastest), then into LDS as sW[r * 65 + c];
+// split so that the loads can be issued long before the block is needed.
 template <typename T>
-__device__ __forceinline__ void stage_diag_block(double* __restrict__ sW, const T* __restrict__ D, long long n, int kb,
-                                                 int tid) {
-  double v[16];
+__device__ __forceinline__ void load_diag_block(double (&v)[16], const T* __restrict__ D, long long n, int kb,
+                                                int tid) {
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
     const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
     v[u] = uhmd::ldz(D, r + (long long)c * n, r < kb && c < kb);
   }
+}
+__device__ __forceinline__ void store_diag_block(double* __restrict__ sW, const double (&v)[16], int tid) {
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
     const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
     sW[r * 65 + c] = v[u];
   }
+}
+template <typename T>
+__device__ __forceinline__ void stage_diag_block(double* __restrict__ sW, const T* __restrict__ D, long long n, int kb,
+                                                 int tid) {
+  double v[16];
+  load_diag_block(v, D, n, kb, tid);
+  store_diag_block(sW, v, tid);
 }
 
 // item {f, 0, first row of the chunk, 0}: r0 == k0 is the block itself,
@@ -855,14 +866,28 @@ __global__ __launch_bounds__(256) void k_sv_chol_fwd_level(const uhmk_front* __r
   double part[SOLVE_MAXR];
 #pragma unroll
   for (int c = 0; c < SOLVE_MAXR; ++c) part[c] = 0.0;
-  for (int j = 0; j < nterm; ++j) {
+  // the diagonal block (W) of an ATL block goes to LDS before the terms
+  // (sW is not used by them), so that its load is off the chain from the
+  // previous block's result
+  if (atl) stage_diag_block(sW, A + r0 + (long long)r0 * n, n, rend - r0, tid);
+  // the terms' L(r, j) entries are loaded one term ahead
+  double e[16], en[16];
+  auto load_term = [&](double (&d)[16], int j) {
     const int c0 = 64 * j, kbj = min(64, fs - c0);
-    double e[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int k = 16 * w + q;
-      e[q] = uhmd::ldz(A, row + (long long)(c0 + k) * n, row < rend && k < kbj);
+      d[q] = uhmd::ldz(A, row + (long long)(c0 + k) * n, row < rend && k < kbj);
     }
+  };
+  if (nterm > 0) load_term(e, 0);
+  for (int j = 0; j < nterm; ++j) {
+    const int c0 = 64 * j, kbj = min(64, fs - c0);
+    if (j > 0) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) e[q] = en[q];
+    }
+    if (j + 1 < nterm) load_term(en, j + 1);
     if (tid < 64)
       for (int c = 0; c < nrhs; ++c) sy[c][tid] = tid < kbj ? poll_value(zf + (long long)c * n + c0 + tid, err) : 0.0;
     __syncthreads();
@@ -885,7 +910,6 @@ __global__ __launch_bounds__(256) void k_sv_chol_fwd_level(const uhmk_front* __r
     return;
   }
   const int kb = rend - r0;
-  stage_diag_block(sW, A + r0 + (long long)r0 * n, n, kb, tid);
   if (tid < 64)
     for (int c = 0; c < nrhs; ++c)
       sy[c][tid] = tid < kb ? xf[(long long)c * n + row] -
@@ -943,6 +967,8 @@ __global__ __launch_bounds__(256) void k_sv_chol_bwd_level(const uhmk_front* __r
   const int nabl = (n - fs + 63) / 64, nblk = (fs + 63) / 64, rb = r0 / 64;
   const int npiece = nabl + (nblk - 1 - rb);
   auto piece_row = [&](int pc) { return pc < nabl ? fs + 64 * pc : 64 * (nblk - 1 - (pc - nabl)); };
+  double dv[16];
+  load_diag_block(dv, A + r0 + (long long)r0 * n, n, kb, tid);
   double v[16];
   auto load_piece = [&](int pc) {
     const int p0 = piece_row(pc), pend = pc < nabl ? n : min(p0 + 64, fs);
@@ -979,7 +1005,7 @@ __global__ __launch_bounds__(256) void k_sv_chol_bwd_level(const uhmk_front* __r
 #pragma unroll
   for (int c = 0; c < SOLVE_MAXR; ++c)
     if (c < nrhs) sp[c][w][lane] = part[c];
-  stage_diag_block(sW, A + r0 + (long long)r0 * n, n, kb, tid);
+  store_diag_block(sW, dv, tid);
   __syncthreads();
   if (tid < 64)
     for (int c = 0; c < nrhs; ++c)
