@@ -102,6 +102,23 @@ void uhm_random_spd_matrix_(uhm_fort_p* mesh);                              /* f
 void uhm_random_matrix_(uhm_fort_p* mesh);                                  /* fort.cxx:232 */
 /* No effect: the Cholesky kernels never read the upper triangle. */
 void uhm_triangularize_(uhm_fort_p* mesh);                                  /* fort.cxx:236 */
+/* Factorization reuse after a change of the mesh (no fort.cxx wrapper: the
+ * reference offers it through Mesh_::backup / check_reuse,
+ * UHM/src/mesh/mesh.cxx:113-225, and the skip in op_decompose,
+ * UHM/src/operation/element.cxx:204-206).  uhm_backup_ keeps mesh's
+ * connectivity in backup (whose own elements, nodes and fronts are dropped);
+ * after the mesh changed (uhm_unlock_, uhm_set_node_, uhm_lock_),
+ * uhm_check_reuse_ flags the elements whose factor and schur node lists
+ * are unchanged (same nodes, same n_dof + p + kind), none of whose
+ * descendants changed; the next uhm_create_matrix_without_buffer_ keeps
+ * their factored fronts, which the next factorization takes over and
+ * every factorization skips.  n_reusable: flagged elements; n_reused:
+ * fronts kept at the last factorization. */
+void uhm_set_node_(uhm_fort_p* mesh, uhm_fort_int* id, uhm_fort_int* phy, uhm_fort_int* n_dof, uhm_fort_int* p);
+void uhm_backup_(uhm_fort_p* mesh, uhm_fort_p* backup);
+void uhm_check_reuse_(uhm_fort_p* mesh, uhm_fort_p* backup);
+void uhm_get_n_reusable_(uhm_fort_p* mesh, uhm_fort_int* n);
+void uhm_get_n_reused_(uhm_fort_p* mesh, uhm_fort_int* n);
 void uhm_lock_(uhm_fort_p* mesh);                                           /* fort.cxx:241 */
 void uhm_unlock_(uhm_fort_p* mesh);                                         /* fort.cxx:245 */
 

@@ -67,6 +67,12 @@ int uhmk_assemble(const uhmk_front* fronts, const uhmk_item* items, int n_items,
                   const uhmk_source* srcs, const int* inv, double* arena64, float* arena32, int prec,
                   const double* stage, hipStream_t stream);
 
+/* count entries of a front from one arena into another, converted between
+ * storage types (prec 0 FP64, 1 FP32): the factored front of an element
+ * that a re-factorization reuses (Mesh_::check_reuse,
+ * UHM/src/mesh/mesh.cxx:159-225), moved into the new pattern's arena. */
+int uhmk_copy_front(const void* src, int src_prec, void* dst, int dst_prec, long long count, hipStream_t stream);
+
 /* Panel LU with partial pivoting restricted to ATL rows [c0, fs) for the
  * fronts in flist; writes relative pivots.  Replaces FLA_LU_piv on the
  * panel inside linal::dense::lu_piv (LINAL/src/dense/lu_piv.cxx:79).

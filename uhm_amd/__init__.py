@@ -89,6 +89,8 @@ _SIGS = {
     "uhm_set_tuning_": [_P, _I, _I], "uhm_get_tuning_": [_P, _I, _I], "uhm_autotune_": [_P, _I, _D],
     "uhm_improve_chol_": [_P, _D, _I, _I], "uhm_improve_lu_piv_": [_P, _D, _I, _I],
     "uhm_get_refine_history_": [_P, _D, _I, _D],
+    "uhm_set_node_": [_P, _I, _I, _I, _I], "uhm_backup_": [_P, _P], "uhm_check_reuse_": [_P, _P],
+    "uhm_get_n_reusable_": [_P, _I], "uhm_get_n_reused_": [_P, _I],
 }
 
 KERNEL_KINDS = ("assemble", "lu_panel", "lu_rows", "chol_diag", "chol_trsm", "gemm", "schur", "lu_front",
@@ -217,6 +219,30 @@ class Mesh:
 
     def build_tree(self):
         _call("uhm_build_tree_", C.byref(self.handle))
+
+    def set_node(self, id, n_dof, p=None, phy=0):
+        """New dof count (and order) of an existing node; the mesh must be
+        unlocked."""
+        _call("uhm_set_node_", C.byref(self.handle), _i(id), _i(phy), _i(n_dof), _i(n_dof if p is None else p))
+
+    def backup(self, other):
+        """Keep this mesh's connectivity in `other` (Mesh_::backup)."""
+        _call("uhm_backup_", C.byref(self.handle), C.byref(other.handle))
+
+    def check_reuse(self, other):
+        """Flag the elements whose factored fronts the next factorization
+        keeps (Mesh_::check_reuse against the connectivity `other` keeps)."""
+        _call("uhm_check_reuse_", C.byref(self.handle), C.byref(other.handle))
+
+    def n_reusable(self):
+        v = C.c_int(0)
+        _call("uhm_get_n_reusable_", C.byref(self.handle), C.byref(v))
+        return v.value
+
+    def n_reused(self):
+        v = C.c_int(0)
+        _call("uhm_get_n_reused_", C.byref(self.handle), C.byref(v))
+        return v.value
 
     def lock(self):
         _call("uhm_lock_", C.byref(self.handle))

@@ -180,6 +180,21 @@ void uhm_random_matrix_(uhm_fort_p* mesh) {
 void uhm_triangularize_(uhm_fort_p* mesh) {
   guard("uhm_triangularize", [&] { (void)M(mesh); });
 }
+void uhm_set_node_(uhm_fort_p* mesh, uhm_fort_int* id, uhm_fort_int* phy, uhm_fort_int* n_dof, uhm_fort_int* p) {
+  guard("uhm_set_node", [&] { M(mesh)->set_node(*id, *phy, *n_dof, *p); });
+}
+void uhm_backup_(uhm_fort_p* mesh, uhm_fort_p* backup) {
+  guard("uhm_backup", [&] { M(mesh)->backup(*M(backup)); });
+}
+void uhm_check_reuse_(uhm_fort_p* mesh, uhm_fort_p* backup) {
+  guard("uhm_check_reuse", [&] { M(mesh)->check_reuse(*M(backup)); });
+}
+void uhm_get_n_reusable_(uhm_fort_p* mesh, uhm_fort_int* n) {
+  guard("uhm_get_n_reusable", [&] { *n = M(mesh)->n_reusable(); });
+}
+void uhm_get_n_reused_(uhm_fort_p* mesh, uhm_fort_int* n) {
+  guard("uhm_get_n_reused", [&] { *n = S(M(mesh))->n_reused(); });
+}
 void uhm_lock_(uhm_fort_p* mesh) {
   guard("uhm_lock", [&] { M(mesh)->lock(); });
 }

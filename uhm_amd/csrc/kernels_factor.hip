@@ -50,6 +50,13 @@ __global__ __launch_bounds__(256) void k_assemble(const uhmk_front* __restrict__
   uhmd::asm_tile<T>(S, F[it.f].n, F[it.f].off, it.a, it.b, inv, arena64, arena32, stage);
 }
 
+// dst[i] = (TD)src[i], i < count (a front's entries into another arena)
+template <typename TS, typename TD>
+__global__ __launch_bounds__(256) void k_copy_cast(const TS* __restrict__ src, TD* __restrict__ dst, long long count) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (i < count) dst[i] = (TD)src[i];
+}
+
 // ------------------------------------------------------------------------
 // Partial LU panel on rows [c0, fs) x columns [c0, c0+kb) of each front,
 // pivots searched in rows [c0, fs) only (the reference restricts pivoting to
@@ -745,6 +752,25 @@ __global__ __launch_bounds__(256) void k_lu_front64(const uhmk_front* __restrict
 }  // namespace
 
 extern "C" {
+
+int uhmk_copy_front(const void* src, int src_prec, void* dst, int dst_prec, long long count, hipStream_t stream) {
+  if (count <= 0) return 0;
+  if (src_prec < 0 || src_prec > 1 || dst_prec < 0 || dst_prec > 1) return -1006;
+  const dim3 g((unsigned)((count + 255) / 256)), b(256);
+  if (src_prec == 0 && dst_prec == 0)
+    hipLaunchKernelGGL((k_copy_cast<double, double>), g, b, 0, stream, static_cast<const double*>(src),
+                       static_cast<double*>(dst), count);
+  else if (src_prec == 0)
+    hipLaunchKernelGGL((k_copy_cast<double, float>), g, b, 0, stream, static_cast<const double*>(src),
+                       static_cast<float*>(dst), count);
+  else if (dst_prec == 0)
+    hipLaunchKernelGGL((k_copy_cast<float, double>), g, b, 0, stream, static_cast<const float*>(src),
+                       static_cast<double*>(dst), count);
+  else
+    hipLaunchKernelGGL((k_copy_cast<float, float>), g, b, 0, stream, static_cast<const float*>(src),
+                       static_cast<float*>(dst), count);
+  return ret(hipGetLastError());
+}
 
 int uhmk_assemble(const uhmk_front* fronts, const uhmk_item* items, int n_items, const int* src_beg,
                   const uhmk_source* srcs, const int* inv, double* arena64, float* arena32, int prec,

@@ -282,6 +282,15 @@ Node_* Mesh_::find_node(int id, int phy) const {
   return it == node_index_.end() ? nullptr : it->second;
 }
 
+void Mesh_::set_node(int id, int phy, int n_dof, int p) {
+  if (locked_) throw std::runtime_error("set_node: mesh is locked (unlock first)");
+  if (n_dof < 0) throw std::runtime_error("set_node: negative dof count");
+  Node_* n = find_node(id, phy);
+  if (!n) throw std::runtime_error("set_node: no such node");
+  n->n_dof = n_dof;
+  n->p = p;
+}
+
 Element_* Mesh_::add_element(int gen) {
   if (locked_) throw std::runtime_error("add_element: mesh is locked");
   elements_.emplace_back();
@@ -461,11 +470,67 @@ bool Mesh_::import_file(const char* path) {
 }
 
 void Mesh_::create_matrix_without_buffer(int datatype, int n_rhs) {
-  // mesh/matrix.cxx:64-90: one front per element sized (fs, ss).
+  // mesh/matrix.cxx:64-90: one front per element sized (fs, ss); the
+  // fronts of reusable elements keep their factored entries, taken from the
+  // previous fronts at the next factorization.
   if (!locked_) throw std::runtime_error("create_matrix_without_buffer: mesh is not locked");
   if (datatype != UHM_REAL) throw std::runtime_error("create_matrix_without_buffer: only UHM_REAL (FLA_DOUBLE) is supported");
   if (n_rhs < 1) throw std::runtime_error("create_matrix_without_buffer: n_rhs < 1");
+  std::unique_ptr<FrontSolver> prev = std::move(solver_);
   solver_.reset(new FrontSolver(this, n_rhs));
+  if (prev && n_reusable() > 0) solver_->set_reuse_source(std::move(prev));
+}
+
+static int reuse_code(const Node_* n) { return n->n_dof + n->p + n->kind; }
+
+void Mesh_::backup(Mesh_& b) const {
+  // mesh.cxx:113-150
+  if (&b == this) throw std::runtime_error("backup: the backup mesh must be another mesh");
+  b.reset();
+  for (const Element_& e : elements_) {
+    ReuseRecord r;
+    for (const auto& nf : e.factor) r.factor.emplace_back(nf.first, reuse_code(nf.first));
+    for (const auto& nf : e.schur) r.schur.emplace_back(nf.first, reuse_code(nf.first));
+    b.backup_[e.id] = std::move(r);
+  }
+}
+
+void Mesh_::check_reuse(const Mesh_& b) {
+  // mesh.cxx:152-225.  The reference clears the ancestors of a
+  // non-reusable element while it walks the elements in id order, so an
+  // ancestor visited later gets its own comparison back; here every element
+  // is compared first and the ancestors are cleared afterwards, which makes
+  // "cannot reuse upper hierarchy" hold whatever the id order.
+  auto same = [](const std::vector<std::pair<Node_*, int>>& a, const std::vector<std::pair<const Node_*, int>>& r) {
+    if (a.size() != r.size()) return false;
+    for (size_t i = 0; i < a.size(); ++i)
+      if (a[i].first != r[i].first || reuse_code(a[i].first) != r[i].second) return false;
+    return true;
+  };
+  for (Element_& e : elements_) {
+    auto it = b.backup_.find(e.id);
+    e.reuse = it != b.backup_.end() && same(e.factor, it->second.factor) && same(e.schur, it->second.schur);
+  }
+  for (Element_& e : elements_)
+    if (!e.reuse)
+      for (Element_* a = e.parent; a; a = a->parent) a->reuse = false;
+}
+
+int Mesh_::n_reusable() const {
+  int c = 0;
+  for (const Element_& e : elements_) c += e.reuse ? 1 : 0;
+  return c;
+}
+
+void Mesh_::reset() {
+  // mesh.cxx:227-235
+  solver_.reset();
+  unlock();
+  elements_.clear();
+  nodes_.clear();
+  node_index_.clear();
+  backup_.clear();
+  id_element_ = 0;
 }
 
 void Mesh_::create_matrix_buffer(int) {

@@ -119,6 +119,9 @@ class Mesh_ {
   // --- mesh construction (mesh.cxx:268-349)
   Node_* add_node(int id, int phy, int n_dof, int p, int kind);
   Node_* find_node(int id, int phy) const;
+  // new dof count and order of an existing node (Node_::set_n_dof / set_p,
+  // node.hxx:72-73), mesh unlocked
+  void set_node(int id, int phy, int n_dof, int p);
   Element_* add_element(int gen = 0);
   Element_* find_element(int id) const;
   int get_n_nodes() const { return (int)nodes_.size(); }
@@ -153,6 +156,20 @@ class Mesh_ {
                 int side, double* buffer);
   void set_rhs();
 
+  // --- factorization reuse (mesh.cxx:113-225)
+  // b keeps this mesh's connectivity: per element id its factor and schur
+  // node lists as (node, n_dof + p + kind), as Mesh_::backup stores them
+  // (b's own elements, nodes and fronts are dropped first, as by reset()).
+  void backup(Mesh_& b) const;
+  // Reuse flag of every element against the connectivity b keeps: set where
+  // the element exists there with the same factor and schur lists (same
+  // nodes, same encoded values), cleared on every ancestor of an element
+  // without it.  The next create_matrix_without_buffer keeps the factored
+  // fronts of the reusable elements for the next factorization.
+  void check_reuse(const Mesh_& b);
+  int n_reusable() const;
+  void reset();
+
   // --- global quantities
   long long get_n_dof() const;           // mesh/matrix.cxx:189-200
   long long get_n_nonzero() const;       // mesh/matrix.cxx:213-226
@@ -170,6 +187,10 @@ class Mesh_ {
   std::vector<int> level_gen_;
   std::vector<Element_*> leaves_;
   std::unique_ptr<FrontSolver> solver_;
+  struct ReuseRecord {
+    std::vector<std::pair<const Node_*, int>> factor, schur;
+  };
+  std::unordered_map<int, ReuseRecord> backup_;  // element id -> connectivity kept by backup()
   friend class FrontSolver;
 };
 

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <numeric>
 #include <stdexcept>
+#include <unordered_map>
 
 
 namespace uhm {
@@ -116,6 +117,9 @@ void FrontSolver::set_precision(int mode, int top_levels) {
   top_fp64_ = top_levels;
   drop_graphs();
   layout_arenas();
+  // fronts kept from a previous pattern are lost with a re-laid-out arena
+  // once adopted; before adoption they are converted to the new storage
+  if (!reuse_src_ && created_) reuse_.assign(fr_.size(), 0);
   schedule_method_ = 0;
   solve_method_ = 0;
   factored_ = false;
@@ -431,9 +435,14 @@ void FrontSolver::build_schedule(int method) {
         items_.push_back(uhmk_item{f, i0, j0, 0});
       }
   };
+  // the fronts to factor: all but those kept from a previous pattern
+  flv_.assign(levels_.size(), {});
+  for (size_t L = 0; L < levels_.size(); ++L)
+    for (int f : levels_[L])
+      if (reuse_.empty() || !reuse_[f]) flv_[L].push_back(f);
   std::vector<char> lp_level(levels_.size(), 0);  // levels with low-precision Schur products
   for (int L = 0; L < (int)levels_.size(); ++L) {
-    const auto& lv = levels_[L];
+    const auto& lv = flv_[L];
     if (chol && nb == 64 && small_fused_ > 0 && (int)lv.size() >= small_fused_) {
       // a level of many small fronts (every fs <= 64): one launch runs the
       // assembly, diagonal block, TRSM and Schur complement of each front
@@ -913,7 +922,7 @@ void FrontSolver::build_schedule(int method) {
     double fl = 0, by = 0;
     switch (l.kind) {
       case Launch::ASSEMBLE:
-        for (int f : levels_[l.level]) {
+        for (int f : flv_[l.level]) {
           const double n = fr_[f].n;
           const double w = chol ? tri(n) : n * n;
           double rd = fr_[f].soff >= 0 ? w : 0.0;
@@ -1032,7 +1041,7 @@ void FrontSolver::build_schedule(int method) {
     std::vector<long long> lpoff(fr_.size(), 0);
     long long tot = 0;
     for (size_t L = 0; L < levels_.size(); ++L)
-      for (int f : levels_[L]) {
+      for (int f : flv_[L]) {
         lpoff[f] = tot;
         if (lp_level[L]) tot += (uhmk_schur_lp_size(fr_[f].n, fr_[f].fs, chol ? 0 : 1) + 63) / 64 * 64;
       }
@@ -1274,10 +1283,60 @@ double FrontSolver::autotune(int method) {
   return best;
 }
 
+void FrontSolver::set_reuse_source(std::unique_ptr<FrontSolver> prev) {
+  reuse_.assign(fr_.size(), 0);
+  reuse_src_.reset();
+  if (!prev || !prev->factored_) return;
+  for (int f = 0; f < (int)fr_.size(); ++f) reuse_[f] = order_[f]->reuse ? 1 : 0;
+  reuse_src_ = std::move(prev);
+  schedule_method_ = 0;
+}
+
+int FrontSolver::n_reused() const {
+  int c = 0;
+  for (char r : reuse_) c += r ? 1 : 0;
+  return c;
+}
+
+void FrontSolver::adopt_reused() {
+  // the previous pattern's front of the same element, its entries converted
+  // to this front's storage when the precision of its level differs
+  FrontSolver& P = *reuse_src_;
+  std::unordered_map<const Element_*, int> old_front;
+  for (int f = 0; f < (int)P.order_.size(); ++f) old_front[P.order_[f]] = f;
+  for (int f = 0; f < (int)fr_.size(); ++f) {
+    if (!reuse_[f]) continue;
+    auto it = old_front.find(order_[f]);
+    if (it == old_front.end() || P.fr_[it->second].n != fr_[f].n || P.fr_[it->second].fs != fr_[f].fs) {
+      // not reusable after all: refactor it and everything above it
+      for (int g = f; g >= 0; g = fr_[g].parent) reuse_[g] = 0;
+    }
+  }
+  for (int f = 0; f < (int)fr_.size(); ++f) {
+    if (!reuse_[f]) continue;
+    const uhmk_front& O = P.fr_[old_front[order_[f]]];
+    const uhmk_front& N = fr_[f];
+    const long long cnt = (long long)N.n * N.n;
+    const void* src = O.prec ? (const void*)(P.d_arena32_.as<float>() + O.off) : (const void*)(P.d_arena_.as<double>() + O.off);
+    void* dst = N.prec ? (void*)(d_arena32_.as<float>() + N.off) : (void*)(d_arena_.as<double>() + N.off);
+    check_k(uhmk_copy_front(src, O.prec, dst, N.prec, cnt, stream_), "reuse: front copy");
+    if (N.fs)
+      check_hip(hipMemcpyAsync(d_piv_.as<int>() + N.poff, P.d_piv_.as<int>() + O.poff, N.fs * sizeof(int),
+                               hipMemcpyDeviceToDevice, stream_),
+                "reuse: pivots");
+  }
+  check_hip(hipStreamSynchronize(stream_), "reuse");
+  reuse_src_.reset();
+}
+
 int FrontSolver::factor(int method) {
   if (method != UHM_LU_PIV && method != UHM_CHOL)
     throw std::runtime_error("factor: only UHM_CHOL and UHM_LU_PIV are supported");
   if (!created_) create_buffers();
+  if (reuse_src_) {
+    adopt_reused();
+    schedule_method_ = 0;
+  }
   upload_stage();
   if (schedule_method_ != method) build_schedule(method);
   hipEvent_t e0, e1;

@@ -119,6 +119,16 @@ class FrontSolver {
                 int with_lhs, int with_rhs);
 
   int factor(int method);  // 0, or element id + 1 of the first zero-pivot / non-SPD front
+
+  // Factorization reuse (Mesh_::check_reuse, mesh.cxx:159-225, and the skip
+  // in op_decompose, operation/element.cxx:204-206): fronts of elements
+  // flagged reusable take their factored entries (and LU pivots) from the
+  // solver `prev` of the previous pattern at the next factorization, and
+  // every factorization skips them.  prev must have factored; a reusable
+  // front whose size differs there is refactored with its ancestors.
+  void set_reuse_source(std::unique_ptr<FrontSolver> prev);
+  int n_reused() const;
+  bool factored() const { return factored_; }
   void solve(int method);
   void check(int method);
   double get_residual() const { return residual_1norm_; }
@@ -331,6 +341,10 @@ class FrontSolver {
   int solve_inv_ = -1;  // solve schedule built for inverse-carrying Cholesky factors (1) or not (0)
   int solve_block_ = -1;  // solve schedule built as block solves (1) or per right-hand side (0)
   bool factored_ = false;
+  std::vector<char> reuse_;                 // per front: factored entries kept, not refactored
+  std::unique_ptr<FrontSolver> reuse_src_;  // fronts of the previous pattern, until adopted
+  std::vector<std::vector<int>> flv_;       // per level: the fronts a factorization processes
+  void adopt_reused();
   bool profile_ = false;
   bool use_graph_ = false;
   hipGraphExec_t factor_graph_ = nullptr;
