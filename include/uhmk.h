@@ -51,6 +51,15 @@ typedef struct uhmk_source {
   int inv;
 } uhmk_source;
 
+/* One copy-in block: an m x ncol element matrix (ncol = m, side 0) or
+ * right-hand side block (side 1) of front f, column major at element koff
+ * of the packed values, its rows' (and for side 0 columns') front positions
+ * at element moff of the packed maps. */
+typedef struct uhmk_cblock {
+  long long koff;
+  int moff, m, ncol, f, side;
+} uhmk_cblock;
+
 /* Work item: four ints whose meaning is fixed per kernel. */
 typedef struct uhmk_item {
   int f, a, b, c;
@@ -66,6 +75,18 @@ typedef struct uhmk_item {
 int uhmk_assemble(const uhmk_front* fronts, const uhmk_item* items, int n_items, const int* src_beg,
                   const uhmk_source* srcs, const int* inv, double* arena64, float* arena32, int prec,
                   const double* stage, hipStream_t stream);
+
+/* Element front-end of copy_in (interf.cxx:108-204 -> Helper_::merge_A /
+ * merge_rhs_b into zeroed leaf fronts, UHM/include/uhm/matrix/uhm/
+ * helper.hxx:191-331): one workgroup per item {f, first block, blocks,
+ * side | zero_first << 1} adds its blocks, in order, into the stage of
+ * leaf front f (side 0: stage[map[i] + map[j] n] += K[i + j m]) or its
+ * right-hand side (side 1: b[xoff + map[i] + r n] += K[i + r m]), after
+ * zeroing the n x n stage / n x nrhs block when zero_first is set; within
+ * a block the positions are distinct, so the sums at every position are
+ * formed in block order, as the host scatter forms them. */
+int uhmk_copy_in(const uhmk_front* fronts, const uhmk_item* items, int n_items, const uhmk_cblock* blocks,
+                 const double* vals, const int* maps, double* stage, double* b, int nrhs, hipStream_t stream);
 
 /* count entries of a front from one arena into another, converted between
  * storage types (prec 0 FP64, 1 FP32): the factored front of an element

@@ -50,6 +50,37 @@ __global__ __launch_bounds__(256) void k_assemble(const uhmk_front* __restrict__
   uhmd::asm_tile<T>(S, F[it.f].n, F[it.f].off, it.a, it.b, inv, arena64, arena32, stage);
 }
 
+// Copy-in blocks of one (front, side) item added in order into the front's
+// stage (side 0) or right-hand side block (side 1), zeroed first when the
+// item says so.
+__global__ __launch_bounds__(256) void k_copy_in(const uhmk_front* __restrict__ F, const uhmk_item* __restrict__ items,
+                                                 const uhmk_cblock* __restrict__ blocks,
+                                                 const double* __restrict__ vals, const int* __restrict__ maps,
+                                                 double* __restrict__ stage, double* __restrict__ b, int nrhs) {
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, side = it.c & 1;
+  if (side == 0 && fr.soff < 0) return;
+  double* __restrict__ T = side ? b + fr.xoff : stage + fr.soff;
+  const long long len = (long long)n * (side ? nrhs : n);
+  if (it.c & 2) {
+    for (long long i = threadIdx.x; i < len; i += 256) T[i] = 0.0;
+    __syncthreads();
+  }
+  for (int q = 0; q < it.b; ++q) {
+    const uhmk_cblock c = blocks[it.a + q];
+    const int* __restrict__ mp = maps + c.moff;
+    const double* __restrict__ K = vals + c.koff;
+    const long long cnt = (long long)c.m * c.ncol;
+    for (long long idx = threadIdx.x; idx < cnt; idx += 256) {
+      const int i = (int)(idx % c.m), j = (int)(idx / c.m);
+      const long long col = side ? j : mp[j];
+      T[mp[i] + col * n] += K[idx];
+    }
+    __syncthreads();
+  }
+}
+
 // dst[i] = (TD)src[i], i < count (a front's entries into another arena)
 template <typename TS, typename TD>
 __global__ __launch_bounds__(256) void k_copy_cast(const TS* __restrict__ src, TD* __restrict__ dst, long long count) {
@@ -752,6 +783,13 @@ __global__ __launch_bounds__(256) void k_lu_front64(const uhmk_front* __restrict
 }  // namespace
 
 extern "C" {
+
+int uhmk_copy_in(const uhmk_front* fronts, const uhmk_item* items, int n_items, const uhmk_cblock* blocks,
+                 const double* vals, const int* maps, double* stage, double* b, int nrhs, hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  hipLaunchKernelGGL(k_copy_in, dim3(n_items), dim3(256), 0, stream, fronts, items, blocks, vals, maps, stage, b, nrhs);
+  return ret(hipGetLastError());
+}
 
 int uhmk_copy_front(const void* src, int src_prec, void* dst, int dst_prec, long long count, hipStream_t stream) {
   if (count <= 0) return 0;

@@ -245,8 +245,10 @@ void FrontSolver::build_symbolic() {
     if (m != fr_[f].n) throw std::runtime_error("build_symbolic: leaf dof count mismatch");
     gen_items_.push_back(uhmk_item{f, e->id, lmap_off_[f], m});
   }
-  hstage_.assign(nf, {});
-  hrhs_.assign(nf, {});
+  cin_blocks_.clear();
+  cin_vals_.clear();
+  cin_maps_.clear();
+  cin_touched_.assign(2 * (size_t)nf, 0);
 }
 
 size_t FrontSolver::device_bytes() const {
@@ -305,14 +307,16 @@ void FrontSolver::create_buffers() {
 // ------------------------------------------------------------ staging
 void FrontSolver::stage_lhs(int f, int m, const int* map, const double* K) {
   // interf.cxx:108-204 -> Helper_::merge_A into a zeroed leaf front:
-  // F[map[i], map[j]] += K[i, j].
+  // F[map[i], map[j]] += K[i, j], in call order, on the device at the next
+  // upload.
   Element_* e = order_[f];
   if (!e->is_leaf()) throw std::runtime_error("copy_in: element matrices go to leaf elements only");
   const int n = fr_[f].n;
-  auto& S = hstage_[f];
-  if (S.empty()) S.assign((size_t)n * n, 0.0);
-  for (int j = 0; j < m; ++j)
-    for (int i = 0; i < m; ++i) S[map[i] + (size_t)map[j] * n] += K[i + (size_t)j * m];
+  for (int i = 0; i < m; ++i)
+    if (map[i] < 0 || map[i] >= n) throw std::runtime_error("copy_in: map outside the front");
+  cin_blocks_.push_back(uhmk_cblock{(long long)cin_vals_.size(), (int)cin_maps_.size(), m, m, f, 0});
+  cin_vals_.insert(cin_vals_.end(), K, K + (size_t)m * m);
+  cin_maps_.insert(cin_maps_.end(), map, map + m);
   stage_dirty_ = true;
   vals_dirty_ = true;
 }
@@ -323,36 +327,63 @@ void FrontSolver::stage_rhs(int f, int m, int nr, const int* map, const double* 
   if (!e->is_leaf()) throw std::runtime_error("copy_in: right-hand sides go to leaf elements only");
   if (nr != nrhs_) throw std::runtime_error("copy_in: rhs column count differs from n_rhs");
   const int n = fr_[f].n;
-  auto& B = hrhs_[f];
-  if (B.empty()) B.assign((size_t)n * nrhs_, 0.0);
-  for (int r = 0; r < nr; ++r)
-    for (int i = 0; i < m; ++i) B[map[i] + (size_t)r * n] += b[i + (size_t)r * m];
+  for (int i = 0; i < m; ++i)
+    if (map[i] < 0 || map[i] >= n) throw std::runtime_error("copy_in: map outside the front");
+  cin_blocks_.push_back(uhmk_cblock{(long long)cin_vals_.size(), (int)cin_maps_.size(), m, nr, f, 1});
+  cin_vals_.insert(cin_vals_.end(), b, b + (size_t)m * nr);
+  cin_maps_.insert(cin_maps_.end(), map, map + m);
   rhs_dirty_ = true;
   bg_dirty_ = true;
 }
 
-void FrontSolver::upload_stage() {
-  if (!stage_dirty_) return;
-  for (size_t f = 0; f < fr_.size(); ++f) {
-    if (hstage_[f].empty()) continue;
-    check_hip(hipMemcpyAsync(d_stage_.as<double>() + fr_[f].soff, hstage_[f].data(),
-                             hstage_[f].size() * sizeof(double), hipMemcpyHostToDevice, stream_),
-              "upload stage");
+void FrontSolver::flush_copy_in() {
+  if (cin_blocks_.empty()) {
+    cin_vals_.clear();
+    cin_maps_.clear();
+    stage_dirty_ = rhs_dirty_ = false;
+    return;
   }
-  check_hip(hipStreamSynchronize(stream_), "upload stage");
-  stage_dirty_ = false;
+  // one workgroup per (front, side), its blocks in call order; a front's
+  // first block since creation / the last generate zeroes it first
+  std::vector<int> ord(cin_blocks_.size());
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+    const uhmk_cblock &x = cin_blocks_[a], &y = cin_blocks_[b];
+    return x.side != y.side ? x.side < y.side : x.f < y.f;
+  });
+  std::vector<uhmk_cblock> blocks;
+  std::vector<uhmk_item> items;
+  for (size_t q = 0; q < ord.size(); ++q) {
+    const uhmk_cblock& c = cin_blocks_[ord[q]];
+    if (items.empty() || items.back().f != c.f || (items.back().c & 1) != c.side) {
+      char& t = cin_touched_[2 * (size_t)c.f + c.side];
+      items.push_back(uhmk_item{c.f, (int)blocks.size(), 0, c.side | (t ? 0 : 2)});
+      t = 1;
+    }
+    ++items.back().b;
+    blocks.push_back(c);
+  }
+  d_cin_vals_.upload(cin_vals_, stream_);
+  d_cin_maps_.upload(cin_maps_, stream_);
+  d_cin_blocks_.upload(blocks, stream_);
+  d_cin_items_.upload(items, stream_);
+  check_k(uhmk_copy_in(dF_.as<uhmk_front>(), d_cin_items_.as<uhmk_item>(), (int)items.size(),
+                       d_cin_blocks_.as<uhmk_cblock>(), d_cin_vals_.as<double>(), d_cin_maps_.as<int>(),
+                       d_stage_.as<double>(), d_b_.as<double>(), nrhs_, stream_),
+          "copy in");
+  check_hip(hipStreamSynchronize(stream_), "copy in");
+  cin_blocks_.clear();
+  cin_vals_.clear();
+  cin_maps_.clear();
+  stage_dirty_ = rhs_dirty_ = false;
+}
+
+void FrontSolver::upload_stage() {
+  if (stage_dirty_) flush_copy_in();
 }
 
 void FrontSolver::upload_rhs() {
-  if (!rhs_dirty_) return;
-  for (size_t f = 0; f < fr_.size(); ++f) {
-    if (hrhs_[f].empty()) continue;
-    check_hip(hipMemcpyAsync(d_b_.as<double>() + fr_[f].xoff, hrhs_[f].data(), hrhs_[f].size() * sizeof(double),
-                             hipMemcpyHostToDevice, stream_),
-              "upload rhs");
-  }
-  check_hip(hipStreamSynchronize(stream_), "upload rhs");
-  rhs_dirty_ = false;
+  if (rhs_dirty_) flush_copy_in();
 }
 
 void FrontSolver::set_rhs() {
@@ -373,13 +404,22 @@ void FrontSolver::generate(unsigned long long seed, double alpha, double beta, i
                                  d_b_.as<double>(), nrhs_, with_lhs, with_rhs, stream_),
           "generate");
   check_hip(hipStreamSynchronize(stream_), "generate");
+  // the generated values replace the stages / right-hand sides: copy-in
+  // blocks not yet scattered are dropped, later ones start from zero
+  auto drop_side = [&](int side) {
+    std::vector<uhmk_cblock> keep;
+    for (const uhmk_cblock& c : cin_blocks_)
+      if (c.side != side) keep.push_back(c);
+    cin_blocks_.swap(keep);
+    for (size_t f = 0; f < fr_.size(); ++f) cin_touched_[2 * f + side] = 0;
+  };
   if (with_lhs) {
-    for (auto& s : hstage_) s.clear();
+    drop_side(0);
     stage_dirty_ = false;
     vals_dirty_ = true;
   }
   if (with_rhs) {
-    for (auto& s : hrhs_) s.clear();
+    drop_side(1);
     rhs_dirty_ = false;
     bg_dirty_ = true;
   }

@@ -287,8 +287,15 @@ class FrontSolver {
   int piv_elems_ = 0;
 
   // host staging
-  std::vector<std::vector<double>> hstage_;
-  std::vector<std::vector<double>> hrhs_;
+  // copy-in batch: the element blocks of copy_in calls in call order (values
+  // and front positions packed), scattered into the stages / right-hand
+  // sides by one device launch at the next upload (uhmk_copy_in)
+  std::vector<uhmk_cblock> cin_blocks_;
+  std::vector<double> cin_vals_;
+  std::vector<int> cin_maps_;
+  std::vector<char> cin_touched_;  // per (front, side): accumulating since creation / the last generate
+  DevBuf d_cin_vals_, d_cin_maps_, d_cin_blocks_, d_cin_items_;
+  void flush_copy_in();
   bool stage_dirty_ = false, rhs_dirty_ = false;
 
   // schedule
