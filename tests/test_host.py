@@ -159,3 +159,33 @@ def test_numeric_path_fails_loudly_without_gpu(lib):
     m.create_matrix_without_buffer(U.UHM_REAL, 1)
     with pytest.raises(U.UHMError, match="HIP device|GPU"):
         m.lu_piv_with_free()
+
+
+def test_import_uhm_fixture_then_build_tree_matches_oracle_lock(lib, oracle_lib):
+    """uhm_mesh_import_ (mesh.cxx:511-579) on a committed ".uhm" file: the
+    mesh is emptied first, the nodes no element holds are dropped
+    (remove_orphan_nodes, mesh.cxx:246-251, 573) and the others keep their
+    creation order; the leaf elements are the file's.  uhm_build_tree_ then
+    gives a tree whose lock (fronts, numbering, mapper runs, schedule)
+    equals the oracle's lock of the same connectivity."""
+    path = os.path.join(ROOT, "tests", "golden", "quad_q2_6x5.uhm")
+    spec = qp_spec((6, 5), 2)
+    m = U.Mesh()
+    m.build_synthetic(1, (2, 2), 2, 0)  # content that the import replaces
+    m.import_file(path)
+    c = m.export_connectivity()
+    assert c["node_id"].tolist() == [nid for nid, _ in spec.nodes]
+    assert len(c["parent"]) == len(spec.elems) and (c["parent"] == -1).all()
+    ids = c["node_id"]
+    for e, nl in enumerate(spec.elems):
+        got = [int(ids[k]) for k in c["enode"][c["eptr"][e]:c["eptr"][e + 1]]]
+        assert sorted(got) == sorted(nl), e
+    m.build_tree()
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    c = m.export_connectivity()
+    roots = [e for e, p in enumerate(c["parent"]) if p < 0]
+    assert len(roots) == 1 and len(c["parent"]) == 2 * len(spec.elems) - 1
+    assert m.get_n_dof() == len(spec.nodes)
+    tree = spec_from_connectivity(c)
+    _compare(m, tree, tree.oracle())

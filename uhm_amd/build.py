@@ -60,7 +60,8 @@ def build(verbose=False, jobs=None):
                 print(out)
     tmp = LIB + ".tmp"
     _run(["g++", "-shared", "-o", tmp] + objs +
-         ["-L" + os.path.join(ROCM, "lib"), "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-lamdhip64"])
+         ["-Wl,--version-script=" + os.path.join(CSRC, "exports.map"),
+          "-L" + os.path.join(ROCM, "lib"), "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-lamdhip64"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -434,11 +434,39 @@ void Mesh_::build_tree_bisection() {
   update_generation();
 }
 
+void Mesh_::remove_orphan_nodes() {
+  // mesh.cxx:246-251.  The kept nodes move to a new container (creation
+  // order and seq unchanged) and every element's node pointers follow them.
+  if (locked_) throw std::runtime_error("remove_orphan_nodes: mesh is locked");
+  std::deque<Node_> kept;
+  std::unordered_map<const Node_*, Node_*> to;
+  for (Node_& n : nodes_)
+    if (n.get_n_owner()) {
+      kept.push_back(n);
+      to[&n] = &kept.back();
+    }
+  if (kept.size() == nodes_.size()) return;
+  for (Element_& e : elements_) {
+    for (auto& nf : e.nodes) nf.first = to.at(nf.first);
+    for (auto& nf : e.factor) nf.first = to.at(nf.first);
+    for (auto& nf : e.schur) nf.first = to.at(nf.first);
+  }
+  nodes_.swap(kept);
+  node_index_.clear();
+  int seq = 0;
+  for (Node_& n : nodes_) {
+    n.seq = seq++;  // creation indices stay in order, without the gaps
+    node_index_[node_key(n.id, n.phy)] = &n;
+  }
+}
+
 bool Mesh_::import_file(const char* path) {
   // mesh.cxx:511-579: "UHM" / n_nodes / id phy ndof p kind / n_elements /
-  // per element: n_nods then one "id phy" line per node.
+  // per element: n_nods then one "id phy" line per node; the mesh is emptied
+  // first and the nodes no element holds are dropped at the end.
   FILE* fp = fopen(path, "r");
   if (!fp) throw std::runtime_error(std::string("import_file: cannot open ") + path);
+  reset();
   char fmt[32] = {0};
   int n_nodes = 0, n_elements = 0;
   auto fail = [&](const char* what) {
@@ -466,6 +494,7 @@ bool Mesh_::import_file(const char* path) {
     }
   }
   fclose(fp);
+  remove_orphan_nodes();
   return true;
 }
 

@@ -130,6 +130,7 @@ class Mesh_ {
   Element_* element_at(int i) { return &elements_[i]; }
   Node_* node_at(int i) { return &nodes_[i]; }
   bool import_file(const char* path);  // ".uhm" text format (mesh.cxx:511-579)
+  void remove_orphan_nodes();          // nodes no element holds (mesh.cxx:246-251)
 
   // --- tree + analysis
   void update_generation();  // leaf 0, parent = min(children) - 1 (element.cxx:63-72)
