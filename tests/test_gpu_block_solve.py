@@ -88,7 +88,9 @@ def test_block_solve_matches_per_column(lib, case):
     err = (np.abs(xb - xs).max(axis=0) / np.abs(xs).max(axis=0)).max()
     say("  max fs", max(m.front_info(f)["fs"] for f in range(m.n_fronts())),
         " worst column rel diff block vs per-column %.3e" % err)
-    assert err <= 1e-12
+    # FP32 levels: the block path multiplies in FP32 (the factor's own
+    # accuracy), the per-column path in FP64 on the same FP32 entries
+    assert err <= (1e-12 if top is None else 1e-5)
     assert not np.array_equal(xb[:, 0], xb[:, 1])
     if top is None:
         _check(m, method)

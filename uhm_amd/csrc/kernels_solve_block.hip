@@ -200,6 +200,112 @@ __global__ __launch_bounds__(256, 2) void k_svb_gemm(const uhmk_front* __restric
       }
 }
 
+// The same products for levels whose fronts are stored in FP32: operands
+// rounded to FP32 on their way to LDS (the right-hand sides from FP64),
+// v_mfma_f32_16x16x4f32 products and sums over 32-deep stages, the tile
+// applied to the FP64 vectors.  The factor entries carry FP32 accuracy, so
+// FP32 arithmetic keeps the solution at the accuracy the factor gives.
+// Operand lanes as in mma_quarter; accumulator register r of lane l holds
+// acc(c = 16 cb + 4 (l / 16) + r, i = 16 ib + l % 16).
+typedef float f4 __attribute__((ext_vector_type(4)));
+constexpr int LDF = 80;  // row stride (floats) of the FP32 LDS panels: the four k rows a wavefront reads hit distinct banks
+
+template <bool TA>
+__global__ __launch_bounds__(256, 2) void k_svb_gemm32(const uhmk_front* __restrict__ F,
+                                                       const uhmk_item* __restrict__ items,
+                                                       const float* __restrict__ arena, double* __restrict__ x,
+                                                       double* __restrict__ z, int nrhs, int fwd, int ks,
+                                                       double* __restrict__ part) {
+  constexpr int KC = 32;
+  __shared__ float sA[2][KC * LDF];
+  __shared__ float sB[2][KC * LDF];
+  const int q = blockIdx.x / ks, sl = blockIdx.x - q * ks;
+  const uhmk_item it = items[q];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs;
+  const int c0 = it.a, r0 = it.b;
+  const int rend = fwd ? n : fs;
+  const int kb0 = fwd ? 0 : fs, ke0 = fwd ? fs : n;
+  const int len = (((ke0 - kb0) + ks - 1) / ks + KC - 1) / KC * KC;
+  const int kbeg = min(ke0, kb0 + sl * len), kend = min(ke0, kbeg + len);
+  const double* __restrict__ S = (fwd ? z : x) + fr.xoff + (long long)c0 * n;
+  const float* __restrict__ A = arena + fr.off;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int cb = 32 * (w & 1), ib = 32 * (w >> 1), qq = lane & 15, kq = lane >> 4;
+  f4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+  float pa[8], pb[8];
+  auto load = [&](int k) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (!TA) {
+        const int i = tid & 63, kk = (tid >> 6) + 4 * u;
+        pa[u] = (float)uhmd::ldz(A, (r0 + i) + (long long)(k + kk) * n, r0 + i < rend && k + kk < kend);
+      } else {
+        const int kk = tid & 31, i = (tid >> 5) + 8 * u;
+        pa[u] = (float)uhmd::ldz(A, (k + kk) + (long long)(r0 + i) * n, r0 + i < rend && k + kk < kend);
+      }
+      const int kk = tid & 31, c = (tid >> 5) + 8 * u;
+      pb[u] = (float)uhmd::ldz(S, (k + kk) + (long long)c * n, k + kk < kend && c0 + c < nrhs);
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (!TA) {
+        const int i = tid & 63, kk = (tid >> 6) + 4 * u;
+        sA[buf][kk * LDF + i] = pa[u];
+      } else {
+        const int kk = tid & 31, i = (tid >> 5) + 8 * u;
+        sA[buf][kk * LDF + i] = pa[u];
+      }
+      const int kk = tid & 31, c = (tid >> 5) + 8 * u;
+      sB[buf][kk * LDF + c] = pb[u];
+    }
+  };
+  if (kbeg < kend) {
+    load(kbeg);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k = kbeg; k < kend; k += KC) {
+      const bool more = k + KC < kend;
+      if (more) load(k + KC);
+#pragma unroll
+      for (int kk = 0; kk < KC; kk += 4) {
+        const float* rb = sB[buf] + (kk + kq) * LDF + cb + qq;
+        const float* ra = sA[buf] + (kk + kq) * LDF + ib + qq;
+        const float b0 = rb[0], b1 = rb[16], a0 = ra[0], a1 = ra[16];
+        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, a0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, a1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, a0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, a1, acc[1][1], 0, 0, 0);
+      }
+      if (more) store(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+  }
+  double* __restrict__ C = (fwd ? x : z) + fr.xoff;
+  double* __restrict__ P = part + (long long)blockIdx.x * 4096;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int c = cb + 16 * a + 4 * kq + g, i = ib + 16 * b + qq;
+        if (ks == 1) {
+          if (r0 + i < rend && c0 + c < nrhs) C[(r0 + i) + (long long)(c0 + c) * n] -= (double)acc[a][b][g];
+        } else {
+          P[c * 64 + i] = (double)acc[a][b][g];
+        }
+      }
+}
+
 // dst(r0 + i, c0 + c) -= sum over the ks slices (in order) of the partial
 // tiles of item q; one workgroup per item.
 __global__ __launch_bounds__(256) void k_svb_reduce(const uhmk_front* __restrict__ F,
@@ -610,10 +716,16 @@ int launch_gemm(int dir, const uhmk_front* fronts, const uhmk_item* items, int n
                 double* z, int nrhs, int ks, double* part, hipStream_t stream) {
   const int fwd = dir == 0 ? 1 : 0;
   const dim3 g((unsigned)((long long)n_items * ks)), b(256);
-  if (dir == 2)
+  if constexpr (sizeof(T) == 4) {
+    if (dir == 2)
+      hipLaunchKernelGGL((k_svb_gemm32<true>), g, b, 0, stream, fronts, items, arena, x, z, nrhs, fwd, ks, part);
+    else
+      hipLaunchKernelGGL((k_svb_gemm32<false>), g, b, 0, stream, fronts, items, arena, x, z, nrhs, fwd, ks, part);
+  } else if (dir == 2) {
     hipLaunchKernelGGL((k_svb_gemm<T, true>), g, b, 0, stream, fronts, items, arena, x, z, nrhs, fwd, ks, part);
-  else
+  } else {
     hipLaunchKernelGGL((k_svb_gemm<T, false>), g, b, 0, stream, fronts, items, arena, x, z, nrhs, fwd, ks, part);
+  }
   if (ks > 1)
     hipLaunchKernelGGL(k_svb_reduce, dim3(n_items), b, 0, stream, fronts, items, x, z, nrhs, fwd, ks, part);
   return ret(hipGetLastError());
