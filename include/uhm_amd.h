@@ -240,11 +240,18 @@ void uhm_get_refine_solves_(uhm_fort_p* mesh, uhm_fort_int* n);
  * smallest mean Schur block (rows) of a tree level whose Schur products
  * take packed BF16 / FP16 operands, the levels below it FP32 products
  * (default 768: on small blocks the packing saves next to nothing and the
- * rounding costs refinement steps; 0: every level). */
+ * rounding costs refinement steps; 0: every level); keys 100 + c,
+ * 110 + c, 120 + c = the panel group, tile threshold and group-launch
+ * policy (as keys 1, 2, 7) of the fronts of size class c (c = 0: fs <= 64,
+ * 1: fs <= 512, 2: larger), whose fronts each tree level schedules as
+ * separate launches, class after class (0 / 0 / -1, the defaults: the
+ * global key). */
 void uhm_set_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value);
 void uhm_get_tuning_(uhm_fort_p* mesh, uhm_fort_int* key, uhm_fort_int* value);
 /* Autotuner: times factorizations of the current values over the knobs
- * (coordinate search) and keeps the fastest setting; best_ms = its time. */
+ * (coordinate search: keys 1, 2, 5, then keys 100 + c and 110 + c of every
+ * size class the pattern has) and keeps the fastest setting; best_ms = its
+ * time. */
 void uhm_autotune_(uhm_fort_p* mesh, uhm_fort_int* method, uhm_fort_double* best_ms);
 /* Graph mode (on = 1): the factorization's and each solve's launch
  * sequence is captured once as a HIP graph and replayed afterwards, for

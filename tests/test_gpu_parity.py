@@ -292,18 +292,31 @@ def test_schedule_knobs_preserve_results(lib):
     m.set_rhs()
     assert m.chol_with_free() == 0
     ref = [np.tril(m.fetch_front(f)) for f in range(m.n_fronts())]
-    for key, val in [(1, 1), (1, 2), (1, 8), (1, 12), (2, 64), (2, 1024)]:
+    # global knobs, then the per-size-class ones (class 1: 64 < fs <= 512,
+    # class 2: larger; keys 100 + c panel group, 110 + c tile threshold,
+    # 120 + c group launches)
+    knobs = [(1, 1), (1, 2), (1, 8), (1, 12), (2, 64), (2, 1024),
+             (101, 2), (102, 1), (111, 64), (112, 64), (121, 0), (122, 1)]
+    defaults = {1: 0, 2: 8192, 101: 0, 102: 0, 111: 0, 112: 0, 121: -1, 122: -1}
+    for key, val in knobs:
         m.set_tuning(key, val)
+        assert m.get_tuning(key) == val
         assert m.chol_with_free() == 0
         for f in range(m.n_fronts()):
             G = np.tril(m.fetch_front(f))
             err = np.abs(G - ref[f]).max() / max(1e-300, np.abs(ref[f]).max())
             assert err <= 1e-12, (key, val, f, err)
-        m.set_tuning(1, 0)
-        m.set_tuning(2, 1024)
+        m.set_tuning(key, defaults[key])
+    classes = sorted({0 if fs <= 64 else (1 if fs <= 512 else 2)
+                      for fs in (m.front_info(f)["fs"] for f in range(m.n_fronts())) if fs > 0})
+    assert len(classes) >= 2
     best, knobs = m.autotune(U.UHM_CHOL)
     say("  autotune", best, knobs)
     assert best > 0 and knobs["chol_group"] in (1, 2, 4, 6, 8, 12)
+    for c in classes:
+        assert knobs["class%d" % c]["tile_threshold"] in (0, 512, 2048, 8192)
+        if c > 0:
+            assert knobs["class%d" % c]["chol_group"] in (0, 1, 2, 4, 6, 8, 12)
     assert m.chol_with_free() == 0
     m.solve_chol()
     m.check_chol()

@@ -99,6 +99,10 @@ _RET = {"uhm_get_last_error_": C.c_int}
 
 
 PREC_FP64, PREC_MIXED, PREC_MIXED_BF16, PREC_MIXED_FP16 = 0, 1, 2, 3
+# front size classes of the per-class schedule knobs (tuning keys 100 + c,
+# 110 + c, 120 + c): fs <= 64, fs <= 512, larger
+N_FRONT_CLASSES = 3
+FRONT_CLASS_LIMITS = (64, 512)
 REFINE_CLASSICAL, REFINE_GMRES, REFINE_AUTO = 0, 1, 2
 
 
@@ -374,8 +378,12 @@ class Mesh:
         returns (best ms, {knob: value})."""
         t = C.c_double(0)
         _call("uhm_autotune_", C.byref(self.handle), _i(method), C.byref(t))
-        return t.value, {"chol_group": self.get_tuning(1), "tile_threshold": self.get_tuning(2),
-                         "lookahead": self.get_tuning(5)}
+        knobs = {"chol_group": self.get_tuning(1), "tile_threshold": self.get_tuning(2),
+                 "lookahead": self.get_tuning(5)}
+        for c in range(N_FRONT_CLASSES):
+            knobs["class%d" % c] = {"chol_group": self.get_tuning(100 + c), "tile_threshold": self.get_tuning(110 + c),
+                                    "group_launch": self.get_tuning(120 + c)}
+        return t.value, knobs
 
     def set_graph(self, on=True):
         """Capture the factorization / solves as HIP graphs and replay them."""

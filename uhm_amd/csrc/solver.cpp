@@ -480,7 +480,7 @@ void FrontSolver::build_schedule(int method) {
   for (size_t L = 0; L < levels_.size(); ++L)
     for (int f : levels_[L])
       if (reuse_.empty() || !reuse_[f]) flv_[L].push_back(f);
-  std::vector<char> lp_level(levels_.size(), 0);  // levels with low-precision Schur products
+  std::vector<char> lp_front(fr_.size(), 0);  // fronts with low-precision Schur products
   for (int L = 0; L < (int)levels_.size(); ++L) {
     const auto& lv = flv_[L];
     if (chol && nb == 64 && small_fused_ > 0 && (int)lv.size() >= small_fused_) {
@@ -515,12 +515,12 @@ void FrontSolver::build_schedule(int method) {
     if (la.count) push_main(la);
     // LU fronts that fit one wavefront factor in one whole-front launch;
     // the panel / row / GEMM launches below cover the others
-    std::vector<int> big;
+    std::vector<int> all_big;
     {
       std::vector<int> small;
       for (int f : lv) {
         if (!chol && fr_[f].fs > 0 && fr_[f].n <= std::min(64, front_small_)) small.push_back(f);
-        else big.push_back(f);
+        else all_big.push_back(f);
       }
       if (!small.empty()) {
         Launch lf;
@@ -532,6 +532,17 @@ void FrontSolver::build_schedule(int method) {
         push_main(lf);
       }
     }
+    // size classes of the level's fronts (front_class: fs <= 64, <= 512,
+    // larger), each scheduled as its own set of launches with the knobs of
+    // its class (panel group, GEMM tile threshold, group launches), one
+    // class after the other
+    std::vector<int> bucket[N_FRONT_CLASSES];
+    for (int f : all_big) bucket[front_class(fr_[f].fs)].push_back(f);
+    for (int cls = 0; cls < N_FRONT_CLASSES; ++cls) {
+    const std::vector<int>& big = bucket[cls];
+    if (big.empty()) continue;
+    const double tile_threshold = cls_tile_[cls] > 0 ? cls_tile_[cls] : tile_threshold_;
+    const int group_kernel = cls_gkern_[cls] >= 0 ? cls_gkern_[cls] : group_kernel_;
     int maxfs = 0;
     for (int f : big) maxfs = std::max(maxfs, fr_[f].fs);
     // Cholesky with 64-wide panels: groups of G panels; each panel updates
@@ -539,7 +550,7 @@ void FrontSolver::build_schedule(int method) {
     // and the trailing matrix receives one update with K = 64 G per group,
     // so the trailing entries are read and written once per group instead
     // of once per panel.
-    const int G = (chol && nb == 64) ? std::max(1, chol_group()) : 1;
+    const int G = (chol && nb == 64) ? std::max(1, class_group(cls)) : 1;
     // trailing update of the panel group [g0, g0 + GW): rows [ge, n),
     // columns [ge, fs), K = [g0, ge), lower tiles
     auto emit_trailing = [&](int g0, int GW) {
@@ -558,7 +569,7 @@ void FrontSolver::build_schedule(int method) {
         sc += fr_[f].fs - ge;
         ++cnt;
       }
-      lt.tile = cnt ? gemm_tile_for(sr / cnt, sc / cnt, tile_threshold_) : 64;
+      lt.tile = cnt ? gemm_tile_for(sr / cnt, sc / cnt, tile_threshold) : 64;
       const int T = lt.tile;
       lt.off = (long long)items_.size();
       for (int f : big) {
@@ -581,7 +592,7 @@ void FrontSolver::build_schedule(int method) {
     // levels with few fronts (the top of the tree, whose panel chains are
     // long); below, the many small fronts of a level fill the chip better
     // with one launch per panel step
-    const bool group_level = group_kernel_ == 1 || (group_kernel_ == 2 && (int)big.size() <= 16);
+    const bool group_level = group_kernel == 1 || (group_kernel == 2 && (int)big.size() <= 16);
     if (chol && nb == 64 && group_level) {
       // each group of G panels in one persistent launch (uhmk_chol_group):
       // per front one item per 64-row tile of rows [g0, n), the tiles of
@@ -770,7 +781,7 @@ void FrontSolver::build_schedule(int method) {
             sc += fr_[f].fs - ge;
             ++cnt;
           }
-          lt.tile = cnt ? gemm_tile_for(sr / cnt, sc / cnt, tile_threshold_) : 64;
+          lt.tile = cnt ? gemm_tile_for(sr / cnt, sc / cnt, tile_threshold) : 64;
         }
         const int T = lt.tile;
         if (la) {
@@ -835,7 +846,7 @@ void FrontSolver::build_schedule(int method) {
           sr += fr_[f].n - c1;
           sc += (chol ? fr_[f].fs : fr_[f].n) - c1;
         }
-        lg.tile = gemm_tile_for(sr / act.size(), sc / act.size(), tile_threshold_);
+        lg.tile = gemm_tile_for(sr / act.size(), sc / act.size(), tile_threshold);
       }
       const int T = lg.tile;
       if (la) {
@@ -898,10 +909,11 @@ void FrontSolver::build_schedule(int method) {
       // refinement steps
       if (lowp && (!cnt || sss / cnt < lp_min_ss_)) lowp = false;
       ls.mma = lowp ? prec_mode_ : -1;
-      lp_level[L] = lowp;
+      if (lowp)
+        for (int f : big) lp_front[f] = 1;
       // the packed low-precision products run 64-wide tiles at every size
       // (eight workgroups per CU hide their load latency)
-      ls.tile = cnt && !lowp ? gemm_tile_for(sss / cnt, sss / cnt, tile_threshold_) : 64;
+      ls.tile = cnt && !lowp ? gemm_tile_for(sss / cnt, sss / cnt, tile_threshold) : 64;
     }
     if (lowp) {
       // pack the operands of every front of the level (64 x 64 tiles over
@@ -934,6 +946,7 @@ void FrontSolver::build_schedule(int method) {
     }
     ls.count = (int)(items_.size() - ls.off);
     if (ls.count) push_main(ls);
+    }  // size classes
   }
   final_wait_ev_ = pending;
   while ((int)la_ev_.size() < n_events) {
@@ -1083,7 +1096,7 @@ void FrontSolver::build_schedule(int method) {
     for (size_t L = 0; L < levels_.size(); ++L)
       for (int f : flv_[L]) {
         lpoff[f] = tot;
-        if (lp_level[L]) tot += (uhmk_schur_lp_size(fr_[f].n, fr_[f].fs, chol ? 0 : 1) + 63) / 64 * 64;
+        if (lp_front[f]) tot += (uhmk_schur_lp_size(fr_[f].n, fr_[f].fs, chol ? 0 : 1) + 63) / 64 * 64;
       }
     lp_elems_ = tot;
     d_lp_.reserve(std::max(1LL, tot) * sizeof(unsigned short));
@@ -1229,6 +1242,18 @@ void FrontSolver::set_tuning(int key, int value) {
     if (value != 0 && value != 1) throw std::runtime_error("set_tuning: diagonal fusion is 0 or 1");
     if (value != fuse_diag_) schedule_method_ = 0;
     fuse_diag_ = value;
+  } else if (key >= 100 && key < 100 + N_FRONT_CLASSES) {
+    if (value < 0) throw std::runtime_error("set_tuning: class panel group takes 0 (global) or a panel count");
+    if (value != cls_group_[key - 100]) schedule_method_ = 0;
+    cls_group_[key - 100] = value;
+  } else if (key >= 110 && key < 110 + N_FRONT_CLASSES) {
+    if (value < 0) throw std::runtime_error("set_tuning: class tile threshold takes 0 (global) or an extent");
+    if (value != cls_tile_[key - 110]) schedule_method_ = 0;
+    cls_tile_[key - 110] = value;
+  } else if (key >= 120 && key < 120 + N_FRONT_CLASSES) {
+    if (value < -1 || value > 2) throw std::runtime_error("set_tuning: class group launches take -1 (global), 0, 1 or 2");
+    if (value != cls_gkern_[key - 120]) schedule_method_ = 0;
+    cls_gkern_[key - 120] = value;
   } else if (key == 10) {
     if (value < 0) throw std::runtime_error("set_tuning: key 10 (low-precision Schur rows) takes a row count");
     if (value != lp_min_ss_) schedule_method_ = 0;
@@ -1251,7 +1276,8 @@ void FrontSolver::set_tuning(int key, int value) {
                              "6: Cholesky diagonal blocks fused into the panel updates, 7: Cholesky panel groups "
                              "in one persistent launch, 8: Cholesky level solves in one launch per level, "
                              "9: fused small-front Cholesky levels, 10: smallest mean Schur block of a level "
-                             "with low-precision products)");
+                             "with low-precision products, 100 + c / 110 + c / 120 + c: panel group / tile "
+                             "threshold / group launches of front size class c)");
   }
 }
 
@@ -1266,14 +1292,18 @@ int FrontSolver::get_tuning(int key) const {
   if (key == 8) return solve_level_;
   if (key == 9) return small_fused_;
   if (key == 10) return lp_min_ss_;
+  if (key >= 100 && key < 100 + N_FRONT_CLASSES) return cls_group_[key - 100];
+  if (key >= 110 && key < 110 + N_FRONT_CLASSES) return cls_tile_[key - 110];
+  if (key >= 120 && key < 120 + N_FRONT_CLASSES) return cls_gkern_[key - 120];
   throw std::runtime_error("get_tuning: unknown key");
 }
 
 double FrontSolver::autotune(int method) {
   // Coordinate search over the schedule knobs on this pattern and these
   // values: the Cholesky panel group (panels per trailing update), the
-  // mean tile extent from which a launch uses 128 x 128 GEMM tiles, then the
-  // Cholesky panel lookahead.  Each
+  // mean tile extent from which a launch uses 128 x 128 GEMM tiles, the
+  // Cholesky panel lookahead, then the per-size-class panel group and tile
+  // threshold of every class the pattern has.  Each
   // candidate is timed as the better of two factorizations; the best
   // setting stays in place and its time is returned.
   auto time_it = [&]() {
@@ -1317,6 +1347,30 @@ double FrontSolver::autotune(int method) {
     const double t = time_it();
     if (t < best) best = t;
     else set_tuning(5, bl);
+  }
+  // then per front size class present: its panel group (Cholesky) and its
+  // tile threshold, the other classes held at their current values
+  bool present[N_FRONT_CLASSES] = {false, false, false};
+  for (const uhmk_front& F : fr_)
+    if (F.fs > 0) present[front_class(F.fs)] = true;
+  auto search = [&](int key, std::initializer_list<int> cand) {
+    const int cur = get_tuning(key);
+    int bv = cur;
+    for (int v : cand) {
+      if (v == cur) continue;
+      set_tuning(key, v);
+      const double t = time_it();
+      if (t < best) {
+        best = t;
+        bv = v;
+      }
+    }
+    set_tuning(key, bv);
+  };
+  for (int c = 0; c < N_FRONT_CLASSES; ++c) {
+    if (!present[c]) continue;
+    if (method == UHM_CHOL && c > 0) search(100 + c, {1, 2, 4, 6, 8, 12});
+    search(110 + c, {512, 2048, 8192});
   }
   use_graph_ = save_graph;
   drop_graphs();

@@ -73,6 +73,11 @@ struct Launch {
 };
 constexpr int N_LAUNCH_KINDS = 12;
 
+// Size classes of fronts for the schedule's knobs: fs <= 64 (one panel),
+// fs <= 512 (a few panels), larger.
+constexpr int N_FRONT_CLASSES = 3;
+inline int front_class(int fs) { return fs <= 64 ? 0 : (fs <= 512 ? 1 : 2); }
+
 // One launch of the level solves.
 struct SolveStep {
   // LEVEL: the Cholesky solves of a level in one launch (fwd as for CSTEP);
@@ -206,6 +211,9 @@ class FrontSolver {
     chol_group_ = g;
   }
   int chol_group() const { return chol_group_ ? chol_group_ : 6; }
+  // per size class: panel group, GEMM tile threshold, group-launch policy
+  // (0 / 0 / -1: the global knob)
+  int class_group(int c) const { return cls_group_[c] > 0 ? cls_group_[c] : chol_group(); }
   void set_profile(bool on) { profile_ = on; }
   // Graph mode: the factorization's launch sequence (and each solve's) is
   // captured once per pattern / method / precision and replayed, for
@@ -256,6 +264,9 @@ class FrontSolver {
   int small_fused_ = 0;  // Cholesky levels of at least this many fronts, all with fs <= 64, run
                          // as one whole-front launch (uhmk_chol_front_small); 0 never (default:
                          // the four level launches measured faster, see set_tuning key 9)
+  int cls_group_[N_FRONT_CLASSES] = {0, 0, 0};
+  int cls_tile_[N_FRONT_CLASSES] = {0, 0, 0};
+  int cls_gkern_[N_FRONT_CLASSES] = {-1, -1, -1};
   int lp_min_ss_ = 768;   // precision modes 2 / 3: levels whose Schur blocks average fewer rows take
                           // FP32 products instead of packed BF16 / FP16 ones
   int solve_level_ = 1;   // Cholesky solves with inverses: one launch per level and direction (1) or
