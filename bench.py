@@ -220,7 +220,7 @@ def run_reseeds(U, m, w, args, n):
 # against a copy of known size
 # (profiles/k_gemm_chol_q4_16cube_hbm_traffic.md): mixed path of the chol
 # workload only; other paths and workloads report null.
-GEMM_TRAFFIC = {("chol", "mixed"): 797.1e6}
+GEMM_TRAFFIC = {("chol", "mixed"): 655.2e6}
 
 
 def gemm_roofline(stats, peak, kernel, traffic=None):
@@ -234,6 +234,20 @@ def gemm_roofline(stats, peak, kernel, traffic=None):
 
 
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 / FP16 MFMA
+
+
+def lowprec_roofline(stats, kernel):
+    """The final Schur complement products on packed BF16 / FP16 operands
+    (k_schur_lp launches of precision modes 2 / 3) against the dense BF16 /
+    FP16 MFMA peak."""
+    ms, fl, _, n = stats["schur_lp"]
+    if not n:
+        return None
+    achieved = fl / (ms * 1e-3) / 1e12
+    return {"kernel": kernel, "bound": "mfma", "achieved": round(achieved, 4), "peak": PEAK_BF16_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16_TFLOPS, 4), "traffic": None, "launches": n,
+            "avg_launch_us": round(ms / n * 1e3, 3)}
 
 
 def assemble_roofline(stats):
@@ -357,6 +371,9 @@ def main():
                                   GEMM_TRAFFIC.get((args.workload, "mixed" if mixed_ok else "fp64"))),
         "roofline_fp64_path": gemm_roofline(r64["stats"], PEAK_FP64_TFLOPS, "k_gemm (FP64 MFMA)"),
         "roofline_extend_add": assemble_roofline(head["stats"]),
+        "roofline_schur_bf16": lowprec_roofline(rbf["stats"], "k_schur_lp (packed BF16 operands)") if rbf else None,
+        "roofline_schur_fp16": (lowprec_roofline(res["fp16"]["stats"], "k_schur_lp (packed FP16 operands)")
+                                if res["fp16"] else None),
         "mixed": _path(flops, rmx, r64, x_oracle),
         "fp64": _path(flops, r64, r64, x_oracle),
         "mixed_bf16": _path(flops, rbf, r64, x_oracle) if rbf else None,

@@ -909,6 +909,7 @@ void FrontSolver::build_schedule(int method) {
       // refinement steps
       if (lowp && (!cnt || sss / cnt < lp_min_ss_)) lowp = false;
       ls.mma = lowp ? prec_mode_ : -1;
+      if (lowp) ls.kind = Launch::SCHUR_LP;
       if (lowp)
         for (int f : big) lp_front[f] = 1;
       // the packed low-precision products run 64-wide tiles at every size
@@ -959,7 +960,7 @@ void FrontSolver::build_schedule(int method) {
     // mode 2: the Schur complement products take BF16 operands; the panel
     // (trailing) updates, which form the factor columns themselves, stay FP32
     // mma 2 / 3: packed BF16 / FP16 Schur products
-    l.mma = prec_mode_ == 0 ? 0 : (prec_mode_ >= 2 && l.kind == Launch::SCHUR && l.mma >= 2 ? prec_mode_ : 1);
+    l.mma = prec_mode_ == 0 ? 0 : (l.kind == Launch::SCHUR_LP ? prec_mode_ : 1);
     // the group launches' small in-group products in the storage type (FP64
     // products on FP64 fronts measured faster there than FP32 ones)
     if (l.kind == Launch::CHOL_GROUP) l.mma = l.prec == 0 ? 0 : 1;
@@ -1067,11 +1068,12 @@ void FrontSolver::build_schedule(int method) {
       case Launch::GEMM:
       case Launch::GEMM_DIAG:
       case Launch::SCHUR:
+      case Launch::SCHUR_LP:
         for (int q = 0; q < l.count; ++q) {
           const uhmk_item& it = items_[l.off + q];
           const int fs = fr_[it.f].fs, n = fr_[it.f].n;
           if (l.kind == Launch::GEMM_DIAG && it.c == 1) fl += flop_chol(std::min(l.nb, fs - it.a));
-          const double K = (l.kind == Launch::SCHUR) ? fs : std::min(l.c0 + l.nb, fs) - l.c0;
+          const double K = (l.kind == Launch::SCHUR || l.kind == Launch::SCHUR_LP) ? fs : std::min(l.c0 + l.nb, fs) - l.c0;
           const int i1 = tile_end(it.a, fs, n, l.tile), j1 = tile_end(it.b, fs, n, l.tile);
           double cnt = 0;
           if (chol)
@@ -1164,7 +1166,8 @@ void FrontSolver::enqueue_factor(int method, std::vector<hipEvent_t>* pev) {
         rc = uhmk_chol_gemm_diag(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, l.mma, info, st);
         break;
       case Launch::SCHUR:
-        if (l.mma >= 2)
+      case Launch::SCHUR_LP:
+        if (l.kind == Launch::SCHUR_LP)
           rc = uhmk_schur_lp(F, IT + l.off, l.count, l.extra ? 0 : 1, l.tile, l.mma == 3, A, l.prec,
                              d_lpoff_.as<long long>(), d_lpamax_.as<unsigned>(), d_lp_.as<unsigned short>(), st);
         else

@@ -53,9 +53,11 @@ struct Launch {
   // CHOL_SMALL: assembly and whole Cholesky factorization (diagonal block,
   // TRSM, Schur complement) of every front of a level with fs <= 64, a
   // workgroup per front (front list)
+  // SCHUR_LP: the Schur complement products of a level on the packed BF16 /
+  // FP16 operands of LP_PACK
   enum Kind {
     ASSEMBLE, LU_PANEL, LU_ROWS, CHOL_DIAG, CHOL_TRSM, GEMM, SCHUR, LU_FRONT, GEMM_DIAG, CHOL_GROUP, LP_PACK,
-    CHOL_SMALL
+    CHOL_SMALL, SCHUR_LP
   } kind;
   int c0 = 0, nb = 0;
   long long off = 0;  // offset into the item (or front-list) array
@@ -71,7 +73,7 @@ struct Launch {
   int wait_ev = -1;   // event the launch's stream waits for first (-1: none)
   int record_ev = -1; // event recorded on the launch's stream after it (-1: none)
 };
-constexpr int N_LAUNCH_KINDS = 12;
+constexpr int N_LAUNCH_KINDS = 13;
 
 // Size classes of fronts for the schedule's knobs: fs <= 64 (one panel),
 // fs <= 512 (a few panels), larger.
