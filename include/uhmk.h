@@ -103,6 +103,14 @@ int uhmk_copy_front(const void* src, int src_prec, void* dst, int dst_prec, long
 int uhmk_lu_panel(const uhmk_front* fronts, const int* flist, int n_fronts, int max_rows, int c0, int nb,
                   void* arena, int prec, int* piv, int* info, int* pperm, hipStream_t stream);
 
+/* The same panel (same arguments, results and pperm layout; the same bits
+ * on FP64 fronts), left-looking over register blocks with the rows left in
+ * place until the end: one barrier per column, the kb pivot rows held in
+ * LDS.  Up to 8192 pivot-candidate rows (uhmk_lu_panel: 6144); -1001
+ * above.  Replaces the same FLA_LU_piv call (LINAL/src/dense/lu_piv.cxx:79). */
+int uhmk_lu_panel_ll(const uhmk_front* fronts, const int* flist, int n_fronts, int max_rows, int c0, int nb,
+                     void* arena, int prec, int* piv, int* info, int* pperm, hipStream_t stream);
+
 /* Row swaps outside the panel (apply_pivots, LINAL/src/dense/pivot.cxx:46-91)
  * from the panel permutation in pperm, U12 = L11^-1 A12 (trsm left lower
  * unit) and L21 = A21 U11^-1 on the ABL rows (trsm right upper,

@@ -71,6 +71,8 @@ def load():
     lib.orc_set_threads.argtypes = [C.c_int]
     lib.orc_dense_lu.restype = C.c_int
     lib.orc_dense_lu.argtypes = [_D, C.c_int, C.c_int, C.c_int, _I]
+    lib.orc_panel_lu.restype = C.c_int
+    lib.orc_panel_lu.argtypes = [_D, C.c_int, C.c_int, C.c_int, _I, _D]
     lib.orc_dense_chol.restype = C.c_int
     lib.orc_dense_chol.argtypes = [_D, C.c_int, C.c_int, C.c_int]
     _lib = lib
@@ -100,6 +102,17 @@ def dense_lu(F, fs, b=256):
     p = np.zeros(max(1, fs), dtype=np.int32)
     info = load().orc_dense_lu(_dp(A), n, fs, b, _ip(p))
     return A, p[:fs], info
+
+
+def panel_lu(A, kb):
+    """Unblocked GEPP panel (getf2) of a copy of the m x kb block A;
+    returns (A, relative pivots, pivot margins, info)."""
+    A = np.array(A, dtype=np.float64, order="F", copy=True)
+    m = A.shape[0]
+    p = np.zeros(max(1, kb), dtype=np.int32)
+    marg = np.zeros(max(1, kb), dtype=np.float64)
+    info = load().orc_panel_lu(_dp(A), m, m, kb, _ip(p), _dp(marg))
+    return A, p[:kb], marg[:kb], info
 
 
 def dense_chol(F, fs, b=256):

@@ -987,4 +987,9 @@ int orc_max_threads(void) {
 
 /* Dense single-front entry points for the golden-vector tests. */
 int orc_dense_lu(double* F, int n, int fs, int b, int* p) { return front_lu(F, n, fs, b, p, NULL, 1); }
+/* One unblocked panel (getf2_panel with k0 = 0) of an m x kb column-major
+ * block with leading dimension ld: relative pivots and their margins. */
+int orc_panel_lu(double* A, int ld, int m, int kb, int* p, double* marg) {
+  return getf2_panel(A, ld, 0, kb, m, p, marg, 1);
+}
 int orc_dense_chol(double* F, int n, int fs, int b) { return front_chol(F, n, fs, b, 1); }

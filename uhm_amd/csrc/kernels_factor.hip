@@ -16,6 +16,8 @@
 //   trsm / gemm / syrk  LINAL/src/dense/{trsm,gemm,syrk}.cxx
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "assemble_device.hpp"
 #include "dense_device.hpp"
 #include "uhmk.h"
@@ -638,6 +640,564 @@ __global__ __launch_bounds__(PNT) void k_lu_panel_reg(const uhmk_front* __restri
   panel_permutation<PNT>(s_piv, kb, m, c0, pperm + (long long)f * PP_STRIDE, tid);
 }
 
+// ------------------------------------------------------------------------
+// Panel LU kernels with the rows left in place (k_lu_panel_rr for panels of
+// at most 512 rows, k_lu_panel_ll above).  One workgroup per front.  Slot
+// s = q * PNT + tid is panel row s (front row c0 + s) and belongs to thread
+// tid.  Rows never move while the panel is factored: each thread keeps the
+// current position of its slots (the swap g <-> p of step g acts on
+// positions) and the step at which a slot became a pivot, and the rows are
+// stored at their final positions once, at the end.  The columns run in
+// blocks of W; a column step is a wavefront argmax (DPP), one barrier that
+// exchanges the wavefronts' candidates (double-buffered in LDS), and the
+// rank-1 update of the block's later columns in registers.  The kb pivot
+// rows live in LDS in pivot order (s_pr, all kb columns); after a block,
+// its pivot rows receive the updates of the block's own pivots for the
+// later columns (U strip).  Every entry sees the rank-1 update of each
+// earlier pivot as one FMA, in pivot order: the operation sequence of the
+// unblocked panel (getf2; FLA_LU_piv on the panel,
+// LINAL/src/dense/lu_piv.cxx:79) and of k_lu_panel, so FP64 fronts get the
+// same bits from all three kernels.  Pivot rule: first maximal |a| in
+// current row order; a column without a comparable entry (all NaN) keeps
+// the current row.
+// ------------------------------------------------------------------------
+template <int N, typename Fn, int... I>
+__device__ __forceinline__ void static_for_impl(Fn&& fn, std::integer_sequence<int, I...>) {
+  (fn(std::integral_constant<int, I>{}), ...);
+}
+// fn(integral_constant<int, i>) for i = 0 .. N-1, every index a compile-time constant
+template <int N, typename Fn>
+__device__ __forceinline__ void static_for(Fn&& fn) {
+  static_for_impl<N>(fn, std::make_integer_sequence<int, N>{});
+}
+
+constexpr int PL_BIG = 0x3fffffff;  // step of a slot that is not (yet) a pivot
+constexpr int PL_LDR = TILE + 1;    // row stride of the pivot rows in LDS
+
+// Workgroup barrier for LDS traffic only: LDS writes complete before it,
+// global loads in flight stay in flight (__syncthreads would wait for them).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// x combined with the value DPP control CTRL moves into each lane (lanes
+// outside ROW_MASK keep their own value)
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ double dpp_max_f64(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(x), __double2loint(x), CTRL, ROW_MASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(x), __double2hiint(x), CTRL, ROW_MASK, 0xF, false);
+  return fmax(x, __hiloint2double(hi, lo));
+}
+
+// Largest value over the wavefront (no NaN among the inputs): quad swaps,
+// half-row and row mirrors, then the row broadcasts 15 and 31 carry the
+// maximum to lane 63.
+__device__ __forceinline__ double wave_max_f64(double x) {
+  x = dpp_max_f64<0xB1>(x);        // quad_perm [1, 0, 3, 2]
+  x = dpp_max_f64<0x4E>(x);        // quad_perm [2, 3, 0, 1]
+  x = dpp_max_f64<0x141>(x);       // row_half_mirror
+  x = dpp_max_f64<0x140>(x);       // row_mirror
+  x = dpp_max_f64<0x142, 0xA>(x);  // row_bcast:15 into rows 1, 3
+  x = dpp_max_f64<0x143, 0xC>(x);  // row_bcast:31 into rows 2, 3
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), 63),
+                          __builtin_amdgcn_readlane(__double2loint(x), 63));
+}
+
+// Wavefront argmax of (value, position, slot): the largest value, the
+// smallest position among equal values.  cv is never NaN (candidates start
+// at -1 and only take comparable values).
+__device__ __forceinline__ void wave_argmax(double& cv, int& cp, int& cs) {
+  const double mx = wave_max_f64(cv);
+  const unsigned long long tie = __ballot(cv == mx);
+  if ((tie & (tie - 1ull)) == 0ull) {
+    const int l = __ffsll((unsigned long long)tie) - 1;
+    cp = __builtin_amdgcn_readlane(cp, l);
+    cs = __builtin_amdgcn_readlane(cs, l);
+  } else {
+    int p = (cv == mx) ? cp : 0x7fffffff;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) p = min(p, __shfl_xor(p, o));
+    const unsigned long long w = __ballot(cv == mx && cp == p);
+    const int l = __ffsll((unsigned long long)w) - 1;
+    cs = __builtin_amdgcn_readlane(cs, l);
+    cp = p;
+  }
+  cv = mx;
+}
+
+struct alignas(16) PanelCand {
+  double v;  // largest |a| of the wavefront's rows (-1: none)
+  int p, s;  // its position and slot
+};
+
+template <int W, int NW>
+struct PanelLds {
+  double pr[TILE * PL_LDR];  // pivot rows in pivot order
+  PanelCand cand[2][NW];
+  double crow[2][NW + 1][W];
+  int piv[TILE], ps[TILE];  // relative pivot and pivot slot of each step
+  int dsl[TILE], dpos[TILE], tgt[TILE];
+  int nd, ncnt, keep;
+};
+
+// Column step K of the block starting at column j0 (step g = j0 + K): the
+// pivot among the slots not yet pivoted, the bookkeeping, and the rank-1
+// update of the block's later columns of every slot not yet pivoted.
+// v[q][j] is column j0 + j of slot q * PNT + tid; pst[q] is the step at
+// which that slot became a pivot (PL_BIG: not yet, -1: no row).  Only
+// slots below kb <= 64 (q = 0 of the first wavefront) can ever be
+// displaced, so a slot's position is its index except for q = 0, whose
+// position is pos0.  Returns the pivot slot; pr[j] (j >= K) receives the
+// pivot row's block values, and so does L.pr (all W of them).
+template <int K, int R, int W, int PNT>
+__device__ __forceinline__ int panel_step(double (&v)[R][W], int& pos0, int (&pst)[R], double (&pr)[W], int j0,
+                                          PanelLds<W, PNT / 64>& L, int tid, int& zero) {
+  constexpr int NW = PNT / 64;
+  const int g = j0 + K, buf = g & 1, lane = tid & 63, wid = tid >> 6;
+  // this thread's candidate: first maximal |a| in position order (slots
+  // q >= 1 sit at their index, which grows with q)
+  double cv = -1.0;
+  int cs = -1;
+#pragma unroll
+  for (int q = 1; q < R; ++q) {
+    const double a = fabs(v[q][K]);
+    if (pst[q] == PL_BIG && a > cv) {
+      cv = a;
+      cs = q * PNT + tid;
+    }
+  }
+  int cp = cs >= 0 ? cs : 0x7fffffff;
+  {
+    const double a = fabs(v[0][K]);
+    if (pst[0] == PL_BIG && (a > cv || (a == cv && pos0 < cp))) {
+      cv = a;
+      cp = pos0;
+      cs = tid;
+    }
+  }
+  wave_argmax(cv, cp, cs);
+  if (lane == 0) L.cand[buf][wid] = PanelCand{cv, cp, cs};
+  // the wavefront's best row, published by its owner (a uniform register
+  // index here became a dynamic index into scratch memory)
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+    if (q * PNT + tid == cs)
+#pragma unroll
+      for (int j = 0; j < W; ++j) L.crow[buf][wid][j] = v[q][j];
+  lds_barrier();
+  // the best of the NW candidates: lane w < NW holds candidate w
+  int bs, bp, bw;
+  {
+    const PanelCand c = (lane < NW) ? L.cand[buf][lane < NW ? lane : 0] : PanelCand{-1.0, 0x7fffffff, -1};
+    double mv = c.v;
+    if constexpr (NW > 1) mv = dpp_max_f64<0xB1>(mv);   // quad_perm [1, 0, 3, 2]
+    if constexpr (NW > 2) mv = dpp_max_f64<0x4E>(mv);   // quad_perm [2, 3, 0, 1]
+    if constexpr (NW > 4) mv = dpp_max_f64<0x141>(mv);  // row_half_mirror
+    mv = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(mv), 0),
+                          __builtin_amdgcn_readlane(__double2loint(mv), 0));
+    const bool top = lane < NW && c.v == mv;
+    const unsigned long long tie = __ballot(top);
+    if ((tie & (tie - 1ull)) == 0ull) {
+      bw = __ffsll((unsigned long long)tie) - 1;
+    } else {
+      int pm = top ? c.p : 0x7fffffff;
+#pragma unroll
+      for (int o = 4; o > 0; o >>= 1) pm = min(pm, __shfl_xor(pm, o));
+      pm = __builtin_amdgcn_readlane(pm, 0);
+      bw = __ffsll((unsigned long long)__ballot(top && c.p == pm)) - 1;
+    }
+    bp = __builtin_amdgcn_readlane(c.p, bw);
+    bs = __builtin_amdgcn_readlane(c.s, bw);
+  }
+  if (bs < 0) {
+    // no comparable entry in the column: the row at position g stays, and
+    // its owner publishes it
+    if (tid < 64 && pst[0] == PL_BIG && pos0 == g) {
+      L.keep = tid;
+#pragma unroll
+      for (int j = 0; j < W; ++j) L.crow[buf][NW][j] = v[0][j];
+    }
+    lds_barrier();
+    bs = L.keep;
+    bp = g;
+    bw = NW;
+  }
+#pragma unroll
+  for (int j = K; j < W; ++j) pr[j] = L.crow[buf][bw][j];
+  const double pv = pr[K];
+  if (!(pv != 0.0) && zero == 0) zero = g + 1;
+  // the pivot row's block values (L part left of K, U part from K on)
+  if (tid < W) L.pr[g * PL_LDR + j0 + tid] = L.crow[buf][bw][tid];
+  if (tid == 0) {
+    L.piv[g] = bp - g;
+    L.ps[g] = bs;
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+    if (q * PNT + tid == bs) pst[q] = g;
+  if (tid < 64 && pst[0] == PL_BIG && pos0 == g) pos0 = bp;  // displaced by the pivot row
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    if (pst[q] == PL_BIG) {
+      double l = v[q][K];
+      if (pv != 0.0) l = l / pv;
+      v[q][K] = l;
+#pragma unroll
+      for (int j = K + 1; j < W; ++j) v[q][j] = fma(-l, pr[j], v[q][j]);
+    }
+  }
+  return bs;
+}
+
+// U strip of the block's pivot rows [j0, j0 + nbr) for the columns
+// [j1, kb): the rows in L.pr hold the updates of every pivot before j0;
+// the block's own pivots are applied here, in order, one thread per column.
+template <int W, int PNT>
+__device__ __forceinline__ void panel_ustrip(PanelLds<W, PNT / 64>& L, int j0, int nbr, int j1, int kb, int tid) {
+  for (int c = j1 + tid; c < kb; c += PNT) {
+    double u[W];
+#pragma unroll
+    for (int r = 0; r < W; ++r) u[r] = (r < nbr) ? L.pr[(j0 + r) * PL_LDR + c] : 0.0;
+#pragma unroll
+    for (int r = 1; r < W; ++r) {
+      // the block's L row r, read just before its use (hoisting every row's
+      // reads at once costs more registers than the kernel has)
+      __builtin_amdgcn_sched_barrier(0);
+      const double* __restrict__ lr = L.pr + (j0 + r) * PL_LDR + j0;
+#pragma unroll
+      for (int i = 0; i < r; ++i) u[r] = fma(-lr[i], u[i], u[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < W; ++r)
+      if (r < nbr) L.pr[(j0 + r) * PL_LDR + c] = u[r];
+  }
+}
+
+// Relative pivots, the first zero pivot and the net row permutation for the
+// row kernel (the layout of panel_permutation): the rows whose content
+// changed (positions [0, kb), then the distinct swap targets beyond kb in
+// order of first use) and, per listed row, the list index of the row whose
+// old content lands there.  The final slot at position g < kb is the
+// pivot of step g (L.ps); L.dsl / L.dpos hold the displaced slots (not
+// pivots, moved) and their final positions.
+template <int W, int PNT>
+__device__ __forceinline__ void panel_finish(PanelLds<W, PNT / 64>& L, const uhmk_front& fr, int f, int c0, int kb,
+                                             int zero, int* __restrict__ piv, int* __restrict__ info,
+                                             int* __restrict__ pperm, int tid) {
+  for (int k = tid; k < kb; k += PNT) piv[fr.poff + c0 + k] = L.piv[k];
+  if (tid == 0 && zero && info[f] == 0) info[f] = c0 + zero;
+  if (tid < 64) {
+    const int k = tid;
+    const int t = (k < kb) ? k + L.piv[k] : -1;
+    bool first = t >= kb;
+#pragma unroll
+    for (int k2 = 0; k2 < 63; ++k2) first = first && !(k2 < k && __builtin_amdgcn_readlane(t, k2) == t);
+    const unsigned long long mask = __ballot(first);
+    if (first) L.tgt[__popcll(mask & ((1ull << k) - 1ull))] = t;
+    if (k == 0) L.ncnt = kb + __popcll(mask);
+  }
+  __syncthreads();
+  const int cnt = L.ncnt, nt = cnt - kb, nd = min(L.nd, TILE);
+  int* __restrict__ pp = pperm + (long long)f * PP_STRIDE;
+  for (int k = tid; k < cnt; k += PNT) {
+    const int lr = (k < kb) ? k : L.tgt[k - kb];
+    int src = lr;
+    if (lr < kb) {
+      src = L.ps[lr];
+    } else {
+      for (int d = 0; d < nd; ++d)
+        if (L.dpos[d] == lr) src = L.dsl[d];
+    }
+    int si = (src < kb) ? src : -1;
+    for (int u = 0; u < nt; ++u)
+      if (si < 0 && L.tgt[u] == src) si = kb + u;
+    pp[1 + k] = c0 + lr;
+    pp[1 + 2 * TILE + k] = si;
+  }
+  if (tid == 0) pp[0] = cnt;
+}
+
+// ---- panels of at most PNT rows: every row's kb columns in registers.
+// Right-looking across the blocks: at the end of a block the pivot rows
+// publish their later columns, the U strip applies the block's own pivots
+// to them, and every slot not yet pivoted applies the block's rank-W update
+// to its later columns (L in its registers, U from LDS).  No memory traffic
+// between the first load and the final stores.
+template <typename T, int PNT>
+__global__ __launch_bounds__(PNT) void k_lu_panel_rr(const uhmk_front* __restrict__ F,
+                                                     const int* __restrict__ flist, int c0, int nb,
+                                                     T* __restrict__ arena, int* __restrict__ piv,
+                                                     int* __restrict__ info, int* __restrict__ pperm) {
+  constexpr int W = 16, NB = TILE / W, NW = PNT / 64;
+  __shared__ PanelLds<W, NW> L;
+  const int f = flist[blockIdx.x];
+  const uhmk_front fr = F[f];
+  const int n = fr.n, fs = fr.fs;
+  const int m = fs - c0, kb = min(nb, m);
+  T* __restrict__ P = arena + fr.off + c0 + (long long)c0 * n;  // panel origin (c0, c0)
+  const int tid = threadIdx.x;
+  int pos0 = tid, pst[1] = {tid < m ? PL_BIG : -1};
+  if (tid == 0) L.nd = 0;
+  double a[TILE];
+#pragma unroll
+  for (int j = 0; j < TILE; ++j) a[j] = uhmd::ldz(P, tid + (long long)j * n, tid < m && j < kb);
+  int zero = 0;
+  static_for<NB>([&](auto bc) {
+    constexpr int B = decltype(bc)::value, J0 = B * W, J1 = J0 + W;
+    if (J0 >= kb) return;
+    double v[1][W], pr[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) v[0][j] = a[J0 + j];
+    const int kend = min(W, kb - J0);
+    static_for<W>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      if (K >= kend) return;
+      panel_step<K, 1, W, PNT>(v, pos0, pst, pr, J0, L, tid, zero);
+    });
+#pragma unroll
+    for (int j = 0; j < W; ++j) a[J0 + j] = v[0][j];
+    if (J1 < kb) {
+      // the block's pivot rows publish their later columns
+      if (pst[0] >= J0 && pst[0] < J1)
+#pragma unroll
+        for (int c = J1; c < TILE; ++c) L.pr[pst[0] * PL_LDR + c] = a[c];
+      lds_barrier();
+      panel_ustrip<W, PNT>(L, J0, W, J1, kb, tid);
+      lds_barrier();
+      if (pst[0] == PL_BIG) {
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+          const double li = a[J0 + i];
+          const double* __restrict__ ur = L.pr + (J0 + i) * PL_LDR;
+#pragma unroll
+          for (int c = J1; c < TILE; ++c) a[c] = fma(-li, ur[c], a[c]);
+        }
+      }
+    }
+  });
+  __syncthreads();
+  // final placement: pivot rows (registers for the columns up to the end of
+  // their block, LDS beyond) at their step, the other rows at their position
+  if (pst[0] >= 0) {
+    const int g = pst[0] == PL_BIG ? pos0 : pst[0];
+    const int j1 = pst[0] == PL_BIG ? TILE : (pst[0] / W + 1) * W;
+#pragma unroll
+    for (int c = 0; c < TILE; ++c)
+      if (c < kb) P[g + (long long)c * n] = (T)(c < j1 ? a[c] : L.pr[g * PL_LDR + c]);
+    if (pst[0] == PL_BIG && pos0 != tid) {
+      const int d = atomicAdd(&L.nd, 1);
+      if (d < TILE) {
+        L.dsl[d] = tid;
+        L.dpos[d] = pos0;
+      }
+    }
+  }
+  __syncthreads();
+  panel_finish<W, PNT>(L, fr, f, c0, kb, zero, piv, info, pperm, tid);
+}
+
+// v -= L(slot, [i0, i0 + LC)) U([i0, i0 + LC), block) for every slot, one
+// U row at a time (reading every row's values ahead of the FMAs costs more
+// registers than the kernels have); rows of pivoted slots hold zeros in l.
+template <int R, int W, int LC, int PNT>
+__device__ __forceinline__ void panel_deferred(double (&v)[R][W], const double (&l)[R][LC],
+                                               const PanelLds<W, PNT / 64>& L, int i0, int j0) {
+#pragma unroll
+  for (int u = 0; u < LC; ++u) {
+    __builtin_amdgcn_sched_barrier(0);
+    const double* __restrict__ ur = L.pr + (i0 + u) * PL_LDR + j0;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      const double uu = ur[j];
+#pragma unroll
+      for (int q = 0; q < R; ++q) v[q][j] = fma(-l[q][u], uu, v[q][j]);
+    }
+  }
+}
+
+// ---- taller panels: left-looking over register blocks of W columns (R x W
+// per thread).  A block first receives the updates of every earlier pivot
+// in pivot order (L of the slot from memory, where the slot's earlier
+// blocks were stored, LC columns at a time; U rows from LDS), then its
+// column steps run.  The block's pivot rows are
+// fetched for the other columns while the later steps run (their L part
+// and their raw values right of the block), then receive the updates of
+// the earlier pivots (U strip, two phases).
+template <int R, int W, int LC, bool PF, typename T, int PNT>
+__global__ __launch_bounds__(PNT) void k_lu_panel_ll(const uhmk_front* __restrict__ F,
+                                                     const int* __restrict__ flist, int c0, int nb,
+                                                     T* __restrict__ arena, int* __restrict__ piv,
+                                                     int* __restrict__ info, int* __restrict__ pperm) {
+  constexpr int NW = PNT / 64;
+  __shared__ PanelLds<W, NW> L;
+  __shared__ double s_stage[TILE * TILE];  // displaced rows for the final placement
+
+  const int f = flist[blockIdx.x];
+  const uhmk_front fr = F[f];
+  const int n = fr.n, fs = fr.fs;
+  const int m = fs - c0, kb = min(nb, m);
+  T* __restrict__ P = arena + fr.off + c0 + (long long)c0 * n;  // panel origin (c0, c0)
+  const int tid = threadIdx.x;
+
+  int pos0 = tid, pst[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) pst[q] = q * PNT + tid < m ? PL_BIG : -1;
+  if (tid == 0) L.nd = 0;
+  int zero = 0;
+  double v[R][W];
+  for (int j0 = 0; j0 < kb; j0 += W) {
+    const int j1 = min(j0 + W, kb);
+    const int nbr = j1 - j0, wl = j0, wr = kb - j1;
+    // raw values of the block for the slots not yet pivoted
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int s = q * PNT + tid;
+      const bool ok = pst[q] == PL_BIG;
+#pragma unroll
+      for (int j = 0; j < W; ++j) v[q][j] = uhmd::ldz(P, s + (long long)(j0 + j) * n, ok && j0 + j < kb);
+    }
+    // updates of the pivots [0, j0), in pivot order, LC L columns at a
+    // time; with PF the next LC columns are loaded ahead of the FMAs (the
+    // taller shapes have no registers left for that)
+    if constexpr (PF) {
+      if (j0 > 0) {
+        double l[R][LC], ln[R][LC];
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+#pragma unroll
+          for (int u = 0; u < LC; ++u) l[q][u] = uhmd::ldz(P, q * PNT + tid + (long long)u * n, pst[q] == PL_BIG);
+        for (int i0 = 0; i0 < j0; i0 += LC) {
+          const bool more = i0 + LC < j0;
+#pragma unroll
+          for (int q = 0; q < R; ++q)
+#pragma unroll
+            for (int u = 0; u < LC; ++u)
+              ln[q][u] = uhmd::ldz(P, q * PNT + tid + (long long)(i0 + LC + u) * n, more && pst[q] == PL_BIG);
+          panel_deferred<R, W, LC, PNT>(v, l, L, i0, j0);
+#pragma unroll
+          for (int q = 0; q < R; ++q)
+#pragma unroll
+            for (int u = 0; u < LC; ++u) l[q][u] = ln[q][u];
+        }
+      }
+    } else {
+      for (int i0 = 0; i0 < j0; i0 += LC) {
+        double l[R][LC];
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+#pragma unroll
+          for (int u = 0; u < LC; ++u)
+            l[q][u] = uhmd::ldz(P, q * PNT + tid + (long long)(i0 + u) * n, pst[q] == PL_BIG);
+        panel_deferred<R, W, LC, PNT>(v, l, L, i0, j0);
+      }
+    }
+    // the block's column steps; after step K, thread t < wl + wr starts the
+    // load of the pivot row's column col(t) (its L part left of the block,
+    // its raw values right of it)
+    const int kend = min(W, kb - j0);
+    const int myc = tid < wl ? tid : j1 + (tid - wl);
+    const bool fetch = tid < wl + wr;
+    double pa[W], pr[W];
+    static_for<W>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      if (K >= kend) return;
+      const int bs = panel_step<K, R, W, PNT>(v, pos0, pst, pr, j0, L, tid, zero);
+      pa[K] = fetch ? (double)P[bs + (long long)myc * n] : 0.0;
+    });
+    // store the block for the slots not pivoted before it
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int s = q * PNT + tid;
+      if (pst[q] >= j0)
+#pragma unroll
+        for (int j = 0; j < W; ++j)
+          if (j0 + j < kb) P[s + (long long)(j0 + j) * n] = (T)v[q][j];
+    }
+    if (fetch)
+#pragma unroll
+      for (int k = 0; k < W; ++k)
+        if (k < nbr) L.pr[(j0 + k) * PL_LDR + myc] = pa[k];
+    // the block's stores reach memory before any thread reads them back
+    // (the block's pivot rows, the deferred updates of the next blocks)
+    __syncthreads();
+    if (wr > 0) {
+      // U strip, phase 1: the pivots before the block, one thread per entry
+      if (j0 > 0) {
+        for (int idx = tid; idx < nbr * wr; idx += PNT) {
+          const int g = j0 + idx % nbr, c = j1 + idx / nbr;
+          double acc = L.pr[g * PL_LDR + c];
+          const double* __restrict__ lr = L.pr + g * PL_LDR;
+#pragma unroll 8
+          for (int i = 0; i < j0; ++i) acc = fma(-lr[i], L.pr[i * PL_LDR + c], acc);
+          L.pr[g * PL_LDR + c] = acc;
+        }
+        lds_barrier();
+      }
+      // phase 2: the block's own pivots
+      panel_ustrip<W, PNT>(L, j0, nbr, j1, kb, tid);
+      lds_barrier();
+    }
+  }
+  __syncthreads();
+  // final placement: pivot rows to positions [0, kb), displaced rows (not
+  // pivots, position changed) staged through LDS to their positions
+  if (pst[0] == PL_BIG && pos0 != tid) {
+    // at most kb: only slots that held a position in [0, kb) get displaced
+    const int d = atomicAdd(&L.nd, 1);
+    if (d < TILE) {
+      L.dsl[d] = tid;
+      L.dpos[d] = pos0;
+    }
+  }
+  __syncthreads();
+  const int nd = min(L.nd, TILE);
+  for (int idx = tid; idx < nd * kb; idx += PNT) {
+    const int d = idx % nd, c = idx / nd;
+    s_stage[d * TILE + c] = (double)P[L.dsl[d] + (long long)c * n];
+  }
+  __syncthreads();
+  for (int idx = tid; idx < kb * kb; idx += PNT) {
+    const int g = idx % kb, c = idx / kb;
+    P[g + (long long)c * n] = (T)L.pr[g * PL_LDR + c];
+  }
+  for (int idx = tid; idx < nd * kb; idx += PNT) {
+    const int d = idx % nd, c = idx / nd;
+    P[L.dpos[d] + (long long)c * n] = (T)s_stage[d * TILE + c];
+  }
+  panel_finish<W, PNT>(L, fr, f, c0, kb, zero, piv, info, pperm, tid);
+}
+
+constexpr int PL_MAXROWS = 8192;  // pivot-candidate rows of the left-looking panel
+
+template <typename T>
+int launch_panel_ll(const uhmk_front* fronts, const int* flist, int n_fronts, int max_rows, int c0, int nb,
+                    T* arena, int* piv, int* info, int* pperm, hipStream_t stream) {
+  const dim3 g(n_fronts);
+#define UHM_PANEL_RR(PNT)                                                                                        \
+  hipLaunchKernelGGL((k_lu_panel_rr<T, PNT>), g, dim3(PNT), 0, stream, fronts, flist, c0, nb, arena, piv, info, \
+                     pperm)
+#define UHM_PANEL_LL(R, W, LC, PF, PNT)                                                                \
+  hipLaunchKernelGGL((k_lu_panel_ll<R, W, LC, PF, T, PNT>), g, dim3(PNT), 0, stream, fronts, flist, c0, nb, arena, \
+                     piv, info, pperm)
+  if (max_rows <= 64) UHM_PANEL_RR(64);
+  else if (max_rows <= 128) UHM_PANEL_RR(128);
+  else if (max_rows <= 256) UHM_PANEL_RR(256);
+  else if (max_rows <= 512) UHM_PANEL_RR(512);
+  else if (max_rows <= 1024) UHM_PANEL_LL(2, 8, 8, true, 512);
+  else if (max_rows <= 1536) UHM_PANEL_LL(3, 8, 8, true, 512);
+  else if (max_rows <= 2048) UHM_PANEL_LL(4, 8, 4, true, 512);
+  else if (max_rows <= 3072) UHM_PANEL_LL(6, 8, 4, false, 512);
+  else if (max_rows <= 4096) UHM_PANEL_LL(8, 8, 2, false, 512);
+  else if (max_rows <= 6144) UHM_PANEL_LL(12, 4, 2, false, 512);
+  else if (max_rows <= PL_MAXROWS) UHM_PANEL_LL(16, 2, 1, false, 512);
+  else return -1001;  // more pivot-candidate rows than the panel kernels hold
+#undef UHM_PANEL_LL
+#undef UHM_PANEL_RR
+  return ret(hipGetLastError());
+}
+
 template <typename T>
 int launch_panel(const uhmk_front* fronts, const int* flist, int n_fronts, int max_rows, int c0, int nb, T* arena,
                  int* piv, int* info, int* pperm, hipStream_t stream) {
@@ -835,6 +1395,19 @@ int uhmk_lu_panel(const uhmk_front* fronts, const int* flist, int n_fronts, int 
   if (prec == 1)
     return launch_panel<float>(fronts, flist, n_fronts, max_rows, c0, nb, static_cast<float*>(arena), piv, info,
                                pperm, stream);
+  return -1006;
+}
+
+int uhmk_lu_panel_ll(const uhmk_front* fronts, const int* flist, int n_fronts, int max_rows, int c0, int nb,
+                     void* arena, int prec, int* piv, int* info, int* pperm, hipStream_t stream) {
+  if (n_fronts <= 0) return 0;
+  if (nb > TILE || nb < 1) return -1000;
+  if (prec == 0)
+    return launch_panel_ll<double>(fronts, flist, n_fronts, max_rows, c0, nb, static_cast<double*>(arena), piv, info,
+                                   pperm, stream);
+  if (prec == 1)
+    return launch_panel_ll<float>(fronts, flist, n_fronts, max_rows, c0, nb, static_cast<float*>(arena), piv, info,
+                                  pperm, stream);
   return -1006;
 }
 

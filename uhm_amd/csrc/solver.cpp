@@ -1144,8 +1144,8 @@ void FrontSolver::enqueue_factor(int method, std::vector<hipEvent_t>* pev) {
                            d_arena_.as<double>(), d_arena32_.as<float>(), l.prec, d_stage_.as<double>(), st);
         break;
       case Launch::LU_PANEL:
-        rc = uhmk_lu_panel(F, FL + l.off, l.count, l.extra, l.c0, l.nb, A, l.prec, P, info, d_pperm_.as<int>(),
-                           st);
+        rc = (lu_panel_ll_ ? uhmk_lu_panel_ll : uhmk_lu_panel)(F, FL + l.off, l.count, l.extra, l.c0, l.nb, A,
+                                                               l.prec, P, info, d_pperm_.as<int>(), st);
         break;
       case Launch::LU_ROWS:
         rc = uhmk_lu_rows(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, d_pperm_.as<int>(), st);
@@ -1257,6 +1257,10 @@ void FrontSolver::set_tuning(int key, int value) {
     if (value < -1 || value > 2) throw std::runtime_error("set_tuning: class group launches take -1 (global), 0, 1 or 2");
     if (value != cls_gkern_[key - 120]) schedule_method_ = 0;
     cls_gkern_[key - 120] = value;
+  } else if (key == 11) {
+    if (value != 0 && value != 1) throw std::runtime_error("set_tuning: key 11 (LU panel kernel) is 0 or 1");
+    if (value != lu_panel_ll_) drop_graphs();  // captured launches name the kernel
+    lu_panel_ll_ = value;
   } else if (key == 10) {
     if (value < 0) throw std::runtime_error("set_tuning: key 10 (low-precision Schur rows) takes a row count");
     if (value != lp_min_ss_) schedule_method_ = 0;
@@ -1279,7 +1283,7 @@ void FrontSolver::set_tuning(int key, int value) {
                              "6: Cholesky diagonal blocks fused into the panel updates, 7: Cholesky panel groups "
                              "in one persistent launch, 8: Cholesky level solves in one launch per level, "
                              "9: fused small-front Cholesky levels, 10: smallest mean Schur block of a level "
-                             "with low-precision products, 100 + c / 110 + c / 120 + c: panel group / tile "
+                             "with low-precision products, 11: LU panel kernel (1 left-looking, 0 right-looking), 100 + c / 110 + c / 120 + c: panel group / tile "
                              "threshold / group launches of front size class c)");
   }
 }
@@ -1295,6 +1299,7 @@ int FrontSolver::get_tuning(int key) const {
   if (key == 8) return solve_level_;
   if (key == 9) return small_fused_;
   if (key == 10) return lp_min_ss_;
+  if (key == 11) return lu_panel_ll_;
   if (key >= 100 && key < 100 + N_FRONT_CLASSES) return cls_group_[key - 100];
   if (key >= 110 && key < 110 + N_FRONT_CLASSES) return cls_tile_[key - 110];
   if (key >= 120 && key < 120 + N_FRONT_CLASSES) return cls_gkern_[key - 120];

@@ -271,6 +271,8 @@ class FrontSolver {
   int cls_gkern_[N_FRONT_CLASSES] = {-1, -1, -1};
   int lp_min_ss_ = 768;   // precision modes 2 / 3: levels whose Schur blocks average fewer rows take
                           // FP32 products instead of packed BF16 / FP16 ones
+  int lu_panel_ll_ = 1;   // LU panels: left-looking register-block kernel (1) or the right-looking
+                          // one (0); the same bits on FP64 fronts
   int solve_level_ = 1;   // Cholesky solves with inverses: one launch per level and direction (1) or
                           // one per 64-row block step (0)
   int group_kernel_ = 2;  // Cholesky: panel groups in one persistent launch (uhmk_chol_group): 0 never,
