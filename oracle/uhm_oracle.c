@@ -20,8 +20,9 @@
  *   flops      LAWN-41                 LINAL/src/util.cxx:56-101, UHM/src/mesh/element.cxx:128-195
  *
  * libflame's FLA_LU_piv / FLA_Chol (not in this container) are restated as
- * the textbook unblocked GEPP (first maximal |a|, LAPACK i?amax) and
- * unblocked Cholesky on each b-wide panel.  The right-hand side is merged
+ * the unblocked GEPP of LAPACK dgetf2 (first maximal |a|, LAPACK i?amax;
+ * multipliers by the reciprocal of a normal pivot) and unblocked Cholesky
+ * on each b-wide panel.  The right-hand side is merged
  * once at solve time (x_f = b_f + sum of the children's xb); the reference
  * merges the b part during factorization and the update during the solve,
  * which is the same sum in another order.
@@ -39,6 +40,7 @@
  * Not pinnable here: libflame's blocked-panel rounding (the reference's
  * tests store no values); the restatement follows LAPACK's unblocked GEPP.
  */
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -660,8 +662,13 @@ static int getf2_panel(double* A, int ld, int k0, int kb, int m, int* p, double*
     p[j] = pr - j;
     if (marg) marg[j] = best > 0.0 ? (best - second) / best : 0.0;
     swap_rows(A, ld, j, pr, k0, k0 + kb);
+    /* multipliers as LAPACK dgetf2 forms them: times the reciprocal of a
+     * normal pivot, by division for a subnormal one */
     const double d = AT(A, ld, j, j);
-    if (d != 0.0) {
+    if (fabs(d) >= DBL_MIN) {
+      const double r = 1.0 / d;
+      for (int i = j + 1; i < m; ++i) AT(A, ld, i, j) *= r;
+    } else if (d != 0.0) {
       for (int i = j + 1; i < m; ++i) AT(A, ld, i, j) /= d;
     } else if (!info) {
       info = j + 1;

@@ -102,6 +102,15 @@ __global__ __launch_bounds__(256) void k_copy_cast(const TS* __restrict__ src, T
 // ------------------------------------------------------------------------
 constexpr int PIB = 8;
 
+// Multiplier a / pv of a row below the pivot pv, formed as LAPACK dgetf2
+// forms it: a times the reciprocal rinv = 1 / pv when |pv| is at least the
+// smallest normal number, by division when pv is subnormal, and a itself
+// when pv is zero (the zero pivot is reported instead).
+__device__ __forceinline__ double lu_mult(double a, double pv, double rinv) {
+  if (fabs(pv) >= 2.2250738585072014e-308) return a * rinv;
+  return pv != 0.0 ? a / pv : a;
+}
+
 constexpr int PP_STRIDE = 1 + 4 * TILE;  // ints per front in the panel-permutation buffer
 constexpr int PP_MAXROWS = 6144;
 
@@ -267,7 +276,7 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__
               for (int jj = 0; jj < PIB; ++jj) v[q][jj] = prow[jj];
           }
         }
-        const double pv = prow[j];
+        const double pv = prow[j], rinv = 1.0 / pv;
         if (tid == 0) {
           s_piv[ib + j] = pi - cr;
           if (!(pv != 0.0) && first_zero == 0) first_zero = cr + 1;
@@ -276,8 +285,7 @@ __global__ __launch_bounds__(PNT) void k_lu_panel(const uhmk_front* __restrict__
         for (int q = 0; q < R; ++q) {
           const int r = rb + tid + PNT * q;
           if (r > cr && r < fs) {
-            double l = v[q][j];
-            if (pv != 0.0) l = l / pv;
+            const double l = lu_mult(v[q][j], pv, rinv);
             v[q][j] = l;
 #pragma unroll
             for (int kk = j + 1; kk < PIB; ++kk) v[q][kk] = fma(-l, prow[kk], v[q][kk]);
@@ -526,8 +534,8 @@ inline int ret(hipError_t e) { return e == hipSuccess ? 0 : -(int)e; }
 // parity).  The columns run in groups of 8 (group index at run time,
 // column inside the group and every register index at compile time).
 // Pivot rule and arithmetic are those of k_lu_panel (first maximal |a| in
-// current order, IEEE division, rank-1 FMA updates in pivot order), so
-// both kernels give the same bits.
+// current order, multipliers as lu_mult forms them, rank-1 FMA updates in
+// pivot order), so both kernels give the same bits.
 // ------------------------------------------------------------------------
 template <typename T, int PNT>
 __global__ __launch_bounds__(PNT) void k_lu_panel_reg(const uhmk_front* __restrict__ F,
@@ -612,11 +620,10 @@ __global__ __launch_bounds__(PNT) void k_lu_panel_reg(const uhmk_front* __restri
       }
       if (tid == 0) s_piv[K] = pp - K;
       __syncthreads();
-      const double pv = s_row[buf][K];
+      const double pv = s_row[buf][K], rinv = 1.0 / pv;
       if (tid == 0 && !(pv != 0.0) && zero == 0) zero = K + 1;
       if (live && pos > K) {
-        double l = ak;
-        if (pv != 0.0) l = l / pv;
+        const double l = lu_mult(ak, pv, rinv);
 #pragma unroll
         for (int h = 0; h < 8; ++h) {
           if (h < g) continue;
@@ -672,6 +679,7 @@ __device__ __forceinline__ void static_for(Fn&& fn) {
 }
 
 constexpr int PL_BIG = 0x3fffffff;  // step of a slot that is not (yet) a pivot
+constexpr int PL_MAXROWS = 8192;    // pivot-candidate rows of the panel kernels
 constexpr int PL_LDR = TILE + 1;    // row stride of the pivot rows in LDS
 
 // Workgroup barrier for LDS traffic only: LDS writes complete before it,
@@ -778,6 +786,38 @@ __device__ __forceinline__ int panel_step(double (&v)[R][W], int& pos0, int (&ps
     }
   }
   wave_argmax(cv, cp, cs);
+  if constexpr (NW == 1 && R == 1) {
+    // one wavefront: its argmax is the pivot; the row comes from the
+    // owner's lane without an LDS round trip
+    int bs = cs, bp = cp;
+    if (bs < 0) {
+      // no comparable entry in the column: the row at position g stays
+      bs = __ffsll((unsigned long long)__ballot(pst[0] == PL_BIG && pos0 == g)) - 1;
+      bp = g;
+    }
+#pragma unroll
+    for (int j = K; j < W; ++j)
+      pr[j] = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v[0][j]), bs),
+                               __builtin_amdgcn_readlane(__double2loint(v[0][j]), bs));
+    if (tid == bs)
+#pragma unroll
+      for (int j = 0; j < W; ++j) L.pr[g * PL_LDR + j0 + j] = v[0][j];
+    const double pv = pr[K], rinv = 1.0 / pv;
+    if (!(pv != 0.0) && zero == 0) zero = g + 1;
+    if (tid == 0) {
+      L.piv[g] = bp - g;
+      L.ps[g] = bs;
+    }
+    if (tid == bs) pst[0] = g;
+    if (pst[0] == PL_BIG && pos0 == g) pos0 = bp;  // displaced by the pivot row
+    if (pst[0] == PL_BIG) {
+      const double l = lu_mult(v[0][K], pv, rinv);
+      v[0][K] = l;
+#pragma unroll
+      for (int j = K + 1; j < W; ++j) v[0][j] = fma(-l, pr[j], v[0][j]);
+    }
+    return bs;
+  }
   if (lane == 0) L.cand[buf][wid] = PanelCand{cv, cp, cs};
   // the wavefront's best row, published by its owner (a uniform register
   // index here became a dynamic index into scratch memory)
@@ -826,7 +866,7 @@ __device__ __forceinline__ int panel_step(double (&v)[R][W], int& pos0, int (&ps
   }
 #pragma unroll
   for (int j = K; j < W; ++j) pr[j] = L.crow[buf][bw][j];
-  const double pv = pr[K];
+  const double pv = pr[K], rinv = 1.0 / pv;
   if (!(pv != 0.0) && zero == 0) zero = g + 1;
   // the pivot row's block values (L part left of K, U part from K on)
   if (tid < W) L.pr[g * PL_LDR + j0 + tid] = L.crow[buf][bw][tid];
@@ -841,8 +881,7 @@ __device__ __forceinline__ int panel_step(double (&v)[R][W], int& pos0, int (&ps
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     if (pst[q] == PL_BIG) {
-      double l = v[q][K];
-      if (pv != 0.0) l = l / pv;
+      const double l = lu_mult(v[q][K], pv, rinv);
       v[q][K] = l;
 #pragma unroll
       for (int j = K + 1; j < W; ++j) v[q][j] = fma(-l, pr[j], v[q][j]);
@@ -881,13 +920,19 @@ __device__ __forceinline__ void panel_ustrip(PanelLds<W, PNT / 64>& L, int j0, i
 // order of first use) and, per listed row, the list index of the row whose
 // old content lands there.  The final slot at position g < kb is the
 // pivot of step g (L.ps); L.dsl / L.dpos hold the displaced slots (not
-// pivots, moved) and their final positions.
+// pivots, moved; all below kb) and their final positions, which are the
+// swap targets beyond kb.  A pivot slot at or beyond kb was never moved
+// before its step, so it is a swap target too.  mapS / mapU (LDS, m bytes
+// each, no initialization needed: only target positions are read) take
+// target position -> displaced slot and target position -> list index.
 template <int W, int PNT>
 __device__ __forceinline__ void panel_finish(PanelLds<W, PNT / 64>& L, const uhmk_front& fr, int f, int c0, int kb,
                                              int zero, int* __restrict__ piv, int* __restrict__ info,
-                                             int* __restrict__ pperm, int tid) {
+                                             int* __restrict__ pperm, unsigned char* mapS, unsigned char* mapU,
+                                             int tid) {
   for (int k = tid; k < kb; k += PNT) piv[fr.poff + c0 + k] = L.piv[k];
   if (tid == 0 && zero && info[f] == 0) info[f] = c0 + zero;
+  lds_barrier();  // the maps may overlay LDS that was read until here
   if (tid < 64) {
     const int k = tid;
     const int t = (k < kb) ? k + L.piv[k] : -1;
@@ -895,24 +940,22 @@ __device__ __forceinline__ void panel_finish(PanelLds<W, PNT / 64>& L, const uhm
 #pragma unroll
     for (int k2 = 0; k2 < 63; ++k2) first = first && !(k2 < k && __builtin_amdgcn_readlane(t, k2) == t);
     const unsigned long long mask = __ballot(first);
-    if (first) L.tgt[__popcll(mask & ((1ull << k) - 1ull))] = t;
+    if (first) {
+      const int u = __popcll(mask & ((1ull << k) - 1ull));
+      L.tgt[u] = t;
+      mapU[t] = (unsigned char)u;
+    }
     if (k == 0) L.ncnt = kb + __popcll(mask);
   }
-  __syncthreads();
-  const int cnt = L.ncnt, nt = cnt - kb, nd = min(L.nd, TILE);
+  const int nd = min(L.nd, TILE);
+  for (int d = tid; d < nd; d += PNT) mapS[L.dpos[d]] = (unsigned char)L.dsl[d];
+  lds_barrier();
+  const int cnt = L.ncnt;
   int* __restrict__ pp = pperm + (long long)f * PP_STRIDE;
   for (int k = tid; k < cnt; k += PNT) {
     const int lr = (k < kb) ? k : L.tgt[k - kb];
-    int src = lr;
-    if (lr < kb) {
-      src = L.ps[lr];
-    } else {
-      for (int d = 0; d < nd; ++d)
-        if (L.dpos[d] == lr) src = L.dsl[d];
-    }
-    int si = (src < kb) ? src : -1;
-    for (int u = 0; u < nt; ++u)
-      if (si < 0 && L.tgt[u] == src) si = kb + u;
+    const int src = (lr < kb) ? L.ps[lr] : (int)mapS[lr];
+    const int si = (src < kb) ? src : kb + (int)mapU[src];
     pp[1 + k] = c0 + lr;
     pp[1 + 2 * TILE + k] = si;
   }
@@ -932,6 +975,7 @@ __global__ __launch_bounds__(PNT) void k_lu_panel_rr(const uhmk_front* __restric
                                                      int* __restrict__ info, int* __restrict__ pperm) {
   constexpr int W = 16, NB = TILE / W, NW = PNT / 64;
   __shared__ PanelLds<W, NW> L;
+  __shared__ unsigned char s_map[2 * PNT];
   const int f = flist[blockIdx.x];
   const uhmk_front fr = F[f];
   const int n = fr.n, fs = fr.fs;
@@ -995,7 +1039,7 @@ __global__ __launch_bounds__(PNT) void k_lu_panel_rr(const uhmk_front* __restric
     }
   }
   __syncthreads();
-  panel_finish<W, PNT>(L, fr, f, c0, kb, zero, piv, info, pperm, tid);
+  panel_finish<W, PNT>(L, fr, f, c0, kb, zero, piv, info, pperm, s_map, s_map + PNT, tid);
 }
 
 // v -= L(slot, [i0, i0 + LC)) U([i0, i0 + LC), block) for every slot, one
@@ -1166,10 +1210,10 @@ __global__ __launch_bounds__(PNT) void k_lu_panel_ll(const uhmk_front* __restric
     const int d = idx % nd, c = idx / nd;
     P[L.dpos[d] + (long long)c * n] = (T)s_stage[d * TILE + c];
   }
-  panel_finish<W, PNT>(L, fr, f, c0, kb, zero, piv, info, pperm, tid);
+  unsigned char* map = reinterpret_cast<unsigned char*>(s_stage);
+  panel_finish<W, PNT>(L, fr, f, c0, kb, zero, piv, info, pperm, map, map + PL_MAXROWS, tid);
 }
 
-constexpr int PL_MAXROWS = 8192;  // pivot-candidate rows of the left-looking panel
 
 template <typename T>
 int launch_panel_ll(const uhmk_front* fronts, const int* flist, int n_fronts, int max_rows, int c0, int nb,
@@ -1189,7 +1233,7 @@ int launch_panel_ll(const uhmk_front* fronts, const int* flist, int n_fronts, in
   else if (max_rows <= 1536) UHM_PANEL_LL(3, 8, 8, true, 512);
   else if (max_rows <= 2048) UHM_PANEL_LL(4, 8, 4, true, 512);
   else if (max_rows <= 3072) UHM_PANEL_LL(6, 8, 4, false, 512);
-  else if (max_rows <= 4096) UHM_PANEL_LL(8, 8, 2, false, 512);
+  else if (max_rows <= 4096) UHM_PANEL_LL(8, 8, 1, false, 512);
   else if (max_rows <= 6144) UHM_PANEL_LL(12, 4, 2, false, 512);
   else if (max_rows <= PL_MAXROWS) UHM_PANEL_LL(16, 2, 1, false, 512);
   else return -1001;  // more pivot-candidate rows than the panel kernels hold
@@ -1241,8 +1285,8 @@ int launch_panel(const uhmk_front* fronts, const int* flist, int n_fronts, int m
 // positions) and stores the row at its final position at the end.  The
 // pivot rule and arithmetic are those of k_lu_panel / k_lu_rows: pivots
 // searched in the ATL rows only, first maximal |a| in current order, ATL
-// multipliers by IEEE division, ABL multipliers by the reciprocal of the
-// pivot, every update a rank-1 FMA in pivot order.
+// multipliers as lu_mult forms them (LAPACK dgetf2), ABL multipliers by the
+// reciprocal of the pivot, every update a rank-1 FMA in pivot order.
 // ------------------------------------------------------------------------
 __device__ __forceinline__ double lane_d(double v, int l) {
   return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
@@ -1316,7 +1360,7 @@ __global__ __launch_bounds__(256) void k_lu_front64(const uhmk_front* __restrict
       if (upd) {
         l = ak;
         if (abl) l = l * (1.0 / pv);
-        else if (pv != 0.0) l = l / pv;
+        else l = lu_mult(l, pv, 1.0 / pv);
       }
 #pragma unroll
       for (int h = 0; h < 8; ++h) {
