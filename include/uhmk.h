@@ -115,9 +115,10 @@ int uhmk_lu_panel_ll(const uhmk_front* fronts, const int* flist, int n_fronts, i
  * from the panel permutation in pperm, U12 = L11^-1 A12 (trsm left lower
  * unit) and L21 = A21 U11^-1 on the ABL rows (trsm right upper,
  * UHM/src/matrix/uhm/fla/lu_piv/decompose.cxx:136-146).
- * items: {f, kind (0 left cols, 1 ABL rows, 2 right cols), start, 0}. */
+ * items: {f, kind (0 left cols, 1 ABL rows, 2 right cols), start, 0}, each
+ * a chunk of `width` columns (rows for kind 1): 16, 32 or 64. */
 int uhmk_lu_rows(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, void* arena,
-                 int prec, const int* pperm, hipStream_t stream);
+                 int prec, const int* pperm, int width, hipStream_t stream);
 
 /* Whole-front LU of fronts with n <= 64, one wavefront per front, rows in
  * registers: pivots searched in ATL (relative format as uhmk_lu_panel),

@@ -184,3 +184,65 @@ def test_mesh_same_bits_both_panel_kernels(lib, kind, dims, p):
     m.solve_lu_piv()
     m.check_lu_piv()
     assert m.backward_error() <= 1e-13
+
+
+@pytest.mark.parametrize("prec", [U.PREC_FP64, U.PREC_MIXED])
+def test_lu_lookahead_same_bits(lib, prec):
+    """The LU lookahead (tuning key 12: the next panel on a second stream
+    beside the rest of the update) computes every tile and row item as the
+    in-order schedule does, so the factors and pivots are the same bits,
+    also when the schedule is captured in a graph and replayed."""
+    m = U.Mesh()
+    m.build_synthetic(2, (9, 8, 8), 3, 0)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    m.create_matrix_buffer()
+    m.set_random_params(seed=21, alpha=64.0, beta=0.1, scale_kind=1, p0=1.0)
+    m.random_matrix()
+    m.set_rhs()
+    m.set_precision(prec, 2)
+    m.set_tuning(12, 0)
+    assert m.lu_piv_with_free() == 0
+    F0 = [m.fetch_front(f) for f in range(m.n_fronts())]
+    P0 = [m.fetch_pivots(f) for f in range(m.n_fronts())]
+    m.set_tuning(12, 1)
+    for graph in (False, True, True):
+        m.set_graph(graph)
+        assert m.lu_piv_with_free() == 0
+        for f in range(m.n_fronts()):
+            assert np.array_equal(P0[f], m.fetch_pivots(f)), (graph, f)
+            assert np.array_equal(F0[f].view(np.uint64), m.fetch_front(f).view(np.uint64)), (graph, f)
+    m.set_graph(False)
+    m.solve_lu_piv()
+    m.check_lu_piv()
+    if prec == U.PREC_FP64:
+        assert m.backward_error() <= 1e-13
+
+
+def test_lu_row_chunks_same_bits(lib):
+    """The row kernel (swaps, U strip, ABL rows) in chunks of 16, 32 or 64
+    columns (tuning key 13) gives every column and row the same operations:
+    the same factors bit for bit."""
+    m = U.Mesh()
+    m.build_synthetic(2, (8, 9, 8), 3, 0)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    m.create_matrix_buffer()
+    m.set_random_params(seed=5, alpha=64.0, beta=0.1, scale_kind=1, p0=1.0)
+    m.random_matrix()
+    m.set_rhs()
+    ref = None
+    for w in (64, 32, 16):
+        m.set_tuning(13, w)
+        assert m.lu_piv_with_free() == 0
+        F = [m.fetch_front(f) for f in range(m.n_fronts())]
+        P = [m.fetch_pivots(f) for f in range(m.n_fronts())]
+        if ref is None:
+            ref = (F, P)
+            continue
+        for f in range(m.n_fronts()):
+            assert np.array_equal(ref[1][f], P[f]), (w, f)
+            assert np.array_equal(ref[0][f].view(np.uint64), F[f].view(np.uint64)), (w, f)
+    m.solve_lu_piv()
+    m.check_lu_piv()
+    assert m.backward_error() <= 1e-13

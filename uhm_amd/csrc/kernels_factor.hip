@@ -432,12 +432,13 @@ constexpr int LDP = TILE + 1;
 // (apply_pivots), the U strip U12 = L11^-1 A12 for the columns right of the
 // panel, and the ABL rows of the panel L21 = A21 U11^-1 (rows [fs, n)).
 // Item: f, a = kind (0 left columns, 1 ABL rows, 2 right columns),
-// b = first column / row of the 64-wide chunk.  The triangular solves use
-// 4 lanes per vector (tri_solve64), 64 vectors per workgroup.
+// b = first column / row of its chunk of NT / 4 (16, 32 or 64).  The
+// triangular solves use 4 lanes per vector (tri_solve64), NT / 4 vectors
+// per workgroup; narrow chunks spread one panel's row work over more CUs.
 // ------------------------------------------------------------------------
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ F,
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_lu_rows(const uhmk_front* __restrict__ F,
                                                  const uhmk_item* __restrict__ items, int c0, int nb,
                                                  T* __restrict__ arena, const int* __restrict__ pperm) {
   const uhmk_item it = items[blockIdx.x];
@@ -446,14 +447,16 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
   const int kb = min(nb, fs - c0);
   T* __restrict__ A = arena + fr.off;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, q = lane & 3;
-  __shared__ double s_X[2 * TILE][TILE];  // rows touched by the swaps, per column
+  constexpr int CW = NT / 4;  // columns (rows for ABL items) per item: four lanes each in the solves
+  __shared__ double s_X[2 * TILE][CW + 1];  // rows touched by the swaps (row stride CW + 1: the
+                                            // gather's consecutive rows on different banks)
   __shared__ double s_T[TILE * LDP];      // triangular panel block
   __shared__ double s_rd[TILE];
   __shared__ int s_prow[2 * TILE], s_psrc[2 * TILE];
   __shared__ int s_cnt;
 
   if (it.a == 1) {
-    uhmd::load_tri64<false, 256>(s_T, LDP, s_rd, A + c0 + (long long)c0 * n, n, kb, tid);
+    uhmd::load_tri64<false, NT>(s_T, LDP, s_rd, A + c0 + (long long)c0 * n, n, kb, tid);
     __syncthreads();
     const int row = it.b + 16 * w + (lane >> 2);
     const bool live = row < n;
@@ -474,7 +477,7 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
   }
 
   const int j0 = it.b;
-  const int width = min(TILE, (it.a == 0 ? c0 : n) - j0);
+  const int width = min(CW, (it.a == 0 ? c0 : n) - j0);
   {
     const int* __restrict__ pp = pperm + (long long)it.f * PP_STRIDE;
     if (tid == 0) s_cnt = pp[0];
@@ -485,13 +488,13 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
   }
   __syncthreads();
   const int pc = s_cnt;
-  for (int idx = tid; idx < width * pc; idx += 256) {
+  for (int idx = tid; idx < width * pc; idx += NT) {
     const int k = idx % pc, t = idx / pc;
     s_X[k][t] = (double)A[s_prow[k] + (long long)(j0 + t) * n];
   }
   __syncthreads();
   if (it.a == 0) {
-    for (int idx = tid; idx < width * pc; idx += 256) {
+    for (int idx = tid; idx < width * pc; idx += NT) {
       const int k = idx % pc, t = idx / pc;
       A[s_prow[k] + (long long)(j0 + t) * n] = (T)s_X[s_psrc[k]][t];
     }
@@ -499,11 +502,11 @@ __global__ __launch_bounds__(256) void k_lu_rows(const uhmk_front* __restrict__ 
   }
   // right columns: rows moved below the strip go back now; the strip is
   // solved in registers, one column per 4 lanes.
-  for (int idx = tid; idx < width * (pc - kb); idx += 256) {
+  for (int idx = tid; idx < width * (pc - kb); idx += NT) {
     const int k = kb + idx % (pc - kb), t = idx / (pc - kb);
     A[s_prow[k] + (long long)(j0 + t) * n] = (T)s_X[s_psrc[k]][t];
   }
-  uhmd::load_tri64<true, 256>(s_T, LDP, s_rd, A + c0 + (long long)c0 * n, n, kb, tid);
+  uhmd::load_tri64<true, NT>(s_T, LDP, s_rd, A + c0 + (long long)c0 * n, n, kb, tid);
   __syncthreads();
   const int t = 16 * w + (lane >> 2);
   double x[16];
@@ -1076,7 +1079,7 @@ __global__ __launch_bounds__(PNT) void k_lu_panel_ll(const uhmk_front* __restric
                                                      int* __restrict__ info, int* __restrict__ pperm) {
   constexpr int NW = PNT / 64;
   __shared__ PanelLds<W, NW> L;
-  __shared__ double s_stage[TILE * TILE];  // displaced rows for the final placement
+  __shared__ double s_stage[TILE * PL_LDR];  // displaced rows for the final placement (padded rows)
 
   const int f = flist[blockIdx.x];
   const uhmk_front fr = F[f];
@@ -1199,7 +1202,7 @@ __global__ __launch_bounds__(PNT) void k_lu_panel_ll(const uhmk_front* __restric
   const int nd = min(L.nd, TILE);
   for (int idx = tid; idx < nd * kb; idx += PNT) {
     const int d = idx % nd, c = idx / nd;
-    s_stage[d * TILE + c] = (double)P[L.dsl[d] + (long long)c * n];
+    s_stage[d * PL_LDR + c] = (double)P[L.dsl[d] + (long long)c * n];
   }
   __syncthreads();
   for (int idx = tid; idx < kb * kb; idx += PNT) {
@@ -1208,12 +1211,26 @@ __global__ __launch_bounds__(PNT) void k_lu_panel_ll(const uhmk_front* __restric
   }
   for (int idx = tid; idx < nd * kb; idx += PNT) {
     const int d = idx % nd, c = idx / nd;
-    P[L.dpos[d] + (long long)c * n] = (T)s_stage[d * TILE + c];
+    P[L.dpos[d] + (long long)c * n] = (T)s_stage[d * PL_LDR + c];
   }
   unsigned char* map = reinterpret_cast<unsigned char*>(s_stage);
   panel_finish<W, PNT>(L, fr, f, c0, kb, zero, piv, info, pperm, map, map + PL_MAXROWS, tid);
 }
 
+
+template <typename T>
+int launch_rows(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, T* arena,
+                const int* pperm, int width, hipStream_t stream) {
+  if (width == 16)
+    hipLaunchKernelGGL((k_lu_rows<T, 64>), dim3(n_items), dim3(64), 0, stream, fronts, items, c0, nb, arena, pperm);
+  else if (width == 32)
+    hipLaunchKernelGGL((k_lu_rows<T, 128>), dim3(n_items), dim3(128), 0, stream, fronts, items, c0, nb, arena, pperm);
+  else if (width == 64)
+    hipLaunchKernelGGL((k_lu_rows<T, 256>), dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb, arena, pperm);
+  else
+    return -1000;
+  return ret(hipGetLastError());
+}
 
 template <typename T>
 int launch_panel_ll(const uhmk_front* fronts, const int* flist, int n_fronts, int max_rows, int c0, int nb,
@@ -1470,19 +1487,16 @@ int uhmk_lu_front64(const uhmk_front* fronts, const int* flist, int n_fronts, vo
   return ret(hipGetLastError());
 }
 
+
 int uhmk_lu_rows(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, void* arena,
-                 int prec, const int* pperm, hipStream_t stream) {
+                 int prec, const int* pperm, int width, hipStream_t stream) {
   if (n_items <= 0) return 0;
   if (nb > TILE || nb < 1) return -1000;
   if (prec == 0)
-    hipLaunchKernelGGL(k_lu_rows<double>, dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb,
-                       static_cast<double*>(arena), pperm);
-  else if (prec == 1)
-    hipLaunchKernelGGL(k_lu_rows<float>, dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb,
-                       static_cast<float*>(arena), pperm);
-  else
-    return -1006;
-  return ret(hipGetLastError());
+    return launch_rows<double>(fronts, items, n_items, c0, nb, static_cast<double*>(arena), pperm, width, stream);
+  if (prec == 1)
+    return launch_rows<float>(fronts, items, n_items, c0, nb, static_cast<float*>(arena), pperm, width, stream);
+  return -1006;
 }
 
 }  // extern "C"

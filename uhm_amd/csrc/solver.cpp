@@ -284,7 +284,9 @@ void FrontSolver::create_buffers() {
   check_hip(hipMemsetAsync(d_rf_.p, 0, d_rf_.bytes, stream_), "memset rf");
   d_piv_.alloc(std::max(1, piv_elems_) * sizeof(int));
   d_perm_.alloc(std::max(1, piv_elems_) * sizeof(int));
-  d_pperm_.alloc(std::max<size_t>(1, fr_.size()) * 257 * sizeof(int));
+  // two panel permutations per front: with the LU lookahead the next panel
+  // (odd / even by panel index) runs while the row kernel reads this one's
+  d_pperm_.alloc(std::max<size_t>(1, fr_.size()) * 2 * 257 * sizeof(int));
   {
     std::vector<int> fsl;
     for (int f = 0; f < (int)fr_.size(); ++f)
@@ -676,19 +678,33 @@ void FrontSolver::build_schedule(int method) {
       lr.level = L;
       lr.kind = chol ? Launch::CHOL_TRSM : Launch::LU_ROWS;
       lr.off = (long long)items_.size();
+      const int rw = chol ? 64 : lu_rows_w_;  // LU row-kernel chunk (tile field of the launch)
+      lr.tile = rw;
       for (int f : act) {
         const int fs = fr_[f].fs, n = fr_[f].n;
         const int c1 = std::min(c0 + nb, fs);
         if (chol) {
           for (int i0 = c1; i0 < n; i0 += 64) items_.push_back(uhmk_item{f, 0, i0, 0});
         } else {
-          for (int j0 = 0; j0 < c0; j0 += 64) items_.push_back(uhmk_item{f, 0, j0, 0});
-          for (int i0 = fs; i0 < n; i0 += 64) items_.push_back(uhmk_item{f, 1, i0, 0});
-          for (int j0 = c1; j0 < n; j0 += 64) items_.push_back(uhmk_item{f, 2, j0, 0});
+          for (int j0 = 0; j0 < c0; j0 += rw) items_.push_back(uhmk_item{f, 0, j0, 0});
+          for (int i0 = fs; i0 < n; i0 += rw) items_.push_back(uhmk_item{f, 1, i0, 0});
+          for (int j0 = c1; j0 < n; j0 += rw) items_.push_back(uhmk_item{f, 2, j0, 0});
         }
       }
       lr.count = (int)(items_.size() - lr.off);
-      if (split_prev) {
+      // LU lookahead: the next panel's column block (row kernel and ATL
+      // update) goes first; the next panel then runs on the lookahead stream
+      // while the main stream applies this panel to the rest
+      const bool lla = !chol && lu_lookahead_ && nb == 64 && c0 + nb < maxfs;
+      if (split_prev && !chol) {
+        // this LU panel on the lookahead stream, after the update of its
+        // columns (ev_next); the main stream waits for it before its row kernel
+        lp.stream = 1;
+        lp.wait_ev = ev_next;
+        lp.record_ev = pending = n_events++;
+        launches_.push_back(lp);
+        if (lr.count && !lla) push_main(lr);
+      } else if (split_prev) {
         // this panel's diagonal block and TRSM on the lookahead stream, after
         // the update of its columns (ev_next); the main stream waits for them
         // before its next launch
@@ -702,10 +718,25 @@ void FrontSolver::build_schedule(int method) {
         launches_.back().record_ev = pending = n_events++;
       } else {
         if (!diag_done) push_main(lp);
-        if (lr.count) push_main(lr);
+        if (lr.count && !lla) push_main(lr);
       }
       split_prev = false;
       const bool la = chol && lookahead_ && c0 + nb < maxfs;
+      Launch lr_rest = lr;
+      if (lla) {
+        // row-kernel items of the next panel's block (kind 2 at c1) first
+        auto next_block = [&](const uhmk_item& it) {
+          const int c1 = std::min(c0 + nb, fr_[it.f].fs);
+          return it.a == 2 && it.b >= c1 && it.b < c1 + 64;
+        };
+        std::stable_partition(items_.begin() + lr.off, items_.end(), next_block);
+        int nn = 0;
+        for (long long q = lr.off; q < (long long)items_.size() && next_block(items_[q]); ++q) ++nn;
+        lr.count = nn;
+        lr_rest.off = lr.off + nn;
+        lr_rest.count -= nn;
+        if (lr.count) push_main(lr);
+      }
 
       if (G > 1) {
         // update of the rest of the group by this panel (K = [c0, c0+nb))
@@ -875,6 +906,39 @@ void FrontSolver::build_schedule(int method) {
         if (lg.count) push_main(lg);
         continue;
       }
+      if (lla) {
+        // (LU) the ATL rows of the next panel's column block in 64 x 64
+        // tiles, then the row kernel's other items and the rest of the update
+        Launch ln = lg;
+        ln.tile = 64;
+        ln.off = (long long)items_.size();
+        for (int f : act) {
+          const int fs = fr_[f].fs, c1 = std::min(c0 + nb, fs);
+          for (int i0 = c1; i0 < fs; i0 += 64) items_.push_back(uhmk_item{f, i0, c1, 0});
+        }
+        ln.count = (int)(items_.size() - ln.off);
+        ln.record_ev = ev_next = n_events++;
+        push_main(ln);
+        split_prev = true;
+        if (lr_rest.count) push_main(lr_rest);
+        lg.tile = 64;
+        lg.off = (long long)items_.size();
+        for (int f : act) {
+          const int fs = fr_[f].fs, n = fr_[f].n;
+          const int c1 = std::min(c0 + nb, fs);
+          tile_starts(c1, n, fs, 64, R);
+          tile_starts(c1, n, fs, 64, C);
+          for (int j0 : C)
+            for (int i0 : R) {
+              if (i0 >= fs && j0 >= fs) continue;
+              if (i0 < fs && j0 == c1) continue;  // in the lookahead launch
+              items_.push_back(uhmk_item{f, i0, j0, 0});
+            }
+        }
+        lg.count = (int)(items_.size() - lg.off);
+        if (lg.count) push_main(lg);
+        continue;
+      }
       lg.off = (long long)items_.size();
       for (int f : act) {
         const int fs = fr_[f].fs, n = fr_[f].n;
@@ -1004,12 +1068,13 @@ void FrontSolver::build_schedule(int method) {
           const uhmk_item& it = items_[l.off + q];
           const int fs = fr_[it.f].fs, n = fr_[it.f].n;
           const double kb = std::min(l.nb, fs - l.c0);
+          const int cw = l.kind == Launch::LU_ROWS ? l.tile : 64;
           if (l.kind == Launch::CHOL_TRSM || it.a == 1) {
-            const double h = std::min(64, n - it.b);
+            const double h = std::min(cw, n - it.b);
             fl += kb * kb * h;
             by += 16.0 * kb * h;
           } else {
-            const double w = std::min(64, (it.a == 0 ? l.c0 : n) - it.b);
+            const double w = std::min(cw, (it.a == 0 ? l.c0 : n) - it.b);
             if (it.a == 2) fl += kb * kb * w;
             by += 16.0 * 2.0 * kb * w;
           }
@@ -1145,10 +1210,10 @@ void FrontSolver::enqueue_factor(int method, std::vector<hipEvent_t>* pev) {
         break;
       case Launch::LU_PANEL:
         rc = (lu_panel_ll_ ? uhmk_lu_panel_ll : uhmk_lu_panel)(F, FL + l.off, l.count, l.extra, l.c0, l.nb, A,
-                                                               l.prec, P, info, d_pperm_.as<int>(), st);
+                                                               l.prec, P, info, pperm_of(l), st);
         break;
       case Launch::LU_ROWS:
-        rc = uhmk_lu_rows(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, d_pperm_.as<int>(), st);
+        rc = uhmk_lu_rows(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, pperm_of(l), l.tile, st);
         break;
       case Launch::LU_FRONT:
         rc = uhmk_lu_front64(F, FL + l.off, l.count, A, l.prec, P, info, st);
@@ -1257,6 +1322,15 @@ void FrontSolver::set_tuning(int key, int value) {
     if (value < -1 || value > 2) throw std::runtime_error("set_tuning: class group launches take -1 (global), 0, 1 or 2");
     if (value != cls_gkern_[key - 120]) schedule_method_ = 0;
     cls_gkern_[key - 120] = value;
+  } else if (key == 13) {
+    if (value != 16 && value != 32 && value != 64)
+      throw std::runtime_error("set_tuning: key 13 (LU row-kernel chunk) is 16, 32 or 64");
+    if (value != lu_rows_w_) schedule_method_ = 0;
+    lu_rows_w_ = value;
+  } else if (key == 12) {
+    if (value != 0 && value != 1) throw std::runtime_error("set_tuning: key 12 (LU lookahead) is 0 or 1");
+    if (value != lu_lookahead_) schedule_method_ = 0;
+    lu_lookahead_ = value;
   } else if (key == 11) {
     if (value != 0 && value != 1) throw std::runtime_error("set_tuning: key 11 (LU panel kernel) is 0 or 1");
     if (value != lu_panel_ll_) drop_graphs();  // captured launches name the kernel
@@ -1283,7 +1357,7 @@ void FrontSolver::set_tuning(int key, int value) {
                              "6: Cholesky diagonal blocks fused into the panel updates, 7: Cholesky panel groups "
                              "in one persistent launch, 8: Cholesky level solves in one launch per level, "
                              "9: fused small-front Cholesky levels, 10: smallest mean Schur block of a level "
-                             "with low-precision products, 11: LU panel kernel (1 left-looking, 0 right-looking), 100 + c / 110 + c / 120 + c: panel group / tile "
+                             "with low-precision products, 11: LU panel kernel (1 left-looking, 0 right-looking), 12: LU lookahead, 13: LU row-kernel chunk (16, 32, 64), 100 + c / 110 + c / 120 + c: panel group / tile "
                              "threshold / group launches of front size class c)");
   }
 }
@@ -1300,6 +1374,8 @@ int FrontSolver::get_tuning(int key) const {
   if (key == 9) return small_fused_;
   if (key == 10) return lp_min_ss_;
   if (key == 11) return lu_panel_ll_;
+  if (key == 12) return lu_lookahead_;
+  if (key == 13) return lu_rows_w_;
   if (key >= 100 && key < 100 + N_FRONT_CLASSES) return cls_group_[key - 100];
   if (key >= 110 && key < 110 + N_FRONT_CLASSES) return cls_tile_[key - 110];
   if (key >= 120 && key < 120 + N_FRONT_CLASSES) return cls_gkern_[key - 120];

@@ -273,6 +273,8 @@ class FrontSolver {
                           // FP32 products instead of packed BF16 / FP16 ones
   int lu_panel_ll_ = 1;   // LU panels: left-looking register-block kernel (1) or the right-looking
                           // one (0); the same bits on FP64 fronts
+  int lu_lookahead_ = 1;  // LU: the next panel on the lookahead stream beside the rest of the update
+  int lu_rows_w_ = 32;    // LU row kernel: columns (ABL rows) per workgroup (measured best of 16, 32, 64)
   int solve_level_ = 1;   // Cholesky solves with inverses: one launch per level and direction (1) or
                           // one per 64-row block step (0)
   int group_kernel_ = 2;  // Cholesky: panel groups in one persistent launch (uhmk_chol_group): 0 never,
@@ -330,6 +332,11 @@ class FrontSolver {
   // device buffers
   DevBuf dF_, d_emap_, d_inv_, d_srcbeg_, d_srcs_, d_lmap_, d_goff_;
   DevBuf d_items_, d_flists_, d_gen_items_, d_lvl_, d_sitems_, d_sflist_, d_fsl_, d_perm_, d_pperm_;
+  // panel permutation buffer of an LU panel / row launch: one of two per
+  // front, alternating by panel index (the lookahead panel writes the other)
+  int* pperm_of(const Launch& l) {
+    return d_pperm_.as<int>() + (size_t)((l.c0 / (l.nb > 0 ? l.nb : 1)) & 1) * fr_.size() * 257;
+  }
   DevBuf d_arena_, d_arena32_, d_piv_, d_x_, d_b_, d_stage_, d_info_;
   DevBuf d_flags_;  // progress flags of the Cholesky group launches, [0]: waits that gave up
   DevBuf d_serr_;   // level solves: waits that gave up
