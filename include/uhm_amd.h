@@ -85,9 +85,11 @@ void uhm_estimate_cost_(uhm_fort_p* mesh, uhm_fort_int* method, uhm_fort_int* da
                         uhm_fort_double* flop_solve, uhm_fort_uint* n_nonzero_factor,
                         uhm_fort_double* buffer);                           /* fort.cxx:177 */
 void uhm_mesh_import_(uhm_fort_p* mesh, uhm_fort_char* filename);           /* fort.cxx:191 */
-/* METIS-free tree: recursive bisection of the breadth-first element order
- * (the reference's default builder needs METIS internals,
- * UHM/src/operation/build_tree_var4.cxx:50-66). */
+/* METIS-free tree: the reference's coarsening builder build_tree_var_1
+ * (UHM/src/operation/build_tree_var1.cxx:53-181; element id order stands in
+ * for its element-address order), since the reference's default builder
+ * needs METIS internals (build_tree_var4.cxx:50-66).  A disconnected mesh's
+ * remaining roots are joined by bisecting their breadth-first order. */
 void uhm_build_tree_(uhm_fort_p* mesh);                                     /* fort.cxx:197 */
 void uhm_create_matrix_without_buffer_(uhm_fort_p* mesh, uhm_fort_int* datatype,
                                        uhm_fort_int* n_rhs);                /* fort.cxx:202 */

@@ -246,3 +246,4 @@ def test_lu_row_chunks_same_bits(lib):
     m.solve_lu_piv()
     m.check_lu_piv()
     assert m.backward_error() <= 1e-13
+

@@ -495,3 +495,44 @@ def test_cholesky_group_launch_matches_panel_path(lib, prec, group):
     else:
         m.improve(U.UHM_CHOL, 1e-12, 30)
         assert m.backward_error() <= 1e-12
+
+
+@pytest.mark.parametrize("method", [U.UHM_LU_PIV, U.UHM_CHOL])
+def test_coarsening_tree_factor_solve_vs_oracle(lib, oracle_lib, method):
+    """A mesh of leaf elements only, its tree from uhm_build_tree_ (the
+    reference's METIS-free coarsening builder), factorized and solved on the
+    GPU: backward error <= 1e-13 and the solution equal to the oracle's on
+    the same tree and element matrices."""
+    from meshes import MeshSpec, qp_spec
+    from oracle.oracle import OracleMesh
+    spec = qp_spec((9, 7), 3)
+    leaves = MeshSpec(spec.nodes, spec.elems, [-1] * len(spec.elems), [[] for _ in spec.elems])
+    m, _ = leaves.build_product(U)
+    m.build_tree()
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    m.create_matrix_buffer()
+    m.set_random_params(seed=8, alpha=16.0, beta=0.1 if method == U.UHM_LU_PIV else 0.0, scale_kind=1, p0=0.5)
+    m.random_matrix()
+    m.set_rhs()
+    if method == U.UHM_LU_PIV:
+        assert m.lu_piv_with_free() == 0
+        m.solve_lu_piv()
+        m.check_lu_piv()
+    else:
+        assert m.chol_with_free() == 0
+        m.solve_chol()
+        m.check_chol()
+    assert m.backward_error() <= 1e-13
+    om = OracleMesh.from_connectivity(m.export_connectivity())
+    om.init_numeric(1)
+    for f in range(m.n_fronts()):
+        fi = m.front_info(f)
+        if fi["leaf"]:
+            om.set_front(fi["id"], m.fetch_stage(f))
+            om.set_rhs_front(fi["id"], m.fetch_rhs_front(f))
+    assert om.factor(method, 64) == 0
+    om.solve(method)
+    xg = m.fetch_x()[:, 0]
+    xo = om.x_global(len(xg))[:, 0]
+    assert np.abs(xg - xo).max() <= 1e-10 * np.abs(xo).max()

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import uhm_amd as U
-from meshes import ex1_spec, qp_spec, spec_from_connectivity
+from meshes import MeshSpec, ex1_spec, qp_spec, spec_from_connectivity
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -189,3 +189,69 @@ def test_import_uhm_fixture_then_build_tree_matches_oracle_lock(lib, oracle_lib)
     assert m.get_n_dof() == len(spec.nodes)
     tree = spec_from_connectivity(c)
     _compare(m, tree, tree.oracle())
+
+
+def _chain_mesh(n_elem, first_id=0, m=None):
+    """1D chain: element k holds end nodes n_k, n_{k+1} and interior node
+    i_k (one dof each)."""
+    m = m or U.Mesh()
+    ends = [m.add_node(first_id + k, 1) for k in range(n_elem + 1)]
+    inner = [m.add_node(first_id + 1000 + k, 1) for k in range(n_elem)]
+    for k in range(n_elem):
+        e = m.add_element()
+        for nd in (ends[k], inner[k], ends[k + 1]):
+            e.add_node(nd)
+    return m
+
+
+def test_build_tree_coarsening_hand_derived_chain(lib):
+    """build_tree_var_1 (build_tree_var1.cxx:53-181) on a chain of four
+    elements, derived by hand: the shared end nodes n1..n3 are the schur
+    nodes; branch weights (shared dofs, minus other schur dofs) are
+    (e0,e1) = (1,-1), (e1,e2) = (1,-2), (e2,e3) = (1,-1), visited from the
+    largest, equal weights last-inserted first: (e2,e3) -> element 4,
+    (e0,e1) -> element 5, (e1,e2) skipped; then (4,5) -> root 6."""
+    m = _chain_mesh(4)
+    m.build_tree()
+    c = m.export_connectivity()
+    assert c["parent"].tolist() == [5, 5, 4, 4, 6, 6, -1]
+    kids = [c["cidx"][c["cptr"][e]:c["cptr"][e + 1]].tolist() for e in range(7)]
+    assert kids == [[], [], [], [], [2, 3], [0, 1], [4, 5]]
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    assert m.get_n_dof() == 9
+
+
+def test_build_tree_coarsening_mesh_matches_oracle_lock(lib, oracle_lib):
+    """On a 2D Q2 mesh of leaves only: one binary tree over all elements,
+    generations decreasing to the root, and its lock equal to the oracle's
+    lock of the same connectivity."""
+    spec = qp_spec((7, 6), 2)
+    leaves = MeshSpec(spec.nodes, spec.elems, [-1] * len(spec.elems), [[] for _ in spec.elems])
+    m, _ = leaves.build_product(U)
+    m.build_tree()
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    c = m.export_connectivity()
+    ne = len(spec.elems)
+    assert len(c["parent"]) == 2 * ne - 1
+    assert sum(1 for p in c["parent"] if p < 0) == 1
+    for e in range(ne, 2 * ne - 1):
+        assert c["cptr"][e + 1] - c["cptr"][e] == 2
+    assert m.get_n_dof() == len(spec.nodes)
+    tree = spec_from_connectivity(c)
+    _compare(m, tree, tree.oracle())
+
+
+def test_build_tree_disconnected_mesh_single_root(lib):
+    """Two chains with no node in common never pair by coarsening (the
+    reference's loop would not end); their roots are joined into one tree."""
+    m = _chain_mesh(3)
+    _chain_mesh(3, first_id=500, m=m)
+    m.build_tree()
+    c = m.export_connectivity()
+    assert len(c["parent"]) == 2 * 6 - 1
+    assert sum(1 for p in c["parent"] if p < 0) == 1
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    assert m.get_n_dof() == 2 * (4 + 3)

@@ -157,7 +157,7 @@ void uhm_mesh_import_(uhm_fort_p* mesh, uhm_fort_char* filename) {
   guard("uhm_mesh_import", [&] { M(mesh)->import_file(filename); });
 }
 void uhm_build_tree_(uhm_fort_p* mesh) {
-  guard("uhm_build_tree", [&] { M(mesh)->build_tree_bisection(); });
+  guard("uhm_build_tree", [&] { M(mesh)->build_tree_coarsening(); });
 }
 void uhm_create_matrix_without_buffer_(uhm_fort_p* mesh, uhm_fort_int* datatype, uhm_fort_int* n_rhs) {
   guard("uhm_create_matrix_without_buffer", [&] { M(mesh)->create_matrix_without_buffer(*datatype, *n_rhs); });

@@ -7,6 +7,7 @@
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <set>
 #include <stdexcept>
 
 #include "solver.hpp"
@@ -384,6 +385,101 @@ void Mesh_::unlock() {
   level_gen_.clear();
   leaves_.clear();
   locked_ = false;
+}
+
+void Mesh_::build_tree_coarsening() {
+  // build_tree_var_1 (UHM/src/operation/build_tree_var1.cxx:53-181): the
+  // METIS-free tree builder by repeated pairwise coarsening of the orphan
+  // elements.  The reference orders element pairs and node owners by
+  // element address; here element id (creation order) stands in for it, so
+  // the tree is deterministic.
+  if (locked_) throw std::runtime_error("build_tree: mesh is locked");
+  // preprocess (lines 56-61): leaves restore their connectivity, then every
+  // element, leaves first, merges its children's schur nodes and separates
+  update_generation();
+  {
+    std::map<int, std::vector<Element_*>> gens;
+    for (auto& e : elements_) {
+      if (e.is_leaf()) e.restore_connectivity();
+      gens[e.generation].push_back(&e);
+    }
+    for (auto it = gens.rbegin(); it != gens.rend(); ++it)
+      for (Element_* e : it->second) {
+        e->merge_nodes_from_children();
+        e->separate_nodes();
+      }
+  }
+  auto owned_by = [](const Node_* n, const Element_* e) {
+    for (const Element_* o : n->owner)
+      if (o == e) return true;
+    return false;
+  };
+  // 1. the orphans in element order (lines 74-78)
+  std::vector<Element_*> orphan, tmp;
+  for (auto& e : elements_)
+    if (e.is_orphan()) orphan.push_back(&e);
+  // 2. coarsening (lines 81-176)
+  while (orphan.size() > 1) {
+    // 2.1 branches: pairs of orphans that share a schur node, each with the
+    // weight (dofs of the schur nodes the pair shares and nobody else holds,
+    // counted once; minus the dofs of every other schur node of either)
+    std::set<std::pair<int, int>> branch;
+    std::vector<std::pair<std::pair<int, int>, std::pair<Element_*, Element_*>>> weighted;  // insertion order
+    for (Element_* o : orphan)
+      for (auto& nf : o->nodes) {
+        if (nf.second != UHM_SEPARATED_SCHUR) continue;
+        std::vector<Element_*> owners = nf.first->owner;
+        std::sort(owners.begin(), owners.end(), [](const Element_* a, const Element_* b) { return a->id < b->id; });
+        for (Element_* e2 : owners) {
+          if (e2 == o) continue;
+          Element_* e[2] = {o, e2};
+          if (e[0]->id > e[1]->id) std::swap(e[0], e[1]);
+          if (!branch.insert({e[0]->id, e[1]->id}).second) continue;
+          std::pair<int, int> w(0, 0);
+          for (int i = 0; i < 2; ++i)
+            for (auto& nn : e[i]->nodes) {
+              if (nn.second != UHM_SEPARATED_SCHUR) continue;
+              if (nn.first->get_n_owner() == 2 && owned_by(nn.first, e[(i + 1) % 2]))
+                w.first += i ? 0 : nn.first->n_dof;
+              else
+                w.second -= nn.first->n_dof;
+            }
+          weighted.push_back({w, {e[0], e[1]}});
+        }
+      }
+    // the multimap of the reference: ascending weight, equal weights in
+    // insertion order; 2.2 visits it in reverse (largest weight first, equal
+    // weights last-inserted first)
+    std::stable_sort(weighted.begin(), weighted.end(),
+                     [](const auto& a, const auto& b) { return a.first < b.first; });
+    bool paired = false;
+    for (auto it = weighted.rbegin(); it != weighted.rend(); ++it) {
+      Element_* e0 = it->second.first;
+      Element_* e1 = it->second.second;
+      if (!e0->is_orphan() || !e1->is_orphan()) continue;
+      const int gen = std::min(e0->generation, e1->generation) - 1;
+      Element_* p = add_element(gen);
+      for (Element_* c : {e0, e1}) {
+        p->add_child(c);
+        c->set_parent(p);
+      }
+      tmp.push_back(p);
+      p->merge_nodes_from_children();
+      p->separate_nodes();
+      paired = true;
+    }
+    // 2.3 the orphans left over join the new parents
+    for (Element_* o : orphan)
+      if (o->is_orphan()) tmp.push_back(o);
+    std::swap(orphan, tmp);
+    tmp.clear();
+    // orphans that share no schur node (a disconnected mesh) never pair,
+    // and the reference's loop would not end: they are joined by the
+    // breadth-first bisection below instead
+    if (!paired) break;
+  }
+  if (orphan.size() > 1) build_tree_bisection();
+  update_generation();
 }
 
 void Mesh_::build_tree_bisection() {

@@ -134,7 +134,8 @@ class Mesh_ {
 
   // --- tree + analysis
   void update_generation();  // leaf 0, parent = min(children) - 1 (element.cxx:63-72)
-  void build_tree_bisection();  // METIS-free fallback tree builder
+  void build_tree_coarsening();  // METIS-free tree builder (build_tree_var1.cxx:53-181)
+  void build_tree_bisection();   // breadth-first bisection of the orphans (disconnected meshes)
   void lock();    // mesh.cxx:438-456
   void unlock();
   bool is_locked() const { return locked_; }
