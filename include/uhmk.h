@@ -317,6 +317,12 @@ int uhmk_solve_mark(double* p, long long n, hipStream_t stream);
 int uhmk_solve_block_step(const uhmk_front* fronts, const uhmk_item* items, int n_items, int mode, int fwd, int k0,
                           const void* arena, int prec, double* x, double* z, int nrhs, hipStream_t stream);
 
+/* The LU (mode 0) block step at k0 = 0 for items whose front has fs <= 32
+ * (each the front's own block): one wavefront per item, the right-hand-side
+ * columns in registers; the same bits as uhmk_solve_block_step. */
+int uhmk_solve_block_step_small(const uhmk_front* fronts, const uhmk_item* items, int n_items, int fwd,
+                                const void* arena, int prec, double* x, double* z, int nrhs, hipStream_t stream);
+
 /* Level-batched off-diagonal products of the block solves, 64 rows x 64
  * right-hand sides per item: dir 0: x[ABL rows] -= ABL z[0:fs) (forward
  * xb update); dir 1: z[ATL rows] -= ATR x[fs:n) (LU backward); dir 2:

@@ -176,3 +176,30 @@ def test_leaf_residual_matches_csr(lib, nrhs):
     assert eta > 1e-12
     assert abs(eta_dev - eta) <= 1e-6 * eta
     assert abs(r1_dev - r1) <= 1e-6 * r1
+
+
+@pytest.mark.parametrize("prec", [U.PREC_FP64, U.PREC_MIXED])
+def test_small_front_block_steps_same_bits(lib, prec):
+    """The LU block steps of fronts with fs <= 32 by one wavefront per item
+    (tuning key 14) give the same solution bits as the general block-step
+    kernel, on a variable-order 2D mesh whose lower levels are all small."""
+    m = U.Mesh()
+    m.build_synthetic(3, (24, 20), 8, 12345)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 70)
+    m.create_matrix_buffer()
+    m.set_random_params(seed=4, alpha=-1.0, beta=0.1)
+    m.random_matrix()
+    m.set_rhs()
+    m.set_precision(prec, 1)
+    assert m.lu_piv_with_free() == 0
+    m.set_tuning(14, 0)
+    m.solve_lu_piv()
+    x0 = m.fetch_x().copy()
+    m.set_tuning(14, 1)
+    m.solve_lu_piv()
+    x1 = m.fetch_x()
+    assert np.array_equal(x0.view(np.uint64), x1.view(np.uint64))
+    if prec == U.PREC_FP64:
+        m.check_lu_piv()
+        assert m.backward_error() <= 1e-13

@@ -673,6 +673,71 @@ __global__ __launch_bounds__(256) void k_svb_step(const uhmk_front* __restrict__
   sub_acc((FWD ? x : z) + fr.xoff, n, r, FWD ? fs : k0, c0, nrhs, acc, lane, w);
 }
 
+// Block step of LU fronts with fs <= 32 at k0 = 0 (the item is the front's
+// own block): one wavefront per (front, 64-column tile of right-hand
+// sides); lane c solves column c0 + c in registers against the triangle
+// staged in LDS step-major (sT[k][i] = T(i, k), zero outside the triangle).
+// Every written entry sees k_svb_step's TriStep operations in the same order
+// (forward, unit L: x_i = fma(-x_k, L(i, k), x_i) for every row at steps
+// k = 0 .. kb-1, zeros included; backward: x_k *= 1 / U(k, k), then the same
+// with U(i, k), i < k, at steps k = kb-1 .. 0), so both give the same bits.
+template <typename T, bool FWD>
+__global__ __launch_bounds__(64) void k_svb_small(const uhmk_front* __restrict__ F,
+                                                  const uhmk_item* __restrict__ items,
+                                                  const T* __restrict__ arena, double* __restrict__ x,
+                                                  double* __restrict__ z, int nrhs) {
+  constexpr int KM = 32, LS = KM + 2;  // step-major rows of KM entries (+2: 16-byte aligned rows)
+  __shared__ double sT[KM * LS];
+  __shared__ double rdg[KM];
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, kb = min(KM, fr.fs);
+  const T* __restrict__ B = arena + fr.off;
+  const int lane = threadIdx.x, c0 = it.a;
+  for (int idx = lane; idx < KM * KM; idx += 64) {
+    const int a = idx & (KM - 1), b = idx / KM;  // block entry (a, b): row a of column b
+    const bool in = a < kb && b < kb && (FWD ? a > b : a <= b);
+    const double v = uhmd::ldz(B, a + (long long)b * n, in);
+    sT[b * LS + a] = (a == b) ? 0.0 : v;
+    if (!FWD && a == b) rdg[a] = a < kb ? 1.0 / v : 1.0;
+  }
+  __syncthreads();
+  const bool live = c0 + lane < nrhs;
+  const long long col = fr.xoff + (long long)(live ? c0 + lane : 0) * n;
+  const double* __restrict__ src = (FWD ? x : z) + col;
+  double xv[KM];
+#pragma unroll
+  for (int k = 0; k < KM; ++k) xv[k] = uhmd::ldz(src, k, live && k < kb);
+  if constexpr (FWD) {
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+      if (k < kb) {
+        const double xk = xv[k];
+        const double* __restrict__ tk = sT + k * LS;
+#pragma unroll
+        for (int i = 0; i < KM; ++i) xv[i] = fma(-xk, tk[i], xv[i]);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = KM - 1; k >= 0; --k) {
+      if (k < kb) {
+        xv[k] = xv[k] * rdg[k];
+        const double xk = xv[k];
+        const double* __restrict__ tk = sT + k * LS;
+#pragma unroll
+        for (int i = 0; i < KM; ++i) xv[i] = fma(-xk, tk[i], xv[i]);
+      }
+    }
+  }
+  if (live) {
+    double* __restrict__ dst = (FWD ? z : x) + col;
+#pragma unroll
+    for (int k = 0; k < KM; ++k)
+      if (k < kb) dst[k] = xv[k];
+  }
+}
+
 // out = max over rows of sum |A(row, :)| (CSR), one wavefront per row.
 __global__ __launch_bounds__(256) void k_rowsum_max(long long nrow, const long long* __restrict__ rowptr,
                                                     const double* __restrict__ val, double* __restrict__ out) {
@@ -746,6 +811,24 @@ int uhmk_solve_block_step(const uhmk_front* fronts, const uhmk_item* items, int 
     return launch_step<float>(mode, fwd, fronts, items, n_items, k0, static_cast<const float*>(arena), x, z, nrhs,
                               stream);
   return -1006;
+}
+
+int uhmk_solve_block_step_small(const uhmk_front* fronts, const uhmk_item* items, int n_items, int fwd,
+                                const void* arena, int prec, double* x, double* z, int nrhs, hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  const dim3 g(n_items), b(64);
+  if (prec == 0) {
+    const double* A = static_cast<const double*>(arena);
+    if (fwd) hipLaunchKernelGGL((k_svb_small<double, true>), g, b, 0, stream, fronts, items, A, x, z, nrhs);
+    else hipLaunchKernelGGL((k_svb_small<double, false>), g, b, 0, stream, fronts, items, A, x, z, nrhs);
+  } else if (prec == 1) {
+    const float* A = static_cast<const float*>(arena);
+    if (fwd) hipLaunchKernelGGL((k_svb_small<float, true>), g, b, 0, stream, fronts, items, A, x, z, nrhs);
+    else hipLaunchKernelGGL((k_svb_small<float, false>), g, b, 0, stream, fronts, items, A, x, z, nrhs);
+  } else {
+    return -1006;
+  }
+  return ret(hipGetLastError());
 }
 
 int uhmk_solve_block_gemm(const uhmk_front* fronts, const uhmk_item* items, int n_items, int dir,

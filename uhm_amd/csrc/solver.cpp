@@ -1322,6 +1322,10 @@ void FrontSolver::set_tuning(int key, int value) {
     if (value < -1 || value > 2) throw std::runtime_error("set_tuning: class group launches take -1 (global), 0, 1 or 2");
     if (value != cls_gkern_[key - 120]) schedule_method_ = 0;
     cls_gkern_[key - 120] = value;
+  } else if (key == 14) {
+    if (value != 0 && value != 1) throw std::runtime_error("set_tuning: key 14 (small-front LU block steps) is 0 or 1");
+    if (value != svb_small_) solve_method_ = 0;
+    svb_small_ = value;
   } else if (key == 13) {
     if (value != 16 && value != 32 && value != 64)
       throw std::runtime_error("set_tuning: key 13 (LU row-kernel chunk) is 16, 32 or 64");
@@ -1357,7 +1361,7 @@ void FrontSolver::set_tuning(int key, int value) {
                              "6: Cholesky diagonal blocks fused into the panel updates, 7: Cholesky panel groups "
                              "in one persistent launch, 8: Cholesky level solves in one launch per level, "
                              "9: fused small-front Cholesky levels, 10: smallest mean Schur block of a level "
-                             "with low-precision products, 11: LU panel kernel (1 left-looking, 0 right-looking), 12: LU lookahead, 13: LU row-kernel chunk (16, 32, 64), 100 + c / 110 + c / 120 + c: panel group / tile "
+                             "with low-precision products, 11: LU panel kernel (1 left-looking, 0 right-looking), 12: LU lookahead, 13: LU row-kernel chunk (16, 32, 64), 14: small-front LU block steps, 100 + c / 110 + c / 120 + c: panel group / tile "
                              "threshold / group launches of front size class c)");
   }
 }
@@ -1376,6 +1380,7 @@ int FrontSolver::get_tuning(int key) const {
   if (key == 11) return lu_panel_ll_;
   if (key == 12) return lu_lookahead_;
   if (key == 13) return lu_rows_w_;
+  if (key == 14) return svb_small_;
   if (key >= 100 && key < 100 + N_FRONT_CLASSES) return cls_group_[key - 100];
   if (key >= 110 && key < 110 + N_FRONT_CLASSES) return cls_tile_[key - 110];
   if (key >= 120 && key < 120 + N_FRONT_CLASSES) return cls_gkern_[key - 120];
@@ -1588,6 +1593,7 @@ void FrontSolver::build_block_solve_schedule(int method) {
   // tiles of one matrix chunk are adjacent so that they meet in L2.
   const bool chol = (method == UHM_CHOL);
   const int mode = chol ? (factor_nb_ == 64 ? 2 : 1) : 0;
+  const bool small = mode == 0 && svb_small_;
   const int ntile = (nrhs_ + 63) / 64;
   auto step = [&](SolveStep::Kind kind, int off, int L) {
     SolveStep st;
@@ -1599,6 +1605,20 @@ void FrontSolver::build_block_solve_schedule(int method) {
   };
   auto push = [&](int f, int r) {
     for (int t = 0; t < ntile; ++t) sitems_.push_back(uhmk_item{f, 64 * t, r, 0});
+  };
+  // LU: the first block step of the level's fronts with fs <= 32 (their
+  // only one) by one wavefront per item (uhmk_solve_block_step_small)
+  auto push_small = [&](int L, int fwd, std::vector<SolveStep>& list) {
+    const int o = (int)sitems_.size();
+    for (int f : levels_[L])
+      if (fr_[f].fs > 0 && fr_[f].fs <= 32) push(f, 0);
+    SolveStep st;
+    st.kind = SolveStep::BSMALL;
+    st.off = o;
+    st.cnt = (int)sitems_.size() - o;
+    st.prec = level_prec_[L];
+    st.fwd = fwd;
+    if (st.cnt) list.push_back(st);
   };
   // split-K for product launches with few items and long K (top levels):
   // enough workgroups to fill the chip, every slice at least 256 deep
@@ -1629,10 +1649,11 @@ void FrontSolver::build_block_solve_schedule(int method) {
     ga.cols = 1;
     sfwd_.push_back(ga);
     for (int k0 = 0; k0 < mfs[L]; k0 += 64) {
+      if (k0 == 0 && small) push_small(L, 1, sfwd_);
       o = (int)sitems_.size();
       for (int f : levels_[L]) {
         const int fs = fr_[f].fs;
-        if (fs <= k0) continue;
+        if (fs <= k0 || (k0 == 0 && small && fs <= 32)) continue;
         push(f, k0);
         for (int r = k0 + 64; r < fs; r += 64) push(f, r);
       }
@@ -1640,7 +1661,7 @@ void FrontSolver::build_block_solve_schedule(int method) {
       st.fwd = 1;
       st.k0 = k0;
       st.mode = mode;
-      sfwd_.push_back(st);
+      if (st.cnt) sfwd_.push_back(st);
     }
     o = (int)sitems_.size();
     for (int f : levels_[L]) {
@@ -1674,7 +1695,7 @@ void FrontSolver::build_block_solve_schedule(int method) {
     for (int k0 = ((mfs[L] - 1) / 64) * 64; k0 >= 0 && mfs[L] > 0; k0 -= 64) {
       o = (int)sitems_.size();
       for (int f : levels_[L]) {
-        if (fr_[f].fs <= k0) continue;
+        if (fr_[f].fs <= k0 || (k0 == 0 && small && fr_[f].fs <= 32)) continue;
         push(f, k0);
         for (int r = 0; r < k0; r += 64) push(f, r);
       }
@@ -1682,7 +1703,8 @@ void FrontSolver::build_block_solve_schedule(int method) {
       st.fwd = 0;
       st.k0 = k0;
       st.mode = mode;
-      sbwd_.push_back(st);
+      if (st.cnt) sbwd_.push_back(st);
+      if (k0 == 0 && small) push_small(L, 0, sbwd_);
     }
   }
   if (part_max > 0 && (size_t)part_max * sizeof(double) > d_part_.bytes) d_part_.alloc(part_max * sizeof(double));
@@ -1933,6 +1955,10 @@ rc = (st.cols ? uhmk_solve_gather_cols : uhmk_solve_gather)(
       case SolveStep::BRANCH:
         rc = (st.cols ? uhmk_solve_branch_cols : uhmk_solve_branch)(F, SI + st.off, st.cnt, d_emap_.as<int>(), X,
                                                                     nrhs_, stream_);
+        break;
+      case SolveStep::BSMALL:
+        rc = uhmk_solve_block_step_small(F, SI + st.off, st.cnt, st.fwd, A, st.prec, X, d_z_.as<double>(), nrhs_,
+                                         stream_);
         break;
       case SolveStep::BSTEP:
         rc = uhmk_solve_block_step(F, SI + st.off, st.cnt, st.mode, st.fwd, st.k0, A, st.prec, X, d_z_.as<double>(),

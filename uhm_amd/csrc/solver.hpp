@@ -84,7 +84,8 @@ inline int front_class(int fs) { return fs <= 64 ? 0 : (fs <= 512 ? 1 : 2); }
 struct SolveStep {
   // LEVEL: the Cholesky solves of a level in one launch (fwd as for CSTEP);
   // MARK: z marked as not yet solved and the error word cleared
-  enum Kind { GATHER, TRSV, GEMV, BRANCH, CSTEP, BSTEP, BGEMM, LEVEL, MARK } kind = GATHER;
+  // BSMALL: the LU block step of fronts with fs <= 32 (one wavefront per item)
+  enum Kind { GATHER, TRSV, GEMV, BRANCH, CSTEP, BSTEP, BGEMM, LEVEL, MARK, BSMALL } kind = GATHER;
   int off = 0, cnt = 0;            // items (gather / gemv / branch) or front list (trsv)
   int max_fs = 0, sb = 0, sb_size = 0;  // trsv
   int cols = 0;                    // gemv: row chunks (0) or column chunks (1); gather / branch:
@@ -274,6 +275,7 @@ class FrontSolver {
   int lu_panel_ll_ = 1;   // LU panels: left-looking register-block kernel (1) or the right-looking
                           // one (0); the same bits on FP64 fronts
   int lu_lookahead_ = 1;  // LU: the next panel on the lookahead stream beside the rest of the update
+  int svb_small_ = 1;     // LU block solves: fronts with fs <= 32 by one wavefront per item
   int lu_rows_w_ = 32;    // LU row kernel: columns (ABL rows) per workgroup (measured best of 16, 32, 64)
   int solve_level_ = 1;   // Cholesky solves with inverses: one launch per level and direction (1) or
                           // one per 64-row block step (0)
