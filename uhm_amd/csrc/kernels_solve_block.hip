@@ -32,6 +32,8 @@
 // entries are FP64 or FP32 (the level's storage); arithmetic is FP64.
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "dense_device.hpp"
 #include "uhmk.h"
 
@@ -673,10 +675,22 @@ __global__ __launch_bounds__(256) void k_svb_step(const uhmk_front* __restrict__
   sub_acc((FWD ? x : z) + fr.xoff, n, r, FWD ? fs : k0, c0, nrhs, acc, lane, w);
 }
 
+template <typename Fn, int... I>
+__device__ __forceinline__ void svb_static_for_impl(Fn&& fn, std::integer_sequence<int, I...>) {
+  (fn(std::integral_constant<int, I>{}), ...);
+}
+// fn(integral_constant<int, i>) for i = 0 .. N-1
+template <int N, typename Fn>
+__device__ __forceinline__ void svb_static_for(Fn&& fn) {
+  svb_static_for_impl(fn, std::make_integer_sequence<int, N>{});
+}
+
 // Block step of LU fronts with fs <= 32 at k0 = 0 (the item is the front's
 // own block): one wavefront per (front, 64-column tile of right-hand
 // sides); lane c solves column c0 + c in registers against the triangle
 // staged in LDS step-major (sT[k][i] = T(i, k), zero outside the triangle).
+// (A 64-entry version for fs <= 64 measured slower than the general step:
+// one wavefront per SIMD with 4096 FMAs each.)
 // Every written entry sees k_svb_step's TriStep operations in the same order
 // (forward, unit L: x_i = fma(-x_k, L(i, k), x_i) for every row at steps
 // k = 0 .. kb-1, zeros included; backward: x_k *= 1 / U(k, k), then the same
@@ -708,28 +722,18 @@ __global__ __launch_bounds__(64) void k_svb_small(const uhmk_front* __restrict__
   double xv[KM];
 #pragma unroll
   for (int k = 0; k < KM; ++k) xv[k] = uhmd::ldz(src, k, live && k < kb);
-  if constexpr (FWD) {
+  // every step's register indices at compile time (a rolled loop would
+  // index xv dynamically, which puts it in scratch memory)
+  svb_static_for<KM>([&](auto kc) {
+    constexpr int k = FWD ? decltype(kc)::value : KM - 1 - decltype(kc)::value;
+    if (k < kb) {
+      if constexpr (!FWD) xv[k] = xv[k] * rdg[k];
+      const double xk = xv[k];
+      const double* __restrict__ tk = sT + k * LS;
 #pragma unroll
-    for (int k = 0; k < KM; ++k) {
-      if (k < kb) {
-        const double xk = xv[k];
-        const double* __restrict__ tk = sT + k * LS;
-#pragma unroll
-        for (int i = 0; i < KM; ++i) xv[i] = fma(-xk, tk[i], xv[i]);
-      }
+      for (int i = 0; i < KM; ++i) xv[i] = fma(-xk, tk[i], xv[i]);
     }
-  } else {
-#pragma unroll
-    for (int k = KM - 1; k >= 0; --k) {
-      if (k < kb) {
-        xv[k] = xv[k] * rdg[k];
-        const double xk = xv[k];
-        const double* __restrict__ tk = sT + k * LS;
-#pragma unroll
-        for (int i = 0; i < KM; ++i) xv[i] = fma(-xk, tk[i], xv[i]);
-      }
-    }
-  }
+  });
   if (live) {
     double* __restrict__ dst = (FWD ? z : x) + col;
 #pragma unroll
