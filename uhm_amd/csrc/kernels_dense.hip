@@ -61,42 +61,63 @@ __device__ __forceinline__ d4 mma16(const double* X, const double* Y, int ld, d4
   return mma16_op<false>(X, ld, Y, ld, c, lane);
 }
 
-// Unblocked Cholesky of the 16 x 16 block at s (row stride ld) by lanes
-// 0..15 of one wavefront (lane i holds row i), then its inverse into w
-// (lane c forms column c).  first <- k + 1 of the first non-positive pivot.
-template <int K>
+// Unblocked Cholesky of a 16-column panel by one wavefront (lane i holds
+// row i of the panel in a[]).  WINV: with it W = L_BB^-1 of the panel's 16 x
+// 16 diagonal block -- lane c < 16 holds column c of W in wv[] (wv = e_c on
+// entry) and substitutes right-looking: at step K, W(K, c) *= 1 / L(K, K)
+// (the rsqrt the step computes anyway), then W(j, c) -= L(j, K) W(K, c) for
+// j > K with the L(j, K) the Cholesky update broadcasts.
+// first <- K + 1 of the first non-positive pivot.
+template <int K, bool WINV>
 struct Chol16Step {
-  __device__ __forceinline__ static void run(double (&a)[16], int i, int& first) {
+  __device__ __forceinline__ static void run(double (&a)[16], double (&wv)[16], int i, int& first) {
     const double d = readlane_d16<K>(a[K]);
     if (!(d > 0.0) && first == 0) first = K + 1;
     const double r = uhmd::rsqrt_d(d);
     a[K] = (i == K) ? d * r : (i > K ? a[K] * r : 0.0);
+    if (WINV) wv[K] = wv[K] * r;
 #pragma unroll
-    for (int j = K + 1; j < 16; ++j) a[j] = fma(-a[K], readlane_d16_rt(a[K], j), a[j]);
-    Chol16Step<K + 1>::run(a, i, first);
+    for (int j = K + 1; j < 16; ++j) {
+      const double ljk = readlane_d16_rt(a[K], j);
+      a[j] = fma(-a[K], ljk, a[j]);
+      if (WINV) wv[j] = fma(-ljk, wv[K], wv[j]);
+    }
+    if (WINV) {
+      // W's updates happen here, at the step: the empty asm makes each entry
+      // a value the step defines (sunk to the lanes that store W, they would
+      // keep every broadcast L(j, K) alive to the end, spilled)
+#pragma unroll
+      for (int j = K; j < 16; ++j) asm volatile("" : "+v"(wv[j]));
+    }
+    Chol16Step<K + 1, WINV>::run(a, wv, i, first);
   }
 };
-template <>
-struct Chol16Step<16> {
-  __device__ __forceinline__ static void run(double (&)[16], int, int&) {}
+template <bool WINV>
+struct Chol16Step<16, WINV> {
+  __device__ __forceinline__ static void run(double (&)[16], double (&)[16], int, int&) {}
 };
 
-
 // Cholesky of the kb x kb diagonal panel block and its inverse, one
-// workgroup of four wavefronts per front, by 16-column panels.  Panel step
-// B: wavefront 0 factors the panel rows [16 B, 64) x 16 columns, a row per
-// lane (v_readlane broadcast of the column entries of the diagonal block,
-// 1 / sqrt from v_rsq_f64 and two Newton steps), which gives the diagonal
-// block and the blocks below it in one pass; the trailing blocks are then
-// updated as 16 x 16 x 16 FP64 MFMA products spread over the wavefronts.
-// Afterwards the four wavefronts invert the four diagonal blocks at once,
-// and the blocks of W = L11^-1 below the diagonal follow diagonal by
-// diagonal, W(I,J) = -W(I,I) sum_{K=J}^{I-1} L(I,K) W(K,J).  W's strict
-// lower part is stored transposed in the strict upper triangle of the
-// diagonal block, which the Cholesky path never reads as matrix data (only
-// the lower triangle of a front is used); the panel TRSM and the solves
-// read it from there (its diagonal is 1 / L(k,k)).  Padding beyond kb is
-// an identity block, so it never produces a non-positive pivot.
+// workgroup of four wavefronts per front, by 16-column panels.  Step B:
+// wavefront 0 factors the panel rows [16 B, 64) x 16 columns, a row per lane
+// (v_readlane broadcast of the column entries of the diagonal block, 1 /
+// sqrt from v_rsq_f64 and two Newton steps), which gives the diagonal block
+// and the blocks below it in one pass, and stores the panel's L to the
+// front.  Meanwhile wavefront 1 inverts the previous panel's diagonal block
+// (lane c forms column c of W(B-1, B-1)), and wavefronts 2-3 apply the
+// previous panel's updates to the blocks right of panel B (deferred from
+// step B - 1) and form W's block row B - 2,
+// W(I, J) = -W(I, I) sum_{K=J}^{I-1} L(I, K) W(K, J).  Then panel B's
+// updates of panel B + 1's blocks, A(I, B+1) -= L(I, B) L(B+1, B)^T, as
+// 16 x 16 x 16 FP64 MFMA products on separate wavefronts.  The last panel
+// forms its own diagonal block's inverse in its column steps, so only W's
+// last two block rows follow the factorization.  Every block of L receives
+// its updates in panel order.  W's strict lower part is stored transposed in
+// the strict upper triangle of the diagonal block, which the Cholesky path
+// never reads as matrix data (only the lower triangle of a front is used);
+// the panel TRSM and the solves read it from there (its diagonal is
+// 1 / L(k,k)).  Padding beyond kb is an identity block, so it never produces
+// a non-positive pivot.
 // LDS of the diagonal-block factorization.
 struct DiagLds {
   double sL[TILE * LDW];
@@ -106,8 +127,9 @@ struct DiagLds {
 };
 
 // Factorization of the block held in S.sL (lower triangle of the kb x kb
-// block, identity beyond kb, zeros above the diagonal; S.sW zero,
-// S.s_first zero, all visible to the workgroup), then L and W back to the
+// block, identity beyond kb, zeros above the diagonal; S.s_first zero, all
+// visible to the workgroup; S.sW is written before it is read), then L and
+// W back to the
 // front at A (the block's first entry, column stride n) and the first
 // non-positive pivot (c0 + its index + 1) to info[f].  Called by all 256
 // threads of the workgroup.
@@ -131,87 +153,105 @@ __device__ __forceinline__ void chol_diag_lds(DiagLds& S, T* __restrict__ A, lon
   // 16-column blocks that hold the kb columns (the rest is identity
   // padding, which no real entry depends on)
   const int nblk = (kb + 15) >> 4, kbp = 16 * nblk;
+  // L(I, J) -= L(I, P) L(J, P)^T by one wavefront
+  auto tile_update = [&](int I, int J, int P) {
+    d4 c = d4{0.0, 0.0, 0.0, 0.0};
+    c = mma16_op<true>(sL + (16 * I) * LDW + 16 * P, LDW, sL + (16 * J) * LDW + 16 * P, LDW, c, lane);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sL[(16 * I + 4 * r + (lane >> 4)) * LDW + 16 * J + (lane & 15)] -= c[r];
+  };
+  // W(I, J) = -W(I, I) sum_{K=J}^{I-1} L(I, K) W(K, J), J < I, by one
+  // wavefront (its own staging sS[w])
+  auto w_block = [&](int I, int J) {
+    d4 sm = d4{0.0, 0.0, 0.0, 0.0};
+    for (int K = J; K < I; ++K) sm = mma16(sL + (16 * I) * LDW + 16 * K, sW + (16 * K) * LDW + 16 * J, LDW, sm, lane);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sS[w][(4 * r + (lane >> 4)) * 17 + (lane & 15)] = sm[r];
+    // the wavefront's own LDS stores complete before its lanes read them
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    d4 wb = d4{0.0, 0.0, 0.0, 0.0};
+    wb = mma16_ld(sW + (16 * I) * LDW + 16 * I, LDW, sS[w], 17, wb, lane);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sW[(16 * I + 4 * r + (lane >> 4)) * LDW + 16 * J + (lane & 15)] = -wb[r];
+  };
   for (int B = 0; B < nblk; ++B) {
     const int b0 = 16 * B;
-    // the 16-column panel [b0, 64) x [b0, b0 + 16): wavefront 0, lane l
-    // holds row b0 + l, so the pivots and the multipliers of the rows below
-    // the diagonal block come out of the same column steps
-    if (w == 0 && lane < kbp - b0) {
-      double a[16];
+    if (w == 0) {
+      if (lane < kbp - b0) {
+        double a[16], wv[16];
 #pragma unroll
-      for (int c = 0; c < 16; ++c) a[c] = (c <= lane) ? sL[(b0 + lane) * LDW + b0 + c] : 0.0;
-      int first = 0;
-      Chol16Step<0>::run(a, lane, first);
-      if (lane == 0 && first && b0 + first <= kb && s_first == 0) s_first = b0 + first;
+        for (int c = 0; c < 16; ++c) a[c] = (c <= lane) ? sL[(b0 + lane) * LDW + b0 + c] : 0.0;
+        int first = 0;
+        if (B + 1 < nblk) {
+          Chol16Step<0, false>::run(a, wv, lane, first);
+        } else {
 #pragma unroll
-      for (int c = 0; c < 16; ++c) sL[(b0 + lane) * LDW + b0 + c] = a[c];
-    }
-    __syncthreads();
-    // trailing blocks A(I,J) -= L(I,B) L(J,B)^T, B < J <= I
-    for (int t = w; t < 6; t += 4) {
-      // t -> (I, J) over the lower blocks right of column block B
-      int Ib = -1, Jb = -1, cnt = 0;
-      for (int J = B + 1; J < nblk; ++J)
-        for (int I2 = J; I2 < nblk; ++I2) {
-          if (cnt == t) {
-            Ib = I2;
-            Jb = J;
+          for (int c = 0; c < 16; ++c) wv[c] = (c == lane) ? 1.0 : 0.0;
+          Chol16Step<0, true>::run(a, wv, lane, first);
+          if (lane < 16) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) sW[(b0 + k) * LDW + b0 + lane] = wv[k];
           }
-          ++cnt;
         }
-      if (Ib < 0) continue;
-      d4 c = d4{0.0, 0.0, 0.0, 0.0};
-      c = mma16_op<true>(sL + (16 * Ib) * LDW + b0, LDW, sL + (16 * Jb) * LDW + b0, LDW, c, lane);
+        if (lane == 0 && first && b0 + first <= kb && s_first == 0) s_first = b0 + first;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        double* e = sL + (16 * Ib + 4 * r + (lane >> 4)) * LDW + 16 * Jb + (lane & 15);
-        *e -= c[r];
+        for (int c = 0; c < 16; ++c) sL[(b0 + lane) * LDW + b0 + c] = a[c];
+        const int row = b0 + lane;
+        if (row < kb) {
+#pragma unroll
+          for (int c = 0; c < 16; ++c)
+            if (c <= lane && b0 + c < kb) st_front<COH>(A + row + (long long)(b0 + c) * n, (T)a[c]);
+        }
       }
+    } else if (w == 1) {
+      // W(B-1, B-1): lane c forms column c; the 16 reciprocals of the
+      // diagonal come first (one division per lane, in parallel), so the
+      // substitution chain holds multiplications only
+      if (B >= 1 && lane < 16) {
+        const int p0 = b0 - 16;
+        const double rdl = 1.0 / sL[(p0 + lane) * LDW + p0 + lane];
+        double wc[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+          double acc = (k == lane) ? 1.0 : 0.0;
+#pragma unroll
+          for (int m = 0; m < k; ++m) acc = fma(-sL[(p0 + k) * LDW + p0 + m], wc[m], acc);
+          wc[k] = acc * readlane_d16_rt(rdl, k);
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) sW[(p0 + k) * LDW + p0 + lane] = wc[k];
+      }
+    } else {
+      // wavefronts 2-3: panel B - 1's updates of the blocks right of panel
+      // B, then W's block row B - 2
+      int item = 0;
+      if (B >= 1)
+        for (int J = B + 1; J < nblk; ++J)
+          for (int I = J; I < nblk; ++I, ++item)
+            if ((item & 1) == w - 2) tile_update(I, J, B - 1);
+      if (B >= 2)
+        for (int J = 0; J < B - 2; ++J, ++item)
+          if ((item & 1) == w - 2) w_block(B - 2, J);
     }
     __syncthreads();
-  }
-  // inverses of the four diagonal blocks, wavefront w block w, lane c
-  // forming column c; the 16 reciprocals of the diagonal come first (one
-  // division per lane, in parallel), so the substitution chain holds
-  // multiplications only
-  if (lane < 16 && w < nblk) {
-    const int b0 = 16 * w;
-    const double rdl = 1.0 / sL[(b0 + lane) * LDW + b0 + lane];
-    double wv[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      double acc = (k == lane) ? 1.0 : 0.0;
-#pragma unroll
-      for (int m = 0; m < k; ++m) acc = fma(-sL[(b0 + k) * LDW + b0 + m], wv[m], acc);
-      wv[k] = acc * readlane_d16_rt(rdl, k);
+    if (B + 1 < nblk) {
+      // panel B's updates of panel B + 1's blocks
+      for (int I = B + 1 + w; I < nblk; I += 4) tile_update(I, B + 1, B);
+      __syncthreads();
     }
-#pragma unroll
-    for (int k = 0; k < 16; ++k) sW[(b0 + k) * LDW + b0 + lane] = wv[k];
   }
+  // W's last two block rows
+  if (nblk >= 3) {
+    for (int J = w; J < nblk - 2; J += 4) w_block(nblk - 2, J);
+    __syncthreads();
+  }
+  for (int J = w; J < nblk - 1; J += 4) w_block(nblk - 1, J);
   __syncthreads();
-  for (int d = 1; d < nblk; ++d) {
-    // wavefront w forms block (I, J) = (w + d, w) of W when it exists
-    const int I = w + d, J = w;
-    d4 S = d4{0.0, 0.0, 0.0, 0.0};
-    if (I < nblk) {
-      for (int K = J; K < I; ++K) S = mma16(sL + (16 * I) * LDW + 16 * K, sW + (16 * K) * LDW + 16 * J, LDW, S, lane);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) sS[w][(4 * r + (lane >> 4)) * 17 + (lane & 15)] = S[r];
-    }
-    __syncthreads();
-    if (I < nblk) {
-      d4 Wb = d4{0.0, 0.0, 0.0, 0.0};
-      Wb = mma16_ld(sW + (16 * I) * LDW + 16 * I, LDW, sS[w], 17, Wb, lane);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) sW[(16 * I + 4 * r + (lane >> 4)) * LDW + 16 * J + (lane & 15)] = -Wb[r];
-    }
-    __syncthreads();
-  }
-  // L (lower) and the strict lower part of W, transposed, back to the front
+  // the strict lower part of W, transposed, to the front
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
     const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
-    if (r < kb && c < kb) st_front<COH>(A + r + (long long)c * n, (T)(c <= r ? sL[r * LDW + c] : sW[c * LDW + r]));
+    if (c > r && c < kb) st_front<COH>(A + r + (long long)c * n, (T)sW[c * LDW + r]);
   }
   if (tid == 0 && s_first && info[f] == 0) info[f] = c0 + s_first;
 }
@@ -237,7 +277,6 @@ __device__ __forceinline__ void chol_diag_front(DiagLds& S, const uhmk_front& fr
     for (int u = 0; u < 16; ++u) {
       const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
       S.sL[r * LDW + c] = (r >= kb && r == c) ? 1.0 : (c <= r ? v[u] : 0.0);
-      S.sW[r * LDW + c] = 0.0;
     }
   }
   if (tid == 0) S.s_first = 0;
@@ -563,7 +602,6 @@ __device__ __forceinline__ void gemm_tile(const uhmk_front& fr, const uhmk_item 
     for (int u = 0; u < 16; ++u) {
       const int idx = tid + 256 * u, r = idx & 63, c = idx >> 6;
       S.sL[r * LDW + c] = (r >= kb && r == c) ? 1.0 : 0.0;
-      S.sW[r * LDW + c] = 0.0;
     }
     if (tid == 0) S.s_first = 0;
     __syncthreads();
@@ -841,6 +879,14 @@ int launch_gemm(const uhmk_front* fronts, const uhmk_item* items, int n_items, i
 // between.  TS storage, TM MFMA type of the updates (as k_gemm); the
 // diagonal block and the TRSM run in FP64.
 // ------------------------------------------------------------------------
+// Workgroup barrier that orders LDS only: global loads in flight stay in
+// flight across it (__syncthreads waits for them)
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 constexpr int GROUP_MAX_PANELS = 16;
 constexpr int SPIN_LIMIT = 1 << 22;
 
@@ -962,6 +1008,24 @@ __global__ __launch_bounds__(256, 2) void k_chol_group(const uhmk_front* __restr
     // own diagonal block as each L(r, p) appears, so that block needs no
     // update of its own once the last TRSM is done
     const int nsolve = DIAGT ? r : P;
+    // the own diagonal block's entries as the earlier launches left them
+    // (nothing else in this launch writes them): loaded before the last
+    // wait for W, so the loads complete while the tile waits
+    TS mv[NB][NB][4];
+    auto load_mv = [&]() {
+      const int dc0 = g0 + T * r, dkb = min(T, ge - dc0);
+#pragma unroll
+      for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          const int ii = iw + a * 16 + cl;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int jj = jw + b * 16 + Mfma<TM>::col(q, lane);
+            mv[a][b][q] = M[ii < dkb && jj <= ii ? (dc0 + ii) + (long long)(dc0 + jj) * n : dc0 + (long long)dc0 * n];
+          }
+        }
+    };
     acc_t accd[NB][NB];
 #pragma unroll
     for (int a = 0; a < NB; ++a)
@@ -1084,29 +1148,16 @@ __global__ __launch_bounds__(256, 2) void k_chol_group(const uhmk_front* __restr
     // reload would, identity beyond kb) through chol_diag_lds
     {
       const int pc0 = g0 + T * r, kb = min(T, ge - pc0);
-      // the block's entries as the earlier launches left them, all loads
-      // in flight at once
-      TS mv[NB][NB][4];
-#pragma unroll
-      for (int a = 0; a < NB; ++a)
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-          const int ii = iw + a * 16 + cl;
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int jj = jw + b * 16 + Mfma<TM>::col(q, lane);
-            mv[a][b][q] = M[ii < kb && jj <= ii ? (pc0 + ii) + (long long)(pc0 + jj) * n : pc0 + (long long)pc0 * n];
-          }
-        }
-      __syncthreads();  // sX read by every wavefront
+      // the loads stay in flight across the LDS set-up
+      load_mv();
+      lds_sync();  // sX read by every wavefront
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
         const int idx = tid + 256 * u, rr = idx & 63, cc = idx >> 6;
         S.sL[rr * LDW + cc] = (rr >= kb && rr == cc) ? 1.0 : 0.0;
-        S.sW[rr * LDW + cc] = 0.0;
       }
       if (tid == 0) S.s_first = 0;
-      __syncthreads();
+      lds_sync();
 #pragma unroll
       for (int a = 0; a < NB; ++a)
 #pragma unroll
@@ -1118,7 +1169,7 @@ __global__ __launch_bounds__(256, 2) void k_chol_group(const uhmk_front* __restr
             if (ii < kb && jj <= ii) S.sL[ii * LDW + jj] = (double)(TS)((double)mv[a][b][q] - (double)accd[a][b][q]);
           }
         }
-      __syncthreads();
+      lds_sync();
       chol_diag_lds<TS, true>(S, M + pc0 + (long long)pc0 * n, n, kb, pc0, info, it.f);
       group_publish(fl + r, r + 1);
     }
