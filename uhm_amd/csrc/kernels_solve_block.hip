@@ -632,15 +632,11 @@ __global__ __launch_bounds__(256) void k_svb_step(const uhmk_front* __restrict__
       const int e = 16 * q + j, k = FWD ? e : 63 - e;
       xv[j] = sX[k * LD + c];
     }
-    if constexpr (FWD && MODE == 0) {
-      if (kb == 64) uhmd::TriStep<0, true, false, true>::run(xv, sT, LD, rdiag, kb, q);
-      else uhmd::TriStep<0, true, false, false>::run(xv, sT, LD, rdiag, kb, q);
-    } else if constexpr (FWD) {
-      if (kb == 64) uhmd::TriStep<0, false, false, true>::run(xv, sT, LD, rdiag, kb, q);
-      else uhmd::TriStep<0, false, false, false>::run(xv, sT, LD, rdiag, kb, q);
-    } else {
-      uhmd::TriStep<0, false, false, true>::run(xv, sT, LD, rdiag, 64, q);
-    }
+    // all 64 steps whatever kb: beyond kb the triangle holds +0, rdiag 1 and
+    // x +0, so those steps leave every entry's bits as they are, and without
+    // a test per step the LDS reads of later steps issue early
+    if constexpr (FWD && MODE == 0) uhmd::TriStep<0, true, false, true>::run(xv, sT, LD, rdiag, 64, q);
+    else uhmd::TriStep<0, false, false, true>::run(xv, sT, LD, rdiag, 64, q);
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const int e = 16 * q + j, k = FWD ? e : 63 - e;
@@ -723,15 +719,32 @@ __global__ __launch_bounds__(64) void k_svb_small(const uhmk_front* __restrict__
 #pragma unroll
   for (int k = 0; k < KM; ++k) xv[k] = uhmd::ldz(src, k, live && k < kb);
   // every step's register indices at compile time (a rolled loop would
-  // index xv dynamically, which puts it in scratch memory)
+  // index xv dynamically, which puts it in scratch memory); all KM steps
+  // whatever kb (beyond kb the triangle holds +0, rdg 1 and x +0: those
+  // steps leave every entry's bits as they are), each step reading the next
+  // step's triangle row into registers before its own FMAs.  Each step sits
+  // behind a test the compiler cannot resolve (always true), which keeps the
+  // steps apart: as one block, the 1024 FMAs and loads get scheduled
+  // together and spill.
+  int km = KM;
+  asm volatile("" : "+s"(km));
+  double tc[KM], tn[KM];
+#pragma unroll
+  for (int i = 0; i < KM; ++i) tn[i] = sT[(FWD ? 0 : KM - 1) * LS + i];
   svb_static_for<KM>([&](auto kc) {
-    constexpr int k = FWD ? decltype(kc)::value : KM - 1 - decltype(kc)::value;
-    if (k < kb) {
+    constexpr int s_ = decltype(kc)::value;
+    constexpr int k = FWD ? s_ : KM - 1 - s_;
+    if (s_ < km) {
+#pragma unroll
+      for (int i = 0; i < KM; ++i) tc[i] = tn[i];
+      if constexpr (s_ + 1 < KM) {
+#pragma unroll
+        for (int i = 0; i < KM; ++i) tn[i] = sT[(FWD ? k + 1 : k - 1) * LS + i];
+      }
       if constexpr (!FWD) xv[k] = xv[k] * rdg[k];
       const double xk = xv[k];
-      const double* __restrict__ tk = sT + k * LS;
 #pragma unroll
-      for (int i = 0; i < KM; ++i) xv[i] = fma(-xk, tk[i], xv[i]);
+      for (int i = 0; i < KM; ++i) xv[i] = fma(-xk, tc[i], xv[i]);
     }
   });
   if (live) {
