@@ -459,10 +459,15 @@ constexpr size_t gemm_lds_bytes() {
 
 // One tile item of the update (the k_gemm computation), smem: LDS of
 // gemm_lds_bytes<T, TM, DIAG>() bytes, 16-byte aligned.
-template <bool TRANSB, int T, typename TS, typename TM, bool DIAG>
+// OS: storage type of the A / B operands.  With P non-null (Schur products,
+// OS = TM) they are read from P, a column-major copy of ABL (row i of the
+// front at P[i - fs], leading dimension ldp, a multiple of 4 from a
+// 16-byte-aligned base) instead of the front; C is the front either way.
+template <bool TRANSB, int T, typename TS, typename TM, bool DIAG, typename OS = TS>
 __device__ __forceinline__ void gemm_tile(const uhmk_front& fr, const uhmk_item it, int c0, int nb, int schur,
                                           TS* __restrict__ arena, int* __restrict__ info,
-                                          unsigned char* __restrict__ smem) {
+                                          unsigned char* __restrict__ smem, const OS* __restrict__ P = nullptr,
+                                          long long ldp = 0) {
   typedef typename Mfma<TM>::acc_t acc_t;
   constexpr int LD = T + 16;          // row stride of the LDS panels (conflict-free half-wave reads)
   constexpr int NB = T / 32;          // 16x16 blocks per wave and dimension
@@ -480,6 +485,9 @@ __device__ __forceinline__ void gemm_tile(const uhmk_front& fr, const uhmk_item 
   const int j1 = min(j0 + T, j0 < fs ? fs : n);
   TS* __restrict__ M = arena + fr.off;
   static_assert(!DIAG || (TRANSB && T == TILE), "fused diagonal blocks: Cholesky updates on 64-wide tiles");
+  const OS* __restrict__ Po = P ? P : reinterpret_cast<const OS*>(M);
+  const long long ldo = P ? ldp : (long long)n;
+  const int ro = P ? fs : 0;
   TM* __restrict__ sA = reinterpret_cast<TM*>(smem);
   TM* __restrict__ sB = sA + KC * LD;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -501,15 +509,16 @@ __device__ __forceinline__ void gemm_tile(const uhmk_front& fr, const uhmk_item 
   // t rows VW (t % RG) .. +VW-1 of k rows t / RG + VKS u, written to LDS in
   // one ds_write (FP64 storage with FP32 products measured slower this way
   // and keeps the element loads)
-  constexpr bool VEC = sizeof(TS) == sizeof(TM);
-  constexpr int VW = Vec16<TS>::N, RG = T / VW, VKS = 256 / RG, NV = KC / VKS;
-  typedef typename Vec16<TS>::t vs_t;
-  typedef typename Vec16<TS>::u vsu_t;
+  constexpr bool VEC = sizeof(OS) == sizeof(TM);
+  static_assert(VEC || std::is_same<OS, TS>::value, "a separate operand copy is read with 16-byte loads");
+  constexpr int VW = Vec16<OS>::N, RG = T / VW, VKS = 256 / RG, NV = KC / VKS;
+  typedef typename Vec16<OS>::t vs_t;
+  typedef typename Vec16<OS>::u vsu_t;
   typedef typename VecM<TM, VW>::t vm_t;
   const int vr = VW * (tid % RG), vk = tid / RG;
   vm_t va[VEC ? NV : 1], vb[VEC ? NV : 1];
   auto ldv = [&](int r0, int rend, int k) -> vm_t {
-    const TS* p = M + r0 + (long long)k * n;
+    const OS* p = Po + (r0 - ro) + (long long)k * ldo;
     vm_t v;
     if (k < kend && r0 + VW - 1 < rend) {
       const vs_t x = *reinterpret_cast<const vsu_t*>(p);
@@ -1237,6 +1246,55 @@ __global__ __launch_bounds__(256, 2) void k_chol_front_small(const uhmk_front* _
       gemm_tile<true, TILE, TS, TM, false>(fr, uhmk_item{f, i0, j0, 0}, 0, TILE, 1, arena, nullptr, smem);
 }
 
+// ------------------------------------------------------------------------
+// Schur products of FP64 fronts with FP32 MFMA on an FP32 copy of the
+// operand (Cholesky, ABR -= ABL ABL^T; the reference's syrk,
+// UHM/src/matrix/uhm/fla/chol/decompose.cxx:95-96, LINAL/src/dense/
+// syrk.cxx:43-123).  k_p32_pack rounds ABL to FP32 once per front into a
+// column-major copy (ldp = ss rounded up to 4, padding rows zero; items
+// {f, 64-row tile, 64-column tile, 0}); k_schur_p32 then runs the tiles of
+// gemm_tile on it with 16-byte loads.  The operands reach LDS as the same
+// FP32 values as the element loads of the front give (one rounding of each
+// entry), so C is the same bits as k_gemm's on the front itself.
+// ------------------------------------------------------------------------
+__host__ __device__ inline long long p32_ld(int n, int fs) { return ((long long)(n - fs) + 3) & ~3LL; }
+
+template <typename TS>
+__global__ __launch_bounds__(256) void k_p32_pack(const uhmk_front* __restrict__ F,
+                                                  const uhmk_item* __restrict__ items, const TS* __restrict__ arena,
+                                                  const long long* __restrict__ off, float* __restrict__ P) {
+  const uhmk_item it = items[blockIdx.x];
+  const uhmk_front fr = F[it.f];
+  const int n = fr.n, fs = fr.fs, ss = n - fs;
+  const long long ldp = p32_ld(n, fs);
+  const TS* __restrict__ M = arena + fr.off;
+  float* __restrict__ D = P + off[it.f];
+  const int tid = threadIdx.x, r = tid & 63;
+  const long long row = 64LL * it.a + r;
+  float v[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int k = 64 * it.b + (tid >> 6) + 4 * u;
+    v[u] = (row < ss && k < fs) ? (float)M[(fs + row) + (long long)k * n] : 0.0f;
+  }
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int k = 64 * it.b + (tid >> 6) + 4 * u;
+    if (row < ldp && k < fs) D[row + (long long)k * ldp] = v[u];
+  }
+}
+
+template <typename TS>
+__global__ __launch_bounds__(256, 2) void k_schur_p32(const uhmk_front* __restrict__ F,
+                                                   const uhmk_item* __restrict__ items, TS* __restrict__ arena,
+                                                   const long long* __restrict__ off, const float* __restrict__ P) {
+  __shared__ __align__(16) unsigned char smem[gemm_lds_bytes<64, float, false>()];
+  const uhmk_item it = items[uhmd::xcd_item(blockIdx.x, gridDim.x)];
+  const uhmk_front fr = F[it.f];
+  gemm_tile<true, 64, TS, float, false, float>(fr, it, 0, 0, 1, arena, nullptr, smem, P + off[it.f],
+                                               p32_ld(fr.n, fr.fs));
+}
+
 template <typename TS, typename TM>
 int launch_chol_group(const uhmk_front* fronts, const uhmk_item* items, int n_items, int g0, int gw, TS* arena,
                       int* flags, int* info, hipStream_t stream) {
@@ -1359,6 +1417,30 @@ int uhmk_gemm_update(const uhmk_front* fronts, const uhmk_item* items, int n_ite
   else
     return -1006;
   if (rc) return rc;
+  return ret(hipGetLastError());
+}
+
+
+long long uhmk_schur_p32_size(int n, int fs) {
+  if (fs <= 0 || n <= fs) return 0;
+  return p32_ld(n, fs) * fs;
+}
+
+int uhmk_schur_p32_pack(const uhmk_front* fronts, const uhmk_item* items, int n_items, const void* arena, int prec,
+                        const long long* off, float* P, hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  if (prec != 0) return -1006;
+  hipLaunchKernelGGL(k_p32_pack<double>, dim3(n_items), dim3(256), 0, stream, fronts, items,
+                     static_cast<const double*>(arena), off, P);
+  return ret(hipGetLastError());
+}
+
+int uhmk_schur_p32(const uhmk_front* fronts, const uhmk_item* items, int n_items, void* arena, int prec,
+                   const long long* off, const float* P, hipStream_t stream) {
+  if (n_items <= 0) return 0;
+  if (prec != 0) return -1006;
+  hipLaunchKernelGGL(k_schur_p32<double>, dim3(n_items), dim3(256), 0, stream, fronts, items,
+                     static_cast<double*>(arena), off, P);
   return ret(hipGetLastError());
 }
 

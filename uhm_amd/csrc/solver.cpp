@@ -483,6 +483,7 @@ void FrontSolver::build_schedule(int method) {
     for (int f : levels_[L])
       if (reuse_.empty() || !reuse_[f]) flv_[L].push_back(f);
   std::vector<char> lp_front(fr_.size(), 0);  // fronts with low-precision Schur products
+  std::vector<char> p32_front(fr_.size(), 0);  // fronts with FP32 Schur products on an FP32 copy of ABL
   for (int L = 0; L < (int)levels_.size(); ++L) {
     const auto& lv = flv_[L];
     if (chol && nb == 64 && small_fused_ > 0 && (int)lv.size() >= small_fused_) {
@@ -980,6 +981,26 @@ void FrontSolver::build_schedule(int method) {
       // (eight workgroups per CU hide their load latency)
       ls.tile = cnt && !lowp ? gemm_tile_for(sss / cnt, sss / cnt, tile_threshold) : 64;
     }
+    // FP32 products on FP64 fronts (Cholesky): the operand rounded to FP32
+    // once per front, then read with 16-byte loads (element loads of FP64
+    // entries were the slow part of these products)
+    const bool p32 = !lowp && chol && schur_p32_ && prec_mode_ >= 1 && level_prec_[L] == 0 && ls.tile == 64;
+    if (p32) {
+      ls.kind = Launch::SCHUR_P32;
+      Launch pk;
+      pk.kind = Launch::P32_PACK;
+      pk.level = L;
+      pk.off = (long long)items_.size();
+      for (int f : big) {
+        const int fs = fr_[f].fs, ss = fr_[f].n - fs;
+        if (!fs || !ss) continue;
+        p32_front[f] = 1;
+        for (int a = 0; a < (ss + 63) / 64; ++a)
+          for (int b = 0; b < (fs + 63) / 64; ++b) items_.push_back(uhmk_item{f, a, b, 0});
+      }
+      pk.count = (int)(items_.size() - pk.off);
+      if (pk.count) push_main(pk);
+    }
     if (lowp) {
       // pack the operands of every front of the level (64 x 64 tiles over
       // the padded packed rows and K)
@@ -1130,15 +1151,26 @@ void FrontSolver::build_schedule(int method) {
           by += 16.0 * n * n;
         }
         break;
+      case Launch::P32_PACK:
+        for (int q = 0; q < l.count; ++q) {
+          const uhmk_item& it = items_[l.off + q];
+          const int fs = fr_[it.f].fs, ss = fr_[it.f].n - fs;
+          const double rows = std::max(0, std::min(64, ss - 64 * it.a)), ks = std::max(0, std::min(64, fs - 64 * it.b));
+          by += 12.0 * rows * ks;  // FP64 read, FP32 write
+        }
+        break;
       case Launch::GEMM:
       case Launch::GEMM_DIAG:
       case Launch::SCHUR:
       case Launch::SCHUR_LP:
+      case Launch::SCHUR_P32:
         for (int q = 0; q < l.count; ++q) {
           const uhmk_item& it = items_[l.off + q];
           const int fs = fr_[it.f].fs, n = fr_[it.f].n;
           if (l.kind == Launch::GEMM_DIAG && it.c == 1) fl += flop_chol(std::min(l.nb, fs - it.a));
-          const double K = (l.kind == Launch::SCHUR || l.kind == Launch::SCHUR_LP) ? fs : std::min(l.c0 + l.nb, fs) - l.c0;
+          const double K = (l.kind == Launch::SCHUR || l.kind == Launch::SCHUR_LP || l.kind == Launch::SCHUR_P32)
+                               ? fs
+                               : std::min(l.c0 + l.nb, fs) - l.c0;
           const int i1 = tile_end(it.a, fs, n, l.tile), j1 = tile_end(it.b, fs, n, l.tile);
           double cnt = 0;
           if (chol)
@@ -1169,6 +1201,21 @@ void FrontSolver::build_schedule(int method) {
     d_lp_.reserve(std::max(1LL, tot) * sizeof(unsigned short));
     d_lpoff_.upload(lpoff, stream_);
     d_lpamax_.reserve(std::max<size_t>(1, 2 * fr_.size()) * sizeof(unsigned));
+  }
+  {
+    // FP32 copies of ABL for the SCHUR_P32 launches (offsets multiples of
+    // 64 entries: 16-byte aligned)
+    std::vector<long long> poff(fr_.size(), 0);
+    long long tot = 0;
+    for (size_t f = 0; f < fr_.size(); ++f)
+      if (p32_front[f]) {
+        poff[f] = tot;
+        tot += (uhmk_schur_p32_size(fr_[f].n, fr_[f].fs) + 63) / 64 * 64;
+      }
+    if (tot) {
+      d_p32_.reserve((size_t)tot * sizeof(float));
+      d_p32off_.upload(poff, stream_);
+    }
   }
   check_hip(hipStreamSynchronize(stream_), "schedule upload");
   if (factor_graph_) (void)hipGraphExecDestroy(factor_graph_);
@@ -1241,6 +1288,12 @@ void FrontSolver::enqueue_factor(int method, std::vector<hipEvent_t>* pev) {
       case Launch::LP_PACK:
         rc = uhmk_schur_lp_pack(F, IT + l.off, l.count, prec_mode_ == 3, A, l.prec, d_lpoff_.as<long long>(),
                                 d_lpamax_.as<unsigned>(), d_lp_.as<unsigned short>(), st);
+        break;
+      case Launch::P32_PACK:
+        rc = uhmk_schur_p32_pack(F, IT + l.off, l.count, A, l.prec, d_p32off_.as<long long>(), d_p32_.as<float>(), st);
+        break;
+      case Launch::SCHUR_P32:
+        rc = uhmk_schur_p32(F, IT + l.off, l.count, A, l.prec, d_p32off_.as<long long>(), d_p32_.as<float>(), st);
         break;
       case Launch::CHOL_GROUP:
         rc = uhmk_chol_group(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, l.mma, d_flags_.as<int>(), info, st);
@@ -1322,6 +1375,10 @@ void FrontSolver::set_tuning(int key, int value) {
     if (value < -1 || value > 2) throw std::runtime_error("set_tuning: class group launches take -1 (global), 0, 1 or 2");
     if (value != cls_gkern_[key - 120]) schedule_method_ = 0;
     cls_gkern_[key - 120] = value;
+  } else if (key == 15) {
+    if (value != 0 && value != 1) throw std::runtime_error("set_tuning: key 15 (FP32 Schur operand copies) is 0 or 1");
+    if (value != schur_p32_) schedule_method_ = 0;
+    schur_p32_ = value;
   } else if (key == 14) {
     if (value != 0 && value != 1) throw std::runtime_error("set_tuning: key 14 (small-front LU block steps) is 0 or 1");
     if (value != svb_small_) solve_method_ = 0;
@@ -1381,6 +1438,7 @@ int FrontSolver::get_tuning(int key) const {
   if (key == 12) return lu_lookahead_;
   if (key == 13) return lu_rows_w_;
   if (key == 14) return svb_small_;
+  if (key == 15) return schur_p32_;
   if (key >= 100 && key < 100 + N_FRONT_CLASSES) return cls_group_[key - 100];
   if (key >= 110 && key < 110 + N_FRONT_CLASSES) return cls_tile_[key - 110];
   if (key >= 120 && key < 120 + N_FRONT_CLASSES) return cls_gkern_[key - 120];
