@@ -172,10 +172,11 @@ void uhm_get_timing_(uhm_fort_p* mesh, uhm_fort_double* factor_ms, uhm_fort_doub
  * persistent launch), lp_pack (Schur operands packed to BF16 / FP16),
  * chol_small (assembly and whole Cholesky factorization of the small
  * fronts of a level in one launch), schur_lp (final Schur complement on
- * packed BF16 / FP16 operands), p32_pack (Schur operands of FP64 fronts
- * copied to FP32), schur_p32 (final Schur complement of FP64 fronts with
- * FP32 products on those copies): milliseconds, algorithmic flops,
- * algorithmic bytes and launch counts (arrays of 15). */
+ * packed BF16 / FP16 operands), p32_pack (update operands of FP64 fronts
+ * copied to FP32), schur_p32 / gemm_p32 (final Schur complement / panel
+ * and trailing updates of FP64 fronts with FP32 products on those copies):
+ * milliseconds, algorithmic flops, algorithmic bytes and launch counts
+ * (arrays of 16). */
 void uhm_set_profile_(uhm_fort_p* mesh, uhm_fort_int* on);
 void uhm_get_kernel_stats_(uhm_fort_p* mesh, uhm_fort_double* ms, uhm_fort_double* flops, uhm_fort_double* bytes,
                            uhm_fort_int* launches);

@@ -226,24 +226,26 @@ int uhmk_schur_lp(const uhmk_front* fronts, const uhmk_item* items, int n_items,
                   void* arena, int prec, const long long* off, const unsigned* amax, const unsigned short* lp,
                   hipStream_t stream);
 
-/* Cholesky Schur complement products of FP64 fronts with FP32 MFMA on an
- * FP32 copy of the operand (ABR -= ABL ABL^T of chol_hier,
- * UHM/src/matrix/uhm/fla/chol/decompose.cxx:95-96, LINAL/src/dense/
- * syrk.cxx:43-123): ABL of front f rounded to FP32 (nearest even, once) at
- * element off[f] of P, column major, leading dimension ss rounded up to 4
- * (padding rows zero); uhmk_schur_p32_size gives the entries a front needs,
- * off[f] must be a multiple of 4.  The products read the copy with 16-byte
- * loads; C (the FP64 front) is the same bits as uhmk_gemm_update's Schur
- * update with mma = 1 on the front itself.  prec must be 0 (-1006
- * otherwise).
- *   uhmk_schur_p32_pack  items {f, 64-row tile of ABL, 64-column tile, 0}
- *   uhmk_schur_p32       items {f, i0, j0, 0}: 64 x 64 tiles of ABR (front
- *                        indices, tiles cut at n, lower ones only) */
-long long uhmk_schur_p32_size(int n, int fs);
-int uhmk_schur_p32_pack(const uhmk_front* fronts, const uhmk_item* items, int n_items, const void* arena, int prec,
-                        const long long* off, float* P, hipStream_t stream);
-int uhmk_schur_p32(const uhmk_front* fronts, const uhmk_item* items, int n_items, void* arena, int prec,
-                   const long long* off, const float* P, hipStream_t stream);
+/* Cholesky updates of FP64 fronts with FP32 MFMA on an FP32 copy of their
+ * operand: trailing updates C -= L(:, [c0, c1)) L(:, [c0, c1))^T on the rows
+ * [c1, n), c1 = min(c0 + nb, fs) (LINAL/src/dense/chol.cxx:86-90), or with
+ * schur != 0 the final Schur complement ABR -= ABL ABL^T (c0 = 0, c1 = fs;
+ * UHM/src/matrix/uhm/fla/chol/decompose.cxx:95-96).  The operand, rows
+ * [c1, n) of the columns [c0, c1), rounded to FP32 (nearest even, once)
+ * into a column-major copy at element 64 * item.c of P (leading dimension
+ * n - c1 rounded up to 4, padding rows zero); uhmk_p32_size gives the
+ * entries a front needs.  The products read the copy with 16-byte loads; C
+ * (the FP64 front) is the same bits as uhmk_gemm_update's NT update with
+ * mma = 1 on the front itself.  prec must be 0 (-1006 otherwise).
+ *   uhmk_p32_pack         items {f, 64-row tile of the copy, 64-column
+ *                         tile, offset / 64}
+ *   uhmk_gemm_update_p32  items {f, i0, j0, offset / 64}: 64 x 64 tiles of
+ *                         C (front indices, lower ones only) */
+long long uhmk_p32_size(int n, int fs, int c0, int nb, int schur);
+int uhmk_p32_pack(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, int schur,
+                  const void* arena, int prec, float* P, hipStream_t stream);
+int uhmk_gemm_update_p32(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, int schur,
+                         void* arena, int prec, const float* P, hipStream_t stream);
 
 /* ---- solve ---------------------------------------------------------------- */
 

@@ -1,6 +1,7 @@
-"""The Schur products of FP64 fronts with FP32 MFMA on an FP32 copy of the
-operand (tuning key 15 = 1, the default: uhmk_schur_p32_pack +
-uhmk_schur_p32, the final syrk of chol_hier,
+"""The Cholesky updates of FP64 fronts with FP32 MFMA on an FP32 copy of
+their operand (tuning key 15 = 1, the default: uhmk_p32_pack +
+uhmk_gemm_update_p32 for the trailing updates after each panel group and
+the final syrk of chol_hier, LINAL/src/dense/chol.cxx:86-90,
 UHM/src/matrix/uhm/fla/chol/decompose.cxx:95-96) against the same products
 on element loads of the front (key 15 = 0).
 
@@ -49,8 +50,10 @@ def test_schur_p32_same_bits(lib):
     assert m.chol_with_free() == 0
     st = m.kernel_stats()
     m.set_profile(False)
-    say("  p32 launches: pack %d, products %d" % (st["p32_pack"][3], st["schur_p32"][3]))
-    assert st["schur_p32"][3] > 0 and st["p32_pack"][3] > 0
+    say("  p32 launches: pack %d, Schur products %d, trailing updates %d" % (st["p32_pack"][3], st["schur_p32"][3],
+                                                                          st["gemm_p32"][3]))
+    assert st["schur_p32"][3] > 0 and st["gemm_p32"][3] > 0
+    assert st["p32_pack"][3] == st["schur_p32"][3] + st["gemm_p32"][3]
     for f in range(m.n_fronts()):
         a, b = F0[f], m.fetch_front(f)
         fs = m.front_info(f)["fs"]

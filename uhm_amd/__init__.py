@@ -94,7 +94,8 @@ _SIGS = {
 }
 
 KERNEL_KINDS = ("assemble", "lu_panel", "lu_rows", "chol_diag", "chol_trsm", "gemm", "schur", "lu_front",
-                "gemm_diag", "chol_group", "lp_pack", "chol_small", "schur_lp", "p32_pack", "schur_p32")
+                "gemm_diag", "chol_group", "lp_pack", "chol_small", "schur_lp", "p32_pack", "schur_p32",
+                "gemm_p32")
 _RET = {"uhm_get_last_error_": C.c_int}
 
 

@@ -9,6 +9,7 @@
 //                                      lu_piv/decompose.cxx:148-149, chol/decompose.cxx:95-96
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "assemble_device.hpp"
@@ -459,10 +460,11 @@ constexpr size_t gemm_lds_bytes() {
 
 // One tile item of the update (the k_gemm computation), smem: LDS of
 // gemm_lds_bytes<T, TM, DIAG>() bytes, 16-byte aligned.
-// OS: storage type of the A / B operands.  With P non-null (Schur products,
-// OS = TM) they are read from P, a column-major copy of ABL (row i of the
-// front at P[i - fs], leading dimension ldp, a multiple of 4 from a
-// 16-byte-aligned base) instead of the front; C is the front either way.
+// OS: storage type of the A / B operands.  With P non-null (OS = TM, NT
+// products) they are read from P, a column-major copy of the update's
+// operand -- rows [kend, n) of the front's columns [kbeg, kend), entry (i,
+// k) at P[(i - kend) + (k - kbeg) ldp], ldp a multiple of 4 from a
+// 16-byte-aligned base -- instead of the front; C is the front either way.
 template <bool TRANSB, int T, typename TS, typename TM, bool DIAG, typename OS = TS>
 __device__ __forceinline__ void gemm_tile(const uhmk_front& fr, const uhmk_item it, int c0, int nb, int schur,
                                           TS* __restrict__ arena, int* __restrict__ info,
@@ -487,7 +489,7 @@ __device__ __forceinline__ void gemm_tile(const uhmk_front& fr, const uhmk_item 
   static_assert(!DIAG || (TRANSB && T == TILE), "fused diagonal blocks: Cholesky updates on 64-wide tiles");
   const OS* __restrict__ Po = P ? P : reinterpret_cast<const OS*>(M);
   const long long ldo = P ? ldp : (long long)n;
-  const int ro = P ? fs : 0;
+  const int ro = P ? kend : 0, ko = P ? kbeg : 0;
   TM* __restrict__ sA = reinterpret_cast<TM*>(smem);
   TM* __restrict__ sB = sA + KC * LD;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -518,7 +520,7 @@ __device__ __forceinline__ void gemm_tile(const uhmk_front& fr, const uhmk_item 
   const int vr = VW * (tid % RG), vk = tid / RG;
   vm_t va[VEC ? NV : 1], vb[VEC ? NV : 1];
   auto ldv = [&](int r0, int rend, int k) -> vm_t {
-    const OS* p = Po + (r0 - ro) + (long long)k * ldo;
+    const OS* p = Po + (r0 - ro) + (long long)(k - ko) * ldo;
     vm_t v;
     if (k < kend && r0 + VW - 1 < rend) {
       const vs_t x = *reinterpret_cast<const vsu_t*>(p);
@@ -1247,52 +1249,58 @@ __global__ __launch_bounds__(256, 2) void k_chol_front_small(const uhmk_front* _
 }
 
 // ------------------------------------------------------------------------
-// Schur products of FP64 fronts with FP32 MFMA on an FP32 copy of the
-// operand (Cholesky, ABR -= ABL ABL^T; the reference's syrk,
-// UHM/src/matrix/uhm/fla/chol/decompose.cxx:95-96, LINAL/src/dense/
-// syrk.cxx:43-123).  k_p32_pack rounds ABL to FP32 once per front into a
-// column-major copy (ldp = ss rounded up to 4, padding rows zero; items
-// {f, 64-row tile, 64-column tile, 0}); k_schur_p32 then runs the tiles of
-// gemm_tile on it with 16-byte loads.  The operands reach LDS as the same
-// FP32 values as the element loads of the front give (one rounding of each
-// entry), so C is the same bits as k_gemm's on the front itself.
+// Cholesky updates of FP64 fronts with FP32 MFMA on an FP32 copy of their
+// operand: the trailing updates C -= L(:, [c0, c1)) L(:, [c0, c1))^T of the
+// rows [c1, n), c1 = min(c0 + nb, fs), and (schur) the final Schur
+// complement ABR -= ABL ABL^T (c0 = 0, c1 = fs; the reference's gemm /
+// syrk, LINAL/src/dense/chol.cxx:86-90, UHM/src/matrix/uhm/fla/chol/
+// decompose.cxx:95-96).  k_p32_pack rounds the operand to FP32 once into a
+// column-major copy (ldp = n - c1 rounded up to 4, padding rows zero;
+// items {f, 64-row tile, 64-column tile, offset / 64}); k_gemm_p32 then runs
+// the tiles of gemm_tile on it with 16-byte loads (items {f, i0, j0, offset
+// / 64}).  The operands reach LDS as the same FP32 values as the element
+// loads of the front give (one rounding of each entry), so C is the same
+// bits as k_gemm's on the front itself.
 // ------------------------------------------------------------------------
-__host__ __device__ inline long long p32_ld(int n, int fs) { return ((long long)(n - fs) + 3) & ~3LL; }
+__host__ __device__ inline long long p32_ld(int n, int c1) { return ((long long)(n - c1) + 3) & ~3LL; }
 
 template <typename TS>
 __global__ __launch_bounds__(256) void k_p32_pack(const uhmk_front* __restrict__ F,
-                                                  const uhmk_item* __restrict__ items, const TS* __restrict__ arena,
-                                                  const long long* __restrict__ off, float* __restrict__ P) {
+                                                  const uhmk_item* __restrict__ items, int c0, int nb, int schur,
+                                                  const TS* __restrict__ arena, float* __restrict__ P) {
   const uhmk_item it = items[blockIdx.x];
   const uhmk_front fr = F[it.f];
-  const int n = fr.n, fs = fr.fs, ss = n - fs;
-  const long long ldp = p32_ld(n, fs);
+  const int n = fr.n, fs = fr.fs;
+  const int kbeg = schur ? 0 : c0, kend = schur ? fs : min(c0 + nb, fs);
+  const int rows = n - kend;
+  const long long ldp = p32_ld(n, kend);
   const TS* __restrict__ M = arena + fr.off;
-  float* __restrict__ D = P + off[it.f];
+  float* __restrict__ D = P + 64LL * it.c;
   const int tid = threadIdx.x, r = tid & 63;
   const long long row = 64LL * it.a + r;
   float v[16];
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
-    const int k = 64 * it.b + (tid >> 6) + 4 * u;
-    v[u] = (row < ss && k < fs) ? (float)M[(fs + row) + (long long)k * n] : 0.0f;
+    const int k = kbeg + 64 * it.b + (tid >> 6) + 4 * u;
+    v[u] = (row < rows && k < kend) ? (float)M[(kend + row) + (long long)k * n] : 0.0f;
   }
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
-    const int k = 64 * it.b + (tid >> 6) + 4 * u;
-    if (row < ldp && k < fs) D[row + (long long)k * ldp] = v[u];
+    const int k = kbeg + 64 * it.b + (tid >> 6) + 4 * u;
+    if (row < ldp && k < kend) D[row + (long long)(k - kbeg) * ldp] = v[u];
   }
 }
 
 template <typename TS>
-__global__ __launch_bounds__(256, 2) void k_schur_p32(const uhmk_front* __restrict__ F,
-                                                   const uhmk_item* __restrict__ items, TS* __restrict__ arena,
-                                                   const long long* __restrict__ off, const float* __restrict__ P) {
+__global__ __launch_bounds__(256, 2) void k_gemm_p32(const uhmk_front* __restrict__ F,
+                                                  const uhmk_item* __restrict__ items, int c0, int nb, int schur,
+                                                  TS* __restrict__ arena, const float* __restrict__ P) {
   __shared__ __align__(16) unsigned char smem[gemm_lds_bytes<64, float, false>()];
   const uhmk_item it = items[uhmd::xcd_item(blockIdx.x, gridDim.x)];
   const uhmk_front fr = F[it.f];
-  gemm_tile<true, 64, TS, float, false, float>(fr, it, 0, 0, 1, arena, nullptr, smem, P + off[it.f],
-                                               p32_ld(fr.n, fr.fs));
+  const int kend = schur ? fr.fs : min(c0 + nb, fr.fs);
+  gemm_tile<true, 64, TS, float, false, float>(fr, it, c0, nb, schur, arena, nullptr, smem, P + 64LL * it.c,
+                                               p32_ld(fr.n, kend));
 }
 
 template <typename TS, typename TM>
@@ -1421,26 +1429,27 @@ int uhmk_gemm_update(const uhmk_front* fronts, const uhmk_item* items, int n_ite
 }
 
 
-long long uhmk_schur_p32_size(int n, int fs) {
-  if (fs <= 0 || n <= fs) return 0;
-  return p32_ld(n, fs) * fs;
+long long uhmk_p32_size(int n, int fs, int c0, int nb, int schur) {
+  const int kbeg = schur ? 0 : c0, kend = schur ? fs : std::min(c0 + nb, fs);
+  if (kend <= kbeg || n <= kend) return 0;
+  return p32_ld(n, kend) * (kend - kbeg);
 }
 
-int uhmk_schur_p32_pack(const uhmk_front* fronts, const uhmk_item* items, int n_items, const void* arena, int prec,
-                        const long long* off, float* P, hipStream_t stream) {
+int uhmk_p32_pack(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, int schur,
+                  const void* arena, int prec, float* P, hipStream_t stream) {
   if (n_items <= 0) return 0;
   if (prec != 0) return -1006;
-  hipLaunchKernelGGL(k_p32_pack<double>, dim3(n_items), dim3(256), 0, stream, fronts, items,
-                     static_cast<const double*>(arena), off, P);
+  hipLaunchKernelGGL(k_p32_pack<double>, dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb, schur,
+                     static_cast<const double*>(arena), P);
   return ret(hipGetLastError());
 }
 
-int uhmk_schur_p32(const uhmk_front* fronts, const uhmk_item* items, int n_items, void* arena, int prec,
-                   const long long* off, const float* P, hipStream_t stream) {
+int uhmk_gemm_update_p32(const uhmk_front* fronts, const uhmk_item* items, int n_items, int c0, int nb, int schur,
+                         void* arena, int prec, const float* P, hipStream_t stream) {
   if (n_items <= 0) return 0;
   if (prec != 0) return -1006;
-  hipLaunchKernelGGL(k_schur_p32<double>, dim3(n_items), dim3(256), 0, stream, fronts, items,
-                     static_cast<double*>(arena), off, P);
+  hipLaunchKernelGGL(k_gemm_p32<double>, dim3(n_items), dim3(256), 0, stream, fronts, items, c0, nb, schur,
+                     static_cast<double*>(arena), P);
   return ret(hipGetLastError());
 }
 

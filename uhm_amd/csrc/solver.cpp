@@ -9,6 +9,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
+#include <map>
 #include <numeric>
 #include <stdexcept>
 #include <unordered_map>
@@ -483,7 +485,6 @@ void FrontSolver::build_schedule(int method) {
     for (int f : levels_[L])
       if (reuse_.empty() || !reuse_[f]) flv_[L].push_back(f);
   std::vector<char> lp_front(fr_.size(), 0);  // fronts with low-precision Schur products
-  std::vector<char> p32_front(fr_.size(), 0);  // fronts with FP32 Schur products on an FP32 copy of ABL
   for (int L = 0; L < (int)levels_.size(); ++L) {
     const auto& lv = flv_[L];
     if (chol && nb == 64 && small_fused_ > 0 && (int)lv.size() >= small_fused_) {
@@ -981,26 +982,6 @@ void FrontSolver::build_schedule(int method) {
       // (eight workgroups per CU hide their load latency)
       ls.tile = cnt && !lowp ? gemm_tile_for(sss / cnt, sss / cnt, tile_threshold) : 64;
     }
-    // FP32 products on FP64 fronts (Cholesky): the operand rounded to FP32
-    // once per front, then read with 16-byte loads (element loads of FP64
-    // entries were the slow part of these products)
-    const bool p32 = !lowp && chol && schur_p32_ && prec_mode_ >= 1 && level_prec_[L] == 0 && ls.tile == 64;
-    if (p32) {
-      ls.kind = Launch::SCHUR_P32;
-      Launch pk;
-      pk.kind = Launch::P32_PACK;
-      pk.level = L;
-      pk.off = (long long)items_.size();
-      for (int f : big) {
-        const int fs = fr_[f].fs, ss = fr_[f].n - fs;
-        if (!fs || !ss) continue;
-        p32_front[f] = 1;
-        for (int a = 0; a < (ss + 63) / 64; ++a)
-          for (int b = 0; b < (fs + 63) / 64; ++b) items_.push_back(uhmk_item{f, a, b, 0});
-      }
-      pk.count = (int)(items_.size() - pk.off);
-      if (pk.count) push_main(pk);
-    }
     if (lowp) {
       // pack the operands of every front of the level (64 x 64 tiles over
       // the padded packed rows and K)
@@ -1052,6 +1033,55 @@ void FrontSolver::build_schedule(int method) {
     // whole small fronts: the Schur products of the mode (FP32 in the
     // low-precision modes, whose packing does not pay on fronts this small)
     if (l.kind == Launch::CHOL_SMALL) l.mma = prec_mode_ == 0 ? 0 : 1;
+  }
+  // Cholesky, FP32 products on FP64 fronts: every Schur and trailing-update
+  // launch of those levels reads an FP32 copy of its operand (element loads
+  // of FP64 entries were the slow part of these products), made by a pack
+  // launch just before it.  One launch's copies at a time: the launches run
+  // in order on the main stream and share one buffer.
+  long long p32_elems = 0;
+  if (chol && schur_p32_ && prec_mode_ >= 1) {
+    std::vector<Launch> out;
+    out.reserve(launches_.size() * 2);
+    for (const Launch& l0 : launches_) {
+      Launch l = l0;
+      const bool schur = l.kind == Launch::SCHUR;
+      const bool conv = (schur || l.kind == Launch::GEMM) && l.extra == 1 && l.prec == 0 && l.mma == 1 &&
+                        l.tile == 64 && l.stream == 0 && l.wait_ev < 0 && l.record_ev < 0 && l.count > 0;
+      if (!conv) {
+        out.push_back(l);
+        continue;
+      }
+      std::map<int, long long> off;  // front -> copy offset / 64
+      long long tot = 0;
+      for (long long q = l.off; q < l.off + l.count; ++q) {
+        const int f = items_[q].f;
+        if (off.count(f)) continue;
+        off[f] = tot / 64;
+        tot += (uhmk_p32_size(fr_[f].n, fr_[f].fs, l.c0, l.nb, schur) + 63) / 64 * 64;
+      }
+      if (!tot || tot / 64 > std::numeric_limits<int>::max()) {
+        out.push_back(l);
+        continue;
+      }
+      p32_elems = std::max(p32_elems, tot);
+      Launch pk = l;
+      pk.kind = Launch::P32_PACK;
+      pk.extra = schur ? 1 : 0;
+      pk.off = (long long)items_.size();
+      for (const auto& fo : off) {
+        const int f = fo.first, fs = fr_[f].fs, n = fr_[f].n;
+        const int kbeg = schur ? 0 : l.c0, kend = schur ? fs : std::min(l.c0 + l.nb, fs);
+        for (int a = 0; a < (n - kend + 63) / 64; ++a)
+          for (int b = 0; b < (kend - kbeg + 63) / 64; ++b) items_.push_back(uhmk_item{f, a, b, (int)fo.second});
+      }
+      pk.count = (int)(items_.size() - pk.off);
+      out.push_back(pk);
+      for (long long q = l.off; q < l.off + l.count; ++q) items_[q].c = (int)off[items_[q].f];
+      l.kind = schur ? Launch::SCHUR_P32 : Launch::GEMM_P32;
+      out.push_back(l);
+    }
+    launches_.swap(out);
   }
   // Algorithmic work of every launch: the flops / bytes the operation needs,
   // not what a kernel happens to execute (diagonal SYRK tiles count their
@@ -1154,8 +1184,10 @@ void FrontSolver::build_schedule(int method) {
       case Launch::P32_PACK:
         for (int q = 0; q < l.count; ++q) {
           const uhmk_item& it = items_[l.off + q];
-          const int fs = fr_[it.f].fs, ss = fr_[it.f].n - fs;
-          const double rows = std::max(0, std::min(64, ss - 64 * it.a)), ks = std::max(0, std::min(64, fs - 64 * it.b));
+          const int fs = fr_[it.f].fs, n = fr_[it.f].n;
+          const int kbeg = l.extra ? 0 : l.c0, kend = l.extra ? fs : std::min(l.c0 + l.nb, fs);
+          const double rows = std::max(0, std::min(64, n - kend - 64 * it.a));
+          const double ks = std::max(0, std::min(64, kend - kbeg - 64 * it.b));
           by += 12.0 * rows * ks;  // FP64 read, FP32 write
         }
         break;
@@ -1164,6 +1196,7 @@ void FrontSolver::build_schedule(int method) {
       case Launch::SCHUR:
       case Launch::SCHUR_LP:
       case Launch::SCHUR_P32:
+      case Launch::GEMM_P32:
         for (int q = 0; q < l.count; ++q) {
           const uhmk_item& it = items_[l.off + q];
           const int fs = fr_[it.f].fs, n = fr_[it.f].n;
@@ -1202,21 +1235,8 @@ void FrontSolver::build_schedule(int method) {
     d_lpoff_.upload(lpoff, stream_);
     d_lpamax_.reserve(std::max<size_t>(1, 2 * fr_.size()) * sizeof(unsigned));
   }
-  {
-    // FP32 copies of ABL for the SCHUR_P32 launches (offsets multiples of
-    // 64 entries: 16-byte aligned)
-    std::vector<long long> poff(fr_.size(), 0);
-    long long tot = 0;
-    for (size_t f = 0; f < fr_.size(); ++f)
-      if (p32_front[f]) {
-        poff[f] = tot;
-        tot += (uhmk_schur_p32_size(fr_[f].n, fr_[f].fs) + 63) / 64 * 64;
-      }
-    if (tot) {
-      d_p32_.reserve((size_t)tot * sizeof(float));
-      d_p32off_.upload(poff, stream_);
-    }
-  }
+  // FP32 operand copies (offsets multiples of 64 entries: 16-byte aligned)
+  if (p32_elems) d_p32_.reserve((size_t)p32_elems * sizeof(float));
   check_hip(hipStreamSynchronize(stream_), "schedule upload");
   if (factor_graph_) (void)hipGraphExecDestroy(factor_graph_);
   factor_graph_ = nullptr;
@@ -1290,10 +1310,12 @@ void FrontSolver::enqueue_factor(int method, std::vector<hipEvent_t>* pev) {
                                 d_lpamax_.as<unsigned>(), d_lp_.as<unsigned short>(), st);
         break;
       case Launch::P32_PACK:
-        rc = uhmk_schur_p32_pack(F, IT + l.off, l.count, A, l.prec, d_p32off_.as<long long>(), d_p32_.as<float>(), st);
+        rc = uhmk_p32_pack(F, IT + l.off, l.count, l.c0, l.nb, l.extra, A, l.prec, d_p32_.as<float>(), st);
         break;
       case Launch::SCHUR_P32:
-        rc = uhmk_schur_p32(F, IT + l.off, l.count, A, l.prec, d_p32off_.as<long long>(), d_p32_.as<float>(), st);
+      case Launch::GEMM_P32:
+        rc = uhmk_gemm_update_p32(F, IT + l.off, l.count, l.c0, l.nb, l.kind == Launch::SCHUR_P32 ? 1 : 0, A, l.prec,
+                                  d_p32_.as<float>(), st);
         break;
       case Launch::CHOL_GROUP:
         rc = uhmk_chol_group(F, IT + l.off, l.count, l.c0, l.nb, A, l.prec, l.mma, d_flags_.as<int>(), info, st);

@@ -55,14 +55,17 @@ struct Launch {
   // workgroup per front (front list)
   // SCHUR_LP: the Schur complement products of a level on the packed BF16 /
   // FP16 operands of LP_PACK
+  // P32_PACK: the operand of the SCHUR_P32 / GEMM_P32 launch that follows
+  // (Cholesky, FP64 fronts, FP32 products) copied to FP32 (items carry the
+  // copy's offset / 64 in c)
   enum Kind {
     ASSEMBLE, LU_PANEL, LU_ROWS, CHOL_DIAG, CHOL_TRSM, GEMM, SCHUR, LU_FRONT, GEMM_DIAG, CHOL_GROUP, LP_PACK,
-    CHOL_SMALL, SCHUR_LP, P32_PACK, SCHUR_P32
+    CHOL_SMALL, SCHUR_LP, P32_PACK, SCHUR_P32, GEMM_P32
   } kind;
   int c0 = 0, nb = 0;
   long long off = 0;  // offset into the item (or front-list) array
   int count = 0;
-  int extra = 0;      // max pivot rows (panel), transb (gemm / schur)
+  int extra = 0;      // max pivot rows (panel), transb (gemm / schur), schur (p32 pack)
   int level = 0;
   int tile = 64;      // gemm / schur tile edge
   int prec = 0;       // storage of the level's fronts: 0 FP64, 1 FP32
@@ -73,7 +76,7 @@ struct Launch {
   int wait_ev = -1;   // event the launch's stream waits for first (-1: none)
   int record_ev = -1; // event recorded on the launch's stream after it (-1: none)
 };
-constexpr int N_LAUNCH_KINDS = 15;
+constexpr int N_LAUNCH_KINDS = 16;
 
 // Size classes of fronts for the schedule's knobs: fs <= 64 (one panel),
 // fs <= 512 (a few panels), larger.
@@ -276,8 +279,8 @@ class FrontSolver {
                           // one (0); the same bits on FP64 fronts
   int lu_lookahead_ = 1;  // LU: the next panel on the lookahead stream beside the rest of the update
   int svb_small_ = 1;     // LU block solves: fronts with fs <= 32 by one wavefront per item
-  int schur_p32_ = 1;     // Cholesky, FP32 products on FP64 fronts: the Schur products on an FP32 copy
-                          // of ABL (read with 16-byte loads) instead of element loads of the front
+  int schur_p32_ = 1;     // Cholesky, FP32 products on FP64 fronts: the Schur and trailing-update products
+                          // on an FP32 copy of their operand (16-byte loads) instead of element loads
   int lu_rows_w_ = 32;    // LU row kernel: columns (ABL rows) per workgroup (measured best of 16, 32, 64)
   int solve_level_ = 1;   // Cholesky solves with inverses: one launch per level and direction (1) or
                           // one per 64-row block step (0)
@@ -347,7 +350,7 @@ class FrontSolver {
   // low-precision Schur operands (modes 2, 3): packed buffer, per-front
   // element offsets, per-front operand maxima (FP16 scales)
   DevBuf d_lp_, d_lpoff_, d_lpamax_;
-  DevBuf d_p32_, d_p32off_;  // FP32 copies of ABL for SCHUR_P32 and their offsets per front
+  DevBuf d_p32_;  // FP32 operand copies of the SCHUR_P32 / GEMM_P32 launches (one launch's at a time)
   long long lp_elems_ = 0;
   void check_solve_err();
   DevBuf d_rf_;  // refinement right-hand side (front layout)
