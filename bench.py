@@ -220,7 +220,7 @@ def run_reseeds(U, m, w, args, n):
 # against a copy of known size
 # (profiles/k_gemm_chol_q4_16cube_hbm_traffic.md): mixed path of the chol
 # workload only; other paths and workloads report null.
-GEMM_TRAFFIC = {("chol", "mixed"): 504.1e6}
+GEMM_TRAFFIC = {("chol", "mixed"): 738.0e6}
 
 
 def gemm_roofline(stats, peak, kernel, traffic=None):
