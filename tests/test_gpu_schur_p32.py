@@ -66,3 +66,42 @@ def test_schur_p32_same_bits(lib):
     m.improve(U.UHM_CHOL, 1e-12, 10)
     say("  refinement:", m.refine_history())
     assert m.backward_error() <= 1e-12
+
+
+def test_p32_graph_replays_match_eager(lib):
+    """The copy launches inside a captured factorization: every replay (with
+    eager work between them) gives the eager factor data and solution."""
+    m = U.Mesh()
+    m.build_synthetic(2, (10, 10, 9), 3, 0)
+    m.lock()
+    m.create_matrix_without_buffer(U.UHM_REAL, 1)
+    m.create_matrix_buffer()
+    m.set_random_params(seed=17, alpha=64.0, beta=0.0, scale_kind=1, p0=0.5)
+    m.random_matrix()
+    m.set_rhs()
+    m.set_precision(U.PREC_MIXED, 2)
+
+    def run():
+        assert m.chol_with_free() == 0
+        m.solve_chol()
+        out = []
+        for f in range(m.n_fronts()):
+            a = m.fetch_front(f)
+            fs = m.front_info(f)["fs"]
+            used = np.tril(np.ones(a.shape, bool))
+            for c0 in range(0, fs, 64):
+                c1 = min(c0 + 64, fs)
+                used[c0:c1, c0:c1] = True
+            out.append(a.view(np.uint64)[used].copy())
+        return out, m.fetch_x().copy()
+
+    F0, x0 = run()
+    m.set_graph(True)
+    for rep in range(3):
+        F1, x1 = run()
+        for f in range(m.n_fronts()):
+            assert np.array_equal(F0[f], F1[f]), (rep, f)
+        assert np.array_equal(x0.view(np.uint64), x1.view(np.uint64)), rep
+        m.check_chol()
+        say("  replay %d: backward error %.2e" % (rep, m.backward_error()))
+    m.set_graph(False)
