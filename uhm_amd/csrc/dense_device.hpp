@@ -80,9 +80,11 @@ __device__ __forceinline__ double rsqrt_d(double d) {
 // register index is static.  The caller stores T with zeros outside the
 // strict triangle that is used (and outside kb), so every update is an
 // unconditional FMA.
-template <int K, bool UNIT, bool TRANS, bool FULL>
+// ST: element type of s (FP32 triangles of FP32 fronts take half the LDS
+// reads; each entry is converted to FP64, exactly, before its FMA).
+template <int K, bool UNIT, bool TRANS, bool FULL, typename ST = double>
 struct TriStep {
-  __device__ __forceinline__ static void run(double (&x)[16], const double* __restrict__ s, int ld,
+  __device__ __forceinline__ static void run(double (&x)[16], const ST* __restrict__ s, int ld,
                                              const double* __restrict__ rdiag, int kb, int q) {
     if (FULL || K < kb) {
       constexpr int KQ = K >> 4, KR = K & 15;
@@ -94,15 +96,15 @@ struct TriStep {
 #pragma unroll
       for (int cr = 0; cr < 16; ++cr) {
         const int c = 16 * q + cr;
-        x[cr] = fma(-xk, TRANS ? s[K * ld + c] : s[c * ld + K], x[cr]);
+        x[cr] = fma(-xk, (double)(TRANS ? s[K * ld + c] : s[c * ld + K]), x[cr]);
       }
-      TriStep<K + 1, UNIT, TRANS, FULL>::run(x, s, ld, rdiag, kb, q);
+      TriStep<K + 1, UNIT, TRANS, FULL, ST>::run(x, s, ld, rdiag, kb, q);
     }
   }
 };
-template <bool UNIT, bool TRANS, bool FULL>
-struct TriStep<64, UNIT, TRANS, FULL> {
-  __device__ __forceinline__ static void run(double (&)[16], const double*, int, const double*, int, int) {}
+template <bool UNIT, bool TRANS, bool FULL, typename ST>
+struct TriStep<64, UNIT, TRANS, FULL, ST> {
+  __device__ __forceinline__ static void run(double (&)[16], const ST*, int, const double*, int, int) {}
 };
 
 // s must hold T(c, k) for k < c < kb and zero everywhere else in the

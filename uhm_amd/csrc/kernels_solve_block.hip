@@ -32,6 +32,7 @@
 // entries are FP64 or FP32 (the level's storage); arithmetic is FP64.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
 #include <utility>
 
 #include "dense_device.hpp"
@@ -498,8 +499,8 @@ __global__ __launch_bounds__(256) void k_front_global_all(const uhmk_front* __re
 // solves run in reversed index order (k' = 63 - k), so that TriStep's
 // ascending steps walk the block from its last row up; the padding rows
 // beyond kb then come first and carry rdiag 1 and zero entries.
-template <typename T, int MODE, bool FWD>
-__device__ __forceinline__ void stage_tri(double* __restrict__ s, double* __restrict__ rdiag,
+template <typename T, int MODE, bool FWD, typename ST>
+__device__ __forceinline__ void stage_tri(ST* __restrict__ s, double* __restrict__ rdiag,
                                           const T* __restrict__ B, long long n, int kb, int tid) {
   double v[16];
 #pragma unroll
@@ -526,10 +527,10 @@ __device__ __forceinline__ void stage_tri(double* __restrict__ s, double* __rest
       c = 63 - b;
       k = 63 - a;                  // T'(c', k') = L(63 - k', 63 - c')
     }
-    if (c > k) s[c * LD + k] = v[u];
-    else if (c < k) s[c * LD + k] = 0.0;
+    if (c > k) s[c * LD + k] = (ST)v[u];
+    else if (c < k) s[c * LD + k] = (ST)0;
     else {
-      s[c * LD + k] = 0.0;
+      s[c * LD + k] = (ST)0;
       if (!(FWD && MODE == 0)) rdiag[k] = a < kb ? 1.0 / v[u] : 1.0;
     }
   }
@@ -621,7 +622,12 @@ __global__ __launch_bounds__(256) void k_svb_step(const uhmk_front* __restrict__
           sX[i * LD + c] = acc[a][b][g];
         }
   } else {
-    stage_tri<T, MODE, FWD>(sT, rdiag, Bk, n, kb, tid);
+    // the triangle in the front's storage type (FP32 fronts: half the LDS
+    // reads of the column steps, which bound this solve; the entries are
+    // exact in either type)
+    typedef typename std::conditional<sizeof(T) == 4, float, double>::type ST;
+    ST* __restrict__ sTs = reinterpret_cast<ST*>(sT);
+    stage_tri<T, MODE, FWD>(sTs, rdiag, Bk, n, kb, tid);
     __syncthreads();
     // 16 vectors per wavefront in quad layout: lane 4 v + q holds entries
     // [16 q, 16 q + 16) of right-hand side 16 w + v (reversed backward)
@@ -635,8 +641,8 @@ __global__ __launch_bounds__(256) void k_svb_step(const uhmk_front* __restrict__
     // all 64 steps whatever kb: beyond kb the triangle holds +0, rdiag 1 and
     // x +0, so those steps leave every entry's bits as they are, and without
     // a test per step the LDS reads of later steps issue early
-    if constexpr (FWD && MODE == 0) uhmd::TriStep<0, true, false, true>::run(xv, sT, LD, rdiag, 64, q);
-    else uhmd::TriStep<0, false, false, true>::run(xv, sT, LD, rdiag, 64, q);
+    if constexpr (FWD && MODE == 0) uhmd::TriStep<0, true, false, true, ST>::run(xv, sTs, LD, rdiag, 64, q);
+    else uhmd::TriStep<0, false, false, true, ST>::run(xv, sTs, LD, rdiag, 64, q);
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const int e = 16 * q + j, k = FWD ? e : 63 - e;
