@@ -109,19 +109,20 @@ struct TriStep<64, UNIT, TRANS, FULL, ST> {
 
 // s must hold T(c, k) for k < c < kb and zero everywhere else in the
 // 64 x 64 block (the diagonal lives in rdiag).
-template <bool UNIT, bool TRANS>
-__device__ __forceinline__ void tri_solve64(double (&x)[16], const double* __restrict__ s, int ld,
+template <bool UNIT, bool TRANS, typename ST>
+__device__ __forceinline__ void tri_solve64(double (&x)[16], const ST* __restrict__ s, int ld,
                                             const double* __restrict__ rdiag, int kb, int lane) {
-  if (kb == 64) TriStep<0, UNIT, TRANS, true>::run(x, s, ld, rdiag, kb, lane & 3);
-  else TriStep<0, UNIT, TRANS, false>::run(x, s, ld, rdiag, kb, lane & 3);
+  if (kb == 64) TriStep<0, UNIT, TRANS, true, ST>::run(x, s, ld, rdiag, kb, lane & 3);
+  else TriStep<0, UNIT, TRANS, false, ST>::run(x, s, ld, rdiag, kb, lane & 3);
 }
 
 // Fill a 64 x 64 LDS block (row stride ld) with the strict lower (LOWER)
 // or strict upper triangle of the kb x kb block at A (leading dimension n),
 // zeros elsewhere, and rdiag[k] = 1 / A[k][k] (k < kb).  NT threads; every
-// thread's loads are issued before its LDS stores so they overlap.
-template <bool LOWER, int NT, typename T>
-__device__ __forceinline__ void load_tri64(double* __restrict__ s, int ld, double* __restrict__ rdiag,
+// thread's loads are issued before its LDS stores so they overlap.  ST, the
+// LDS element type, may be FP32 for an FP32 front (its entries are exact).
+template <bool LOWER, int NT, typename T, typename ST>
+__device__ __forceinline__ void load_tri64(ST* __restrict__ s, int ld, double* __restrict__ rdiag,
                                            const T* __restrict__ A, long long n, int kb, int tid) {
   constexpr int PER = 64 * 64 / NT;
   double v[PER];
@@ -133,7 +134,7 @@ __device__ __forceinline__ void load_tri64(double* __restrict__ s, int ld, doubl
 #pragma unroll
   for (int u = 0; u < PER; ++u) {
     const int idx = tid + NT * u, r = idx & 63, c = idx >> 6;
-    s[r * ld + c] = (r == c) ? 0.0 : v[u];
+    s[r * ld + c] = (r == c) ? (ST)0 : (ST)v[u];
     if (r == c && r < kb) rdiag[r] = 1.0 / v[u];
   }
 }

@@ -16,6 +16,7 @@
 //   trsm / gemm / syrk  LINAL/src/dense/{trsm,gemm,syrk}.cxx
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
 #include <utility>
 
 #include "assemble_device.hpp"
@@ -450,7 +451,10 @@ __global__ __launch_bounds__(NT) void k_lu_rows(const uhmk_front* __restrict__ F
   constexpr int CW = NT / 4;  // columns (rows for ABL items) per item: four lanes each in the solves
   __shared__ double s_X[2 * TILE][CW + 1];  // rows touched by the swaps (row stride CW + 1: the
                                             // gather's consecutive rows on different banks)
-  __shared__ double s_T[TILE * LDP];      // triangular panel block
+  // triangular panel block, in the front's storage type (FP32 fronts: half
+  // the LDS reads of the triangular solves, and a third workgroup per CU)
+  typedef typename std::conditional<sizeof(T) == 4, float, double>::type ST;
+  __shared__ ST s_T[TILE * LDP];
   __shared__ double s_rd[TILE];
   __shared__ int s_prow[2 * TILE], s_psrc[2 * TILE];
   __shared__ int s_cnt;
