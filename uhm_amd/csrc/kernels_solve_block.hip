@@ -702,8 +702,12 @@ __global__ __launch_bounds__(64) void k_svb_small(const uhmk_front* __restrict__
                                                   const uhmk_item* __restrict__ items,
                                                   const T* __restrict__ arena, double* __restrict__ x,
                                                   double* __restrict__ z, int nrhs) {
-  constexpr int KM = 32, LS = KM + 2;  // step-major rows of KM entries (+2: 16-byte aligned rows)
-  __shared__ double sT[KM * LS];
+  constexpr int KM = 32, LS = KM + 4;  // step-major rows of KM entries (+4: 16-byte aligned rows)
+  // the triangle in the front's storage type (its entries are exact there;
+  // each is converted to FP64 at its FMA): FP32 fronts read half the bytes
+  // and hold the current and next rows in half the registers
+  typedef typename std::conditional<sizeof(T) == 4, float, double>::type ST;
+  __shared__ __align__(16) ST sT[KM * LS];
   __shared__ double rdg[KM];
   const uhmk_item it = items[blockIdx.x];
   const uhmk_front fr = F[it.f];
@@ -714,7 +718,7 @@ __global__ __launch_bounds__(64) void k_svb_small(const uhmk_front* __restrict__
     const int a = idx & (KM - 1), b = idx / KM;  // block entry (a, b): row a of column b
     const bool in = a < kb && b < kb && (FWD ? a > b : a <= b);
     const double v = uhmd::ldz(B, a + (long long)b * n, in);
-    sT[b * LS + a] = (a == b) ? 0.0 : v;
+    sT[b * LS + a] = (a == b) ? (ST)0 : (ST)v;
     if (!FWD && a == b) rdg[a] = a < kb ? 1.0 / v : 1.0;
   }
   __syncthreads();
@@ -734,7 +738,7 @@ __global__ __launch_bounds__(64) void k_svb_small(const uhmk_front* __restrict__
   // together and spill.
   int km = KM;
   asm volatile("" : "+s"(km));
-  double tc[KM], tn[KM];
+  ST tc[KM], tn[KM];
 #pragma unroll
   for (int i = 0; i < KM; ++i) tn[i] = sT[(FWD ? 0 : KM - 1) * LS + i];
   svb_static_for<KM>([&](auto kc) {
@@ -750,7 +754,7 @@ __global__ __launch_bounds__(64) void k_svb_small(const uhmk_front* __restrict__
       if constexpr (!FWD) xv[k] = xv[k] * rdg[k];
       const double xk = xv[k];
 #pragma unroll
-      for (int i = 0; i < KM; ++i) xv[i] = fma(-xk, tc[i], xv[i]);
+      for (int i = 0; i < KM; ++i) xv[i] = fma(-xk, (double)tc[i], xv[i]);
     }
   });
   if (live) {
